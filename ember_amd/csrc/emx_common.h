@@ -1,0 +1,97 @@
+// Shared host/device definitions for the ember_amd HIP library.
+#pragma once
+
+#include <cstdint>
+
+namespace emx {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kMaxRounds = 8;      // reactions per lane: up to 8 * 64 = 512 reactions
+constexpr int kMaxFallRounds = 2;  // falloff reactions occupy the first rounds (<= 128)
+constexpr int kMaxVars = 64;       // nSpecies + 2 <= 64: one lane per state variable
+constexpr int kMaxChunkRounds = 4; // species-sum chunks: <= 256 chunks
+constexpr int kInterpN = 200;      // InterpKinetics default table size (src/chemistry0d.h:70-72)
+constexpr double kInterpTmin = 273.0;
+constexpr double kInterpTmax = 3500.0;
+
+constexpr double kGasConstant = 6.02214076e26 * 1.380649e-23;  // J/kmol/K
+constexpr double kSmallNumber = 1e-300;
+constexpr double kBigNumber = 1e300;
+
+// reaction kinds in device order (falloff first, then three-body, then elementary)
+enum RxnKind : int { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3, kNone = 4 };
+
+// info word of a reaction slot
+__host__ __device__ inline int info_kind(int32_t w) { return w & 7; }
+__host__ __device__ inline int info_rev(int32_t w) { return (w >> 3) & 1; }
+__host__ __device__ inline int info_group(int32_t w) { return (w >> 8) & 0xff; }   // 255 = none
+__host__ __device__ inline int info_hasT2(int32_t w) { return (w >> 4) & 1; }
+
+// packed stoichiometry: byte0 = molecule count, bytes 1..3 = species indices
+__host__ __device__ inline int st_n(uint32_t s) { return s & 0xff; }
+__host__ __device__ inline int st_k(uint32_t s, int j) { return (s >> (8 * (j + 1))) & 0xff; }
+
+// species-sum entry: slot (bits 0..15), multiplicity (16..23), side (24: 1 = product)
+__host__ __device__ inline int en_slot(uint32_t e) { return e & 0xffff; }
+__host__ __device__ inline int en_nu(uint32_t e) { return (e >> 16) & 0xff; }
+__host__ __device__ inline int en_prod(uint32_t e) { return (e >> 24) & 1; }
+
+// Device-side mechanism description: pointers into one device allocation.
+struct DevMech {
+    int K, I, nVars;       // species, reactions, K + 2
+    int nRounds;           // reaction rounds of 64 slots
+    int nFallRounds;       // rounds holding falloff reactions
+    int nGroups;           // distinct third-body efficiency sets
+    int nChunks;           // species-sum chunks
+    int nChunkRounds;
+    int interp;            // 1: InterpKinetics tables
+    double P, Pref;
+    // species [K]
+    const double* W;
+    const double* rW;
+    const double* Tmid;
+    const double* nasa;    // [K][14]: low a0..a6, high a0..a6
+    // reactions in device order [nRounds * 64]
+    const double* A;
+    const double* b;
+    const double* EaR;
+    const double* dn;
+    const double* A0;
+    const double* b0;
+    const double* EaR0;
+    const double* troeA;
+    const double* rT3;
+    const double* rT1;
+    const double* T2;
+    const uint32_t* reac;
+    const uint32_t* prod;
+    const int32_t* info;
+    const int32_t* orig;   // slot -> reaction index in the mechanism file (-1 padding)
+    // third-body groups
+    const int32_t* grp_off;   // [nGroups + 1]
+    const int32_t* grp_sp;
+    const double* grp_eff;    // efficiency - default
+    const double* grp_def;    // [nGroups]
+    // species sums
+    const int32_t* chunk_off; // [nChunks + 1]
+    const uint32_t* entry;
+    const int32_t* sp_chunk0; // [K]
+    const int32_t* sp_nchunk; // [K]
+    // interp tables [kInterpN][nRounds*64]
+    const double* kf_c;
+    const double* kf_s;
+    const double* kc_c;
+    const double* kc_s;
+};
+
+// Integrator options as seen by the kernels
+struct DevOpts {
+    double qss_epsmin, qss_epsmax, qss_dtmin, qss_dtmax;
+    int qss_itermax;
+    double qss_abstol, qss_minval;
+    int qss_stability;
+    double rtol, atol_momentum, atol_energy, atol_species, min_step;
+    int max_steps;
+};
+
+}  // namespace emx

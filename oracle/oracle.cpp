@@ -1,0 +1,1186 @@
+// CPU FP64 oracle for the Ember split-step hot path -- TEST INFRASTRUCTURE ONLY.
+//
+// Restates, in plain C++17, the algorithms of:
+//   src/qssintegrator.cpp            (QssIntegrator, CHEMEQ2-style QSS)
+//   src/sourceSystem.cpp:24-31,398-531 (SourceSystem::updateThermo, SourceSystemQSS)
+//   src/sourceSystem.cpp:104-164,268-338 (SourceSystemCVODE::f, fdJacobian)
+//   src/chemistry0d.cpp:264-436,487-672 (InterpKinetics, MultiArrheniusInterp, CanteraGas)
+//   src/diffusionSystem.cpp:10-76     (DiffusionSystem::get_A, get_k)
+//   src/integrator.cpp:37-47,133-204  (Integrator::integrateToTime, TridiagonalIntegrator::step)
+//   src/grid.cpp:43-69                (OneDimGrid::updateValues metrics)
+//   src/flameSolver.cpp:702-718,754-854 (production / diffusion stage drivers)
+// and the parts of Cantera 3.2 those call (ideal-gas Phase state setters, NasaPoly2,
+// BulkKinetics::updateROP, ThirdBodyCalc, FalloffRate/TroeRate, StoichManagerN),
+// restated from Cantera's published algorithms.  SUNDIALS CVODE is replaced by a
+// variable-order BDF (Shampine & Reichelt, "The MATLAB ODE Suite", SIAM J. Sci.
+// Comput. 18 (1997), NDF family with kappa = 0), applied with Ember's tolerances and
+// Ember's finite-difference Jacobian.
+//
+// Only the tests, the smoke check and the benchmark's CPU-baseline leg load this.
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr double kGasConstant = 6.02214076e26 * 1.380649e-23;  // J/kmol/K (Cantera ct_defs.h)
+constexpr double kSmallNumber = 1e-300;                        // Cantera SmallNumber
+constexpr double kBigNumber = 1e300;                           // Cantera BigNumber
+
+// ---------------------------------------------------------------------------
+// Mechanism tables (written by ember_amd/mechanism.py)
+// ---------------------------------------------------------------------------
+struct Rxn {
+    int type = 0, rev = 1, nr = 0, np = 0;
+    int r[3] = {-1, -1, -1}, p[3] = {-1, -1, -1};
+    double A = 0, b = 0, EaR = 0, A0 = 0, b0 = 0, EaR0 = 0;
+    double troeA = 0, T3 = 0, T1 = 0, T2 = 0;
+    int hasT2 = 0;
+    double defeff = 1.0;
+    std::vector<std::pair<int, double>> eff;
+};
+
+struct Mech {
+    int K = 0, I = 0, E = 0;
+    double pref = 101325.0;
+    std::vector<std::string> names;
+    std::vector<double> W, Tmid, low, high;  // low/high: K x 7
+    std::vector<Rxn> rxn;
+    std::vector<double> perturb;
+    std::vector<double> dn;
+    // Cantera StoichManagerN lists: grouped by molecule count (1, 2, 3)
+    struct Stoich { int rxn; int n; int k[3]; };
+    std::vector<Stoich> reac, prod, revprod;
+    std::vector<int> thirdbody;  // reactions with a third body (three-body + falloff)
+    std::vector<int> tb_slot;    // reaction -> slot in concm, or -1
+    // interp tables (InterpKinetics / MultiArrheniusInterp), frozen range
+    int nInterp = 200;
+    double TminInt = 273.0, TmaxInt = 3500.0;
+    std::vector<double> kf_const, kf_slope, kc_const, kc_slope;  // [n][I]
+    bool interp_ready = false;
+    double interp_P = -1;
+};
+
+void build_stoich(Mech& m) {
+    m.reac.clear(); m.prod.clear(); m.revprod.clear();
+    for (int n = 1; n <= 3; n++) {
+        for (int i = 0; i < m.I; i++) {
+            const Rxn& r = m.rxn[i];
+            if (r.nr == n) m.reac.push_back({i, n, {r.r[0], r.r[1], r.r[2]}});
+        }
+    }
+    for (int n = 1; n <= 3; n++) {
+        for (int i = 0; i < m.I; i++) {
+            const Rxn& r = m.rxn[i];
+            if (r.np == n) {
+                m.prod.push_back({i, n, {r.p[0], r.p[1], r.p[2]}});
+                if (r.rev) m.revprod.push_back({i, n, {r.p[0], r.p[1], r.p[2]}});
+            }
+        }
+    }
+    m.dn.assign(m.I, 0.0);
+    m.thirdbody.clear();
+    m.tb_slot.assign(m.I, -1);
+    for (int i = 0; i < m.I; i++) {
+        m.dn[i] = double(m.rxn[i].np) - double(m.rxn[i].nr);
+        if (m.rxn[i].type != 0) {
+            m.tb_slot[i] = (int)m.thirdbody.size();
+            m.thirdbody.push_back(i);
+        }
+    }
+}
+
+Mech* load_mech(const char* path) {
+    FILE* f = fopen(path, "r");
+    if (!f) return nullptr;
+    auto* m = new Mech();
+    char tag[64];
+    char buf[256];
+    bool ok = true;
+    if (fscanf(f, "%63s %255s", tag, buf) != 2 || strcmp(tag, "EMXM") != 0) ok = false;
+    while (ok && fscanf(f, "%63s", tag) == 1) {
+        if (!strcmp(tag, "END")) break;
+        if (!strcmp(tag, "name")) { ok = fscanf(f, "%255s", buf) == 1; }
+        else if (!strcmp(tag, "counts")) {
+            ok = fscanf(f, "%d %d %d", &m->K, &m->I, &m->E) == 3;
+            m->names.reserve(m->K);
+        } else if (!strcmp(tag, "pref")) { ok = fscanf(f, "%lf", &m->pref) == 1; }
+        else if (!strcmp(tag, "elements")) {
+            for (int e = 0; e < m->E && ok; e++) ok = fscanf(f, "%255s", buf) == 1;
+        } else if (!strcmp(tag, "S")) {
+            double v[1 + 3 + 14];
+            ok = fscanf(f, "%255s", buf) == 1;
+            for (int j = 0; j < 18 && ok; j++) ok = fscanf(f, "%lf", &v[j]) == 1;
+            for (int e = 0; e < m->E && ok; e++) { double c; ok = fscanf(f, "%lf", &c) == 1; }
+            m->names.push_back(buf);
+            m->W.push_back(v[0]);
+            m->Tmid.push_back(v[2]);
+            for (int j = 0; j < 7; j++) m->low.push_back(v[4 + j]);
+            for (int j = 0; j < 7; j++) m->high.push_back(v[11 + j]);
+        } else if (!strcmp(tag, "R")) {
+            Rxn r;
+            int neff = 0;
+            ok = fscanf(f, "%d %d %d %d %d %d %d %d %d %d", &r.type, &r.rev, &r.nr, &r.r[0], &r.r[1],
+                        &r.r[2], &r.np, &r.p[0], &r.p[1], &r.p[2]) == 10;
+            ok = ok && fscanf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %d %lf %d", &r.A, &r.b,
+                              &r.EaR, &r.A0, &r.b0, &r.EaR0, &r.troeA, &r.T3, &r.T1, &r.T2,
+                              &r.hasT2, &r.defeff, &neff) == 13;
+            for (int j = 0; j < neff && ok; j++) {
+                int k; double e;
+                ok = fscanf(f, "%d %lf", &k, &e) == 2;
+                r.eff.emplace_back(k, e);
+            }
+            m->rxn.push_back(r);
+        } else {
+            ok = false;
+        }
+    }
+    fclose(f);
+    if (!ok || (int)m->names.size() != m->K || (int)m->rxn.size() != m->I) {
+        delete m;
+        return nullptr;
+    }
+    m->perturb.assign(m->I, 1.0);
+    build_stoich(*m);
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// Cantera NasaPoly2 / NasaPoly1::updateProperties
+// ---------------------------------------------------------------------------
+void nasa_props(const Mech& m, int k, const double* tt, double& cp_R, double& h_RT, double& s_R) {
+    const double* c = (tt[0] <= m.Tmid[k]) ? &m.low[7 * k] : &m.high[7 * k];
+    double ct0 = c[0];
+    double ct1 = c[1] * tt[0];
+    double ct2 = c[2] * tt[1];
+    double ct3 = c[3] * tt[2];
+    double ct4 = c[4] * tt[3];
+    cp_R = ct0 + ct1 + ct2 + ct3 + ct4;
+    h_RT = ct0 + 0.5 * ct1 + 1.0 / 3.0 * ct2 + 0.25 * ct3 + 0.2 * ct4 + c[5] * tt[4];
+    s_R = ct0 * tt[5] + ct1 + 0.5 * ct2 + 1.0 / 3.0 * ct3 + 0.25 * ct4 + c[6];
+}
+
+// ---------------------------------------------------------------------------
+// Ideal gas phase + bulk kinetics (Cantera restatement; one instance per thread,
+// like Ember's per-thread CanteraGas, src/flameSolver.h:167)
+// ---------------------------------------------------------------------------
+struct Gas {
+    Mech& m;
+    int kinetics;
+    double P;
+    double T = 300.0, mmw = 0, dens = 0;
+    std::vector<double> y, ym, conc;
+    std::vector<double> cpR, hRT, sR, grt, mu0;
+    std::vector<double> kf0, rkcn, concm, rfn, ropf, ropr, ropnet;
+    double thermoT = -1.0, rateT = -1.0;
+
+    Gas(Mech& mech, int kin, double pressure)
+        : m(mech), kinetics(kin), P(pressure), y(m.K), ym(m.K), conc(m.K), cpR(m.K), hRT(m.K),
+          sR(m.K), grt(m.K), mu0(m.K), kf0(m.I), rkcn(m.I), concm(m.thirdbody.size()),
+          rfn(m.I), ropf(m.I), ropr(m.I), ropnet(m.I) {}
+
+    // Phase::setMassFractions: clip negatives, normalise
+    void setMassFractions(const double* Y) {
+        for (int k = 0; k < m.K; k++) y[k] = std::max(Y[k], 0.0);
+        double norm = 0.0;
+        for (int k = 0; k < m.K; k++) norm += y[k];
+        double s = 1.0 / norm;
+        for (int k = 0; k < m.K; k++) y[k] = y[k] * s;
+        double sum = 0.0;
+        for (int k = 0; k < m.K; k++) { ym[k] = y[k] * (1.0 / m.W[k]); sum += ym[k]; }
+        mmw = 1.0 / sum;
+    }
+    // Phase::setMassFractions_NoNorm
+    void setMassFractions_NoNorm(const double* Y) {
+        double sum = 0.0;
+        for (int k = 0; k < m.K; k++) { y[k] = Y[k]; ym[k] = y[k] * (1.0 / m.W[k]); sum += ym[k]; }
+        mmw = 1.0 / sum;
+    }
+    // ThermoPhase::setState_TP for IdealGasPhase: density from the pressure
+    void setState_TP(double Tin) {
+        T = Tin;
+        dens = P * mmw / (T * kGasConstant);
+    }
+    double pressure() const { return kGasConstant * (dens / mmw) * T; }
+    double RT() const { return T * kGasConstant; }
+
+    void updateThermo() {
+        if (thermoT == T) return;
+        double tt[6] = {T, T * T, T * T * T, T * T * T * T, 1.0 / T, std::log(T)};
+        for (int k = 0; k < m.K; k++) {
+            nasa_props(m, k, tt, cpR[k], hRT[k], sR[k]);
+            grt[k] = hRT[k] - sR[k];
+        }
+        thermoT = T;
+    }
+    double cp_mass() {
+        updateThermo();
+        double s = 0.0;
+        for (int k = 0; k < m.K; k++) s += ym[k] * cpR[k];
+        double cp_mole = kGasConstant * (mmw * s);
+        return cp_mole / mmw;
+    }
+    void enthalpies(double* hk) {
+        updateThermo();
+        double rt = RT();
+        for (int k = 0; k < m.K; k++) hk[k] = rt * hRT[k];
+    }
+
+    // reverse-rate factor 1/Kc at the current T (BulkKinetics::updateROP, first block)
+    void rkc_at(double* out) {
+        updateThermo();
+        double rt = RT();
+        double tmp = std::log(pressure() / m.pref) * rt;
+        for (int k = 0; k < m.K; k++) mu0[k] = rt * grt[k] + tmp;
+        double logStandConc = std::log(pressure() / rt);
+        double rrt = 1.0 / rt;
+        std::vector<double>& dg = rfn;  // scratch
+        std::fill(dg.begin(), dg.end(), 0.0);
+        for (const auto& s : m.revprod) {
+            if (s.n == 1) dg[s.rxn] += mu0[s.k[0]];
+            else if (s.n == 2) dg[s.rxn] += mu0[s.k[0]] + mu0[s.k[1]];
+            else dg[s.rxn] += mu0[s.k[0]] + mu0[s.k[1]] + mu0[s.k[2]];
+        }
+        for (const auto& s : m.reac) {
+            if (s.n == 1) dg[s.rxn] -= mu0[s.k[0]];
+            else if (s.n == 2) dg[s.rxn] -= (mu0[s.k[0]] + mu0[s.k[1]]);
+            else dg[s.rxn] -= (mu0[s.k[0]] + mu0[s.k[1]] + mu0[s.k[2]]);
+        }
+        for (int i = 0; i < m.I; i++) {
+            out[i] = m.rxn[i].rev ? std::min(std::exp(dg[i] * rrt - m.dn[i] * logStandConc), kBigNumber)
+                                  : 0.0;
+        }
+    }
+
+    static double arrhenius(double A, double b, double EaR, double logT, double recipT) {
+        return A * std::exp(b * logT - EaR * recipT);
+    }
+
+    // InterpKinetics / MultiArrheniusInterp tables over [TminInt, TmaxInt], frozen.
+    void build_interp() {
+        if (m.interp_ready && m.interp_P == P) return;
+        int n = m.nInterp;
+        m.kf_const.assign((size_t)n * m.I, 0.0);
+        m.kf_slope.assign((size_t)n * m.I, 0.0);
+        m.kc_const.assign((size_t)n * m.I, 0.0);
+        m.kc_slope.assign((size_t)n * m.I, 0.0);
+        double Tsave = T, dsave = dens;
+        for (int j = 0; j < n; j++) {
+            double Tn = m.TminInt + ((double)j) / (n - 1) * (m.TmaxInt - m.TminInt);
+            T = Tn;
+            dens = P * mmw / (T * kGasConstant);
+            thermoT = -1;
+            rkc_at(&m.kc_const[(size_t)j * m.I]);
+            double logT = std::log(Tn), recipT = 1.0 / Tn;
+            for (int i = 0; i < m.I; i++) {
+                const Rxn& r = m.rxn[i];
+                if (r.type <= 1) m.kf_const[(size_t)j * m.I + i] = arrhenius(r.A, r.b, r.EaR, logT, recipT);
+            }
+        }
+        double dT = (m.TmaxInt - m.TminInt) / (n - 1);
+        for (int j = 0; j < n - 1; j++) {
+            for (int i = 0; i < m.I; i++) {
+                size_t a = (size_t)j * m.I + i, b = (size_t)(j + 1) * m.I + i;
+                m.kf_slope[a] = (m.kf_const[b] - m.kf_const[a]) / dT;
+                m.kc_slope[a] = (m.kc_const[b] - m.kc_const[a]) / dT;
+            }
+        }
+        T = Tsave; dens = dsave; thermoT = -1;
+        m.interp_ready = true;
+        m.interp_P = P;
+    }
+
+    // T-dependent rate data: kf for Arrhenius-type reactions and rkcn
+    bool updateRateConstants() {
+        if (rateT == T) return true;
+        if (kinetics == 1) {
+            if (T < m.TminInt || T >= m.TmaxInt) return false;
+            size_t n = (size_t)std::floor((T - m.TminInt) / (m.TmaxInt - m.TminInt) * (m.nInterp - 1));
+            double dT = T - n * (m.TmaxInt - m.TminInt) / (m.nInterp - 1) - m.TminInt;
+            for (int i = 0; i < m.I; i++) {
+                rkcn[i] = m.kc_const[n * m.I + i] + m.kc_slope[n * m.I + i] * dT;
+                if (m.rxn[i].type <= 1)
+                    kf0[i] = m.kf_const[n * m.I + i] + m.kf_slope[n * m.I + i] * dT;
+            }
+        } else {
+            rkc_at(rkcn.data());
+            double logT = std::log(T), recipT = 1.0 / T;
+            for (int i = 0; i < m.I; i++) {
+                const Rxn& r = m.rxn[i];
+                if (r.type <= 1) kf0[i] = arrhenius(r.A, r.b, r.EaR, logT, recipT);
+            }
+        }
+        rateT = T;
+        return true;
+    }
+
+    // BulkKinetics::updateROP (composition-dependent part every call)
+    bool updateROP() {
+        if (!updateRateConstants()) return false;
+        for (int k = 0; k < m.K; k++) conc[k] = ym[k] * dens;
+        double ctot = dens / mmw;
+        // ThirdBodyCalc::update
+        for (size_t s = 0; s < m.thirdbody.size(); s++) {
+            const Rxn& r = m.rxn[m.thirdbody[s]];
+            double sum = 0.0;
+            for (const auto& e : r.eff) sum += (e.second - r.defeff) * conc[e.first];
+            concm[s] = r.defeff * ctot + sum;
+        }
+        // falloff rates (FalloffRate::evalFromStruct)
+        double logT = std::log(T), recipT = 1.0 / T;
+        for (int i = 0; i < m.I; i++) {
+            const Rxn& r = m.rxn[i];
+            if (r.type < 2) continue;
+            double work = 0.0;
+            if (r.type == 3) {
+                double rt3 = 1.0 / r.T3, rt1 = 1.0 / r.T1;
+                double Fcent = (1.0 - r.troeA) * std::exp(-T * rt3) + r.troeA * std::exp(-T * rt1);
+                if (r.T2 != 0.0) Fcent += std::exp(-r.T2 / T);
+                work = std::log10(std::max(Fcent, kSmallNumber));
+            }
+            double klow = arrhenius(r.A0, r.b0, r.EaR0, logT, recipT);
+            double khigh = arrhenius(r.A, r.b, r.EaR, logT, recipT);
+            double pr = concm[m.tb_slot[i]] * klow / (khigh + kSmallNumber);
+            double F = 1.0;
+            if (r.type == 3) {
+                double lpr = std::log10(std::max(pr, kSmallNumber));
+                double cc = -0.4 - 0.67 * work;
+                double nn = 0.75 - 1.27 * work;
+                double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
+                double lf = work / (1.0 + f1 * f1);
+                F = std::pow(10.0, lf);
+            }
+            pr *= F / (1.0 + pr);
+            kf0[i] = pr * khigh;
+        }
+        for (int i = 0; i < m.I; i++) rfn[i] = kf0[i] * m.perturb[i];
+        for (int i = 0; i < m.I; i++) ropf[i] = rfn[i];
+        for (int i = 0; i < m.I; i++)
+            if (m.rxn[i].type == 1) ropf[i] *= concm[m.tb_slot[i]];
+        for (int i = 0; i < m.I; i++) ropr[i] = ropf[i];
+        for (const auto& s : m.reac) {
+            if (s.n == 1) ropf[s.rxn] *= conc[s.k[0]];
+            else if (s.n == 2) ropf[s.rxn] *= conc[s.k[0]] * conc[s.k[1]];
+            else ropf[s.rxn] *= conc[s.k[0]] * conc[s.k[1]] * conc[s.k[2]];
+        }
+        for (int i = 0; i < m.I; i++) ropr[i] *= rkcn[i];
+        for (const auto& s : m.revprod) {
+            if (s.n == 1) ropr[s.rxn] *= conc[s.k[0]];
+            else if (s.n == 2) ropr[s.rxn] *= conc[s.k[0]] * conc[s.k[1]];
+            else ropr[s.rxn] *= conc[s.k[0]] * conc[s.k[1]] * conc[s.k[2]];
+        }
+        for (int i = 0; i < m.I; i++) ropnet[i] = ropf[i] - ropr[i];
+        return true;
+    }
+
+    static void increment(const std::vector<Mech::Stoich>& L, const double* in, double* out) {
+        for (const auto& s : L)
+            for (int j = 0; j < s.n; j++) out[s.k[j]] += in[s.rxn];
+    }
+    static void decrement(const std::vector<Mech::Stoich>& L, const double* in, double* out) {
+        for (const auto& s : L)
+            for (int j = 0; j < s.n; j++) out[s.k[j]] -= in[s.rxn];
+    }
+    bool creationRates(double* cdot) {
+        if (!updateROP()) return false;
+        std::fill(cdot, cdot + m.K, 0.0);
+        increment(m.prod, ropf.data(), cdot);
+        increment(m.reac, ropr.data(), cdot);
+        return true;
+    }
+    void destructionRates(double* ddot) {  // after creationRates / updateROP
+        std::fill(ddot, ddot + m.K, 0.0);
+        increment(m.revprod, ropr.data(), ddot);
+        increment(m.reac, ropf.data(), ddot);
+    }
+    bool netRates(double* wdot) {
+        if (!updateROP()) return false;
+        std::fill(wdot, wdot + m.K, 0.0);
+        increment(m.prod, ropnet.data(), wdot);
+        decrement(m.reac, ropnet.data(), wdot);
+        return true;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// QssIntegrator + SourceSystemQSS (src/qssintegrator.cpp, src/sourceSystem.cpp:398-531)
+// ---------------------------------------------------------------------------
+int isign(double x) { return (x > 0) ? 1 : (x < 0) ? -1 : 0; }
+
+struct SourceQSS {
+    Gas& gas;
+    const orc_options& o;
+    int K, N;
+    // integrator state
+    std::vector<double> y, q, d, rtau, rtaus, y1, ys, qs, scratch, ym1, ym2, ymin, rtaub, alpha;
+    std::vector<char> enforce;
+    double tn = 0, ts = 0, dt = 0, tstart = 0, tfd = 1.000008;
+    bool firstStep = true;
+    int gcount = 0, rcount = 0;
+    // system state
+    std::vector<double> splitConst, Yv, hk, wQ, wD;
+    double U = 0, T = 0, rho = 0, cp = 0, qLoss = 0, qDot = 0;
+    bool failed_rates = false;
+    double qint = 0, qdot_step = 0;
+
+    SourceQSS(Gas& g, const orc_options& opt)
+        : gas(g), o(opt), K(g.m.K), N(g.m.K + 2), y(N), q(N), d(N), rtau(N), rtaus(N), y1(N), ys(N),
+          qs(N), scratch(N), ym1(N), ym2(N), ymin(N), rtaub(N), alpha(N), enforce(N, 1),
+          splitConst(N), Yv(K), hk(K), wQ(K), wD(K) {
+        // SourceSystemQSS::initialize / setOptions
+        enforce[0] = 0;
+        std::fill(ymin.begin(), ymin.end(), o.qss_minval);
+        ymin[0] = -1e4;
+    }
+
+    void unroll_y(const double* yy, bool corrector) {
+        if (!corrector) T = yy[1];
+        U = yy[0];
+        for (int k = 0; k < K; k++) Yv[k] = yy[2 + k];
+    }
+
+    void odefun(double t, const double* yy, double* qq, double* dd, bool corrector) {
+        (void)t;
+        unroll_y(yy, corrector);
+        gas.setMassFractions(Yv.data());
+        gas.setState_TP(T);
+        if (!gas.creationRates(wQ.data())) failed_rates = true;
+        gas.destructionRates(wD.data());
+        if (!corrector) {
+            gas.enthalpies(hk.data());
+            rho = gas.dens;
+            cp = gas.cp_mass();
+        }
+        double s = 0.0;
+        for (int k = 0; k < K; k++) s += (wQ[k] - wD[k]) * hk[k];
+        qDot = -s;
+        double dUdtQ = splitConst[0], dUdtD = 0.0;
+        double dTdtQ = qDot / (rho * cp) + splitConst[1];
+        double dTdtD = qLoss / (rho * cp);
+        qq[0] = dUdtQ; dd[0] = dUdtD;
+        qq[1] = dTdtQ; dd[1] = dTdtD;
+        for (int k = 0; k < K; k++) {
+            qq[2 + k] = wQ[k] * gas.m.W[k] / rho + splitConst[2 + k];
+            dd[2 + k] = wD[k] * gas.m.W[k] / rho;
+        }
+    }
+
+    void setState(const double* yIn) {
+        for (int i = 0; i < N; i++) y[i] = enforce[i] ? std::max(yIn[i], ymin[i]) : yIn[i];
+        gcount = 0; rcount = 0; tstart = 0; tn = 0; firstStep = true;
+        qint = 0;
+    }
+
+    void getInitialStepSize(double tf) {
+        firstStep = false;
+        double scrtch = 1.0e-25;
+        for (int i = 0; i < N; i++) {
+            if (std::abs(y[i]) > o.qss_abstol) {
+                double absq = std::abs(q[i]);
+                double scr2 = std::abs(1 / y[i]) * isign(0.1 * o.qss_epsmin * absq - d[i]);
+                double scr1 = scr2 * d[i];
+                scrtch = std::max(std::max(scr1, -std::abs(absq - d[i]) * scr2), scrtch);
+            }
+        }
+        double sqreps = 0.5;
+        dt = std::min(sqreps / scrtch, tf);
+        dt = std::min(dt, o.qss_dtmax);
+    }
+
+    int integrateOneStep(double tf) {
+        odefun(tn + tstart, y.data(), q.data(), d.data(), false);
+        gcount += 1;
+        qdot_step = qDot;
+        if (firstStep) getInitialStepSize(tf);
+        ts = tn;
+        for (int i = 0; i < N; i++) rtau[i] = enforce[i] ? dt * d[i] / y[i] : 0.0;
+        qs = q; ys = y; rtaus = rtau;
+        const bool stab_on = o.qss_stability_check != 0;
+        while (true) {
+            for (int i = 0; i < N; i++) {
+                double r = rtau[i];
+                scratch[i] = (q[i] - d[i]) /
+                             (1.0 + r * (180 + r * (60 + r * (11 + r))) / (360 + r * (60 + r * (12 + r))));
+            }
+            double eps = 1e-10;
+            for (int iter = 0; iter < o.qss_itermax; iter++) {
+                if (stab_on) { ym2 = ym1; ym1 = y; }
+                for (int i = 0; i < N; i++) y[i] = std::max(ymin[i], ys[i] + dt * scratch[i]);
+                if (iter == 0) { tn = ts + dt; y1 = y; }
+                odefun(tn + tstart, y.data(), q.data(), d.data(), true);
+                gcount += 1;
+                eps = 1.0e-10;
+                for (int i = 0; i < N; i++) rtaub[i] = enforce[i] ? 0.5 * (rtaus[i] + dt * d[i] / y[i]) : 0.0;
+                for (int i = 0; i < N; i++) {
+                    double r = rtaub[i];
+                    alpha[i] = (180. + r * (60. + r * (11. + r))) / (360. + r * (60. + r * (12. + r)));
+                }
+                for (int i = 0; i < N; i++)
+                    scratch[i] = (qs[i] * (1.0 - alpha[i]) + q[i] * alpha[i] - ys[i] * rtaub[i] / dt) /
+                                 (1.0 + alpha[i] * rtaub[i]);
+            }
+            for (int i = 0; i < N; i++) {
+                double scr2 = ys[i] + dt * scratch[i];
+                if (enforce[i]) scr2 = std::max(scr2, ymin[i]);
+                double scr1 = std::abs(scr2 - y1[i]);
+                y[i] = std::max(scr2, ymin[i]);
+                if (std::abs(y[i]) > o.qss_abstol && 0.25 * (ys[i] + y[i]) > ymin[i]) {
+                    scr1 = scr1 / y[i];
+                    eps = std::max(.5 * (scr1 + std::min(std::abs(q[i] - d[i]) / (q[i] + d[i] + 1e-30), scr1)), eps);
+                }
+            }
+            if (stab_on) { ym2 = ym1; ym1 = y; }
+            eps /= o.qss_epsmin;
+            if (dt <= o.qss_dtmin + 1e-16 * tn) return -1;
+            double stab = 0;
+            if (stab_on && o.qss_itermax >= 3) {
+                stab = 0.01;
+                for (int i = 0; i < N; i++)
+                    if (std::abs(y[i]) > o.qss_abstol)
+                        stab = std::max(stab, std::abs(y[i] - ym1[i]) / (std::abs(ym1[i] - ym2[i]) + 1e-20 * y[i]));
+            }
+            if (eps <= o.qss_epsmax && stab <= 1.0) {
+                qint += dt * qdot_step;
+                if (tf <= tn * tfd) return 0;
+            } else {
+                tn = ts;
+            }
+            double rteps = 0.5 * (eps + 1.0);
+            rteps = 0.5 * (rteps + eps / rteps);
+            rteps = 0.5 * (rteps + eps / rteps);
+            double dto = dt;
+            dt = std::min(dt * (1.0 / rteps + 0.005), tfd * (tf - tn));
+            dt = std::min(dt, o.qss_dtmax);
+            if (stab_on) dt = std::min(dt, dto / (stab + .001));
+            if (eps > o.qss_epsmax || stab > 1) {
+                rcount += 1;
+                for (int i = 0; i < N; i++) rtaus[i] *= dt / dto;
+            } else {
+                return 0;
+            }
+        }
+    }
+
+    int integrateToTime(double tf) {
+        while (tfd * tn < tf) {
+            int ret = integrateOneStep(tf);
+            if (failed_rates) return -2;
+            if (ret) return ret;
+        }
+        return 0;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// SourceSystemCVODE semantics with a variable-order BDF
+// ---------------------------------------------------------------------------
+struct SourceBDF {
+    Gas& gas;
+    const orc_options& o;
+    int K, N;
+    std::vector<double> splitConst, Yv, hk, wdot, atol;
+    double qLoss = 0, qDot = 0;
+    int nfe = 0, nsteps = 0, njac = 0;
+    bool failed_rates = false;
+
+    SourceBDF(Gas& g, const orc_options& opt)
+        : gas(g), o(opt), K(g.m.K), N(g.m.K + 2), splitConst(N), Yv(K), hk(K), wdot(K), atol(N) {
+        atol[0] = o.atol_momentum;
+        atol[1] = o.atol_energy;
+        for (int k = 0; k < K; k++) atol[2 + k] = o.atol_species;
+    }
+
+    // SourceSystemCVODE::f (src/sourceSystem.cpp:128-164)
+    void f(const double* y, double* ydot) {
+        nfe++;
+        double T = y[1];
+        for (int k = 0; k < K; k++) Yv[k] = y[2 + k];
+        gas.setMassFractions_NoNorm(Yv.data());
+        gas.setState_TP(T);
+        if (!gas.netRates(wdot.data())) failed_rates = true;
+        gas.enthalpies(hk.data());
+        double rho = gas.dens;
+        double cp = gas.cp_mass();
+        double s = 0.0;
+        for (int k = 0; k < K; k++) s += wdot[k] * hk[k];
+        qDot = -s;
+        qDot -= qLoss;
+        ydot[0] = splitConst[0];
+        ydot[1] = qDot / (rho * cp) + splitConst[1];
+        for (int k = 0; k < K; k++) ydot[2 + k] = wdot[k] * gas.m.W[k] / rho + splitConst[2 + k];
+    }
+
+    // SourceSystemCVODE::fdJacobian (src/sourceSystem.cpp:268-293); J column-major
+    void fdJacobian(const double* y, const double* ydot, double* J) {
+        njac++;
+        std::vector<double> yp(y, y + N), yd2(N);
+        const double eps = std::sqrt(std::numeric_limits<double>::epsilon());
+        const double atl = std::numeric_limits<double>::epsilon();
+        for (int i = 0; i < N; i++) {
+            for (int k = 0; k < N; k++) yp[k] = y[k];
+            double dy = (std::abs(y[i]) > atl) ? std::abs(y[i]) * eps : std::abs(y[i]) * eps + atl;
+            yp[i] += dy;
+            f(yp.data(), yd2.data());
+            for (int k = 0; k < N; k++) J[(size_t)i * N + k] = (yd2[k] - ydot[k]) / dy;
+        }
+    }
+
+    double rms(const double* v, const double* scale) const {
+        double s = 0.0;
+        for (int i = 0; i < N; i++) { double r = v[i] / scale[i]; s += r * r; }
+        return std::sqrt(s / N);
+    }
+
+    // LU with partial pivoting of M = I - c J (row-major copy), returns false if singular
+    static bool lu_factor(int n, std::vector<double>& A, std::vector<int>& piv) {
+        for (int k = 0; k < n; k++) {
+            int p = k;
+            double best = std::abs(A[(size_t)k * n + k]);
+            for (int i = k + 1; i < n; i++) {
+                double v = std::abs(A[(size_t)i * n + k]);
+                if (v > best) { best = v; p = i; }
+            }
+            piv[k] = p;
+            if (best == 0.0) return false;
+            if (p != k)
+                for (int j = 0; j < n; j++) std::swap(A[(size_t)k * n + j], A[(size_t)p * n + j]);
+            double inv = 1.0 / A[(size_t)k * n + k];
+            for (int i = k + 1; i < n; i++) {
+                double l = A[(size_t)i * n + k] * inv;
+                A[(size_t)i * n + k] = l;
+                if (l != 0.0)
+                    for (int j = k + 1; j < n; j++) A[(size_t)i * n + j] -= l * A[(size_t)k * n + j];
+            }
+        }
+        return true;
+    }
+    static void lu_solve(int n, const std::vector<double>& A, const std::vector<int>& piv, double* b) {
+        for (int k = 0; k < n; k++) {
+            if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+        }
+        for (int i = 0; i < n; i++) {
+            double s = b[i];
+            for (int j = 0; j < i; j++) s -= A[(size_t)i * n + j] * b[j];
+            b[i] = s;
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double s = b[i];
+            for (int j = i + 1; j < n; j++) s -= A[(size_t)i * n + j] * b[j];
+            b[i] = s / A[(size_t)i * n + i];
+        }
+    }
+
+    // Shampine-Reichelt difference-array step change: D[0..q] <- (R U)^T D[0..q]
+    void change_D(std::vector<double>& D, int order, double factor) {
+        int m1 = order + 1;
+        auto R = [&](double fac, std::vector<double>& out) {
+            std::vector<double> M((size_t)m1 * m1, 0.0);
+            for (int j = 0; j < m1; j++) M[j] = 1.0;
+            for (int i = 1; i < m1; i++)
+                for (int j = 1; j < m1; j++) M[(size_t)i * m1 + j] = (i - 1 - fac * j) / (double)i;
+            out.assign((size_t)m1 * m1, 0.0);
+            for (int j = 0; j < m1; j++) {
+                double c = 1.0;
+                for (int i = 0; i < m1; i++) { c *= M[(size_t)i * m1 + j]; out[(size_t)i * m1 + j] = c; }
+            }
+        };
+        std::vector<double> Rm, Um, RU((size_t)m1 * m1, 0.0);
+        R(factor, Rm);
+        R(1.0, Um);
+        for (int i = 0; i < m1; i++)
+            for (int j = 0; j < m1; j++) {
+                double s = 0.0;
+                for (int l = 0; l < m1; l++) s += Rm[(size_t)i * m1 + l] * Um[(size_t)l * m1 + j];
+                RU[(size_t)i * m1 + j] = s;
+            }
+        std::vector<double> Dn((size_t)m1 * N, 0.0);
+        for (int j = 0; j < m1; j++)
+            for (int v = 0; v < N; v++) {
+                double s = 0.0;
+                for (int i = 0; i < m1; i++) s += RU[(size_t)i * m1 + j] * D[(size_t)i * N + v];
+                Dn[(size_t)j * N + v] = s;
+            }
+        std::copy(Dn.begin(), Dn.end(), D.begin());
+    }
+
+    // integrate y from 0 to tf; returns 0 or <0
+    int integrate(double* y, double tf, double* qint_out) {
+        const int MAXO = 5, NEWTON_MAXITER = 4;
+        const double MIN_FACTOR = 0.2, MAX_FACTOR = 10.0;
+        const double rtol = o.rtol;
+        double gamma[MAXO + 2], alpha_[MAXO + 2], errc[MAXO + 2];
+        gamma[0] = 0.0;
+        for (int k = 1; k <= MAXO + 1; k++) gamma[k] = gamma[k - 1] + 1.0 / k;
+        for (int k = 0; k <= MAXO + 1; k++) { alpha_[k] = gamma[k]; errc[k] = 1.0 / (k + 1); }
+        const double EPS = std::numeric_limits<double>::epsilon();
+        const double newton_tol = std::max(10 * EPS / rtol, std::min(0.03, std::sqrt(rtol)));
+
+        std::vector<double> f0(N), scale(N), f1(N), y1(N);
+        double t = 0.0;
+        f(y, f0.data());
+        double qprev = qDot;
+        double qint = 0.0;
+        // initial step (Hairer/Shampine select_initial_step, order 1)
+        for (int i = 0; i < N; i++) scale[i] = atol[i] + std::abs(y[i]) * rtol;
+        double d0 = rms(y, scale.data()), d1 = rms(f0.data(), scale.data());
+        double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+        h0 = std::min(h0, tf);
+        for (int i = 0; i < N; i++) y1[i] = y[i] + h0 * f0[i];
+        f(y1.data(), f1.data());
+        for (int i = 0; i < N; i++) y1[i] = f1[i] - f0[i];
+        double d2 = rms(y1.data(), scale.data()) / h0;
+        double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? std::max(1e-6, h0 * 1e-3)
+                                                   : std::pow(0.01 / std::max(d1, d2), 1.0 / 2.0);
+        double h_abs = std::min(100 * h0, h1);
+        h_abs = std::min(h_abs, tf);
+
+        int order = 1, n_equal_steps = 0;
+        std::vector<double> D((size_t)(MAXO + 3) * N, 0.0);
+        for (int i = 0; i < N; i++) { D[i] = y[i]; D[N + i] = f0[i] * h_abs; }
+        std::vector<double> J((size_t)N * N), LU((size_t)N * N);
+        std::vector<int> piv(N);
+        bool lu_valid = false;
+        fdJacobian(y, f0.data(), J.data());
+        std::vector<double> ypred(N), psi(N), ynew(N), dvec(N), dy(N), fy(N), err(N);
+
+        while (t < tf) {
+            double min_step = 10 * std::abs(std::nextafter(t, std::numeric_limits<double>::infinity()) - t);
+            min_step = std::max(min_step, o.min_step);
+            if (h_abs < min_step) { h_abs = min_step; }
+            bool accepted = false;
+            bool current_jac = false;
+            double error_norm = 0, safety = 0;
+            int n_iter = 0;
+            double t_new = 0;
+            while (!accepted) {
+                if (h_abs < min_step) return -3;
+                if (nsteps >= o.max_steps) return -4;
+                double h = h_abs;
+                t_new = t + h;
+                if (t_new - tf > 0) {
+                    t_new = tf;
+                    change_D(D, order, std::abs(t_new - t) / h_abs);
+                    n_equal_steps = 0;
+                    lu_valid = false;
+                }
+                h = t_new - t;
+                h_abs = std::abs(h);
+                for (int v = 0; v < N; v++) {
+                    double s = 0.0;
+                    for (int i = 0; i <= order; i++) s += D[(size_t)i * N + v];
+                    ypred[v] = s;
+                    scale[v] = atol[v] + rtol * std::abs(ypred[v]);
+                }
+                for (int v = 0; v < N; v++) {
+                    double s = 0.0;
+                    for (int i = 1; i <= order; i++) s += D[(size_t)i * N + v] * gamma[i];
+                    psi[v] = s / alpha_[order];
+                }
+                double c = h / alpha_[order];
+                bool converged = false;
+                while (!converged) {
+                    if (!lu_valid) {
+                        for (int i = 0; i < N; i++)
+                            for (int j = 0; j < N; j++)
+                                LU[(size_t)i * N + j] = (i == j ? 1.0 : 0.0) - c * J[(size_t)j * N + i];
+                        if (!lu_factor(N, LU, piv)) return -5;
+                        lu_valid = true;
+                    }
+                    // solve_bdf_system
+                    std::fill(dvec.begin(), dvec.end(), 0.0);
+                    ynew = ypred;
+                    double dy_norm_old = -1.0;
+                    converged = false;
+                    int k;
+                    for (k = 0; k < NEWTON_MAXITER; k++) {
+                        f(ynew.data(), fy.data());
+                        bool finite = true;
+                        for (int v = 0; v < N; v++) finite = finite && std::isfinite(fy[v]);
+                        if (!finite) break;
+                        for (int v = 0; v < N; v++) dy[v] = c * fy[v] - psi[v] - dvec[v];
+                        lu_solve(N, LU, piv, dy.data());
+                        double dy_norm = rms(dy.data(), scale.data());
+                        double rate = (dy_norm_old < 0) ? -1.0 : dy_norm / dy_norm_old;
+                        if (rate >= 0 && (rate >= 1 || std::pow(rate, NEWTON_MAXITER - k) / (1 - rate) * dy_norm > newton_tol))
+                            break;
+                        for (int v = 0; v < N; v++) { ynew[v] += dy[v]; dvec[v] += dy[v]; }
+                        if (dy_norm == 0 || (rate >= 0 && rate / (1 - rate) * dy_norm < newton_tol)) {
+                            converged = true;
+                            break;
+                        }
+                        dy_norm_old = dy_norm;
+                    }
+                    n_iter = k + 1;
+                    if (!converged) {
+                        if (current_jac) break;
+                        std::vector<double> fp(N);
+                        f(ypred.data(), fp.data());
+                        fdJacobian(ypred.data(), fp.data(), J.data());
+                        lu_valid = false;
+                        current_jac = true;
+                    }
+                }
+                if (failed_rates) return -2;
+                if (!converged) {
+                    double factor = 0.5;
+                    h_abs *= factor;
+                    change_D(D, order, factor);
+                    n_equal_steps = 0;
+                    lu_valid = false;
+                    continue;
+                }
+                safety = 0.9 * (2 * NEWTON_MAXITER + 1) / (2 * NEWTON_MAXITER + n_iter);
+                for (int v = 0; v < N; v++) scale[v] = atol[v] + rtol * std::abs(ynew[v]);
+                for (int v = 0; v < N; v++) err[v] = errc[order] * dvec[v];
+                error_norm = rms(err.data(), scale.data());
+                if (error_norm > 1) {
+                    double factor = std::max(MIN_FACTOR, safety * std::pow(error_norm, -1.0 / (order + 1)));
+                    h_abs *= factor;
+                    change_D(D, order, factor);
+                    n_equal_steps = 0;
+                } else {
+                    accepted = true;
+                }
+            }
+            nsteps++;
+            n_equal_steps++;
+            double hstep = t_new - t;
+            t = t_new;
+            for (int v = 0; v < N; v++) y[v] = ynew[v];
+            // heat release on the accepted step (trapezoid); not counted in nfe
+            {
+                std::vector<double> tmp(N);
+                f(y, tmp.data());
+                nfe--;
+                qint += 0.5 * hstep * (qprev + qDot);
+                qprev = qDot;
+            }
+            for (int v = 0; v < N; v++) {
+                D[(size_t)(order + 2) * N + v] = dvec[v] - D[(size_t)(order + 1) * N + v];
+                D[(size_t)(order + 1) * N + v] = dvec[v];
+            }
+            for (int i = order; i >= 0; i--)
+                for (int v = 0; v < N; v++) D[(size_t)i * N + v] += D[(size_t)(i + 1) * N + v];
+            if (n_equal_steps < order + 1) continue;
+            double em = std::numeric_limits<double>::infinity(), ep = std::numeric_limits<double>::infinity();
+            if (order > 1) {
+                for (int v = 0; v < N; v++) err[v] = errc[order - 1] * D[(size_t)order * N + v];
+                em = rms(err.data(), scale.data());
+            }
+            if (order < MAXO) {
+                for (int v = 0; v < N; v++) err[v] = errc[order + 1] * D[(size_t)(order + 2) * N + v];
+                ep = rms(err.data(), scale.data());
+            }
+            double norms[3] = {em, error_norm, ep};
+            double factors[3];
+            int best = 0;
+            for (int j = 0; j < 3; j++) {
+                factors[j] = std::pow(norms[j], -1.0 / (order + j));
+                if (std::isnan(factors[j])) factors[j] = 0;
+            }
+            for (int j = 1; j < 3; j++) if (factors[j] > factors[best]) best = j;
+            order += best - 1;
+            double factor = std::min(MAX_FACTOR, safety * factors[best]);
+            h_abs *= factor;
+            change_D(D, order, factor);
+            n_equal_steps = 0;
+            lu_valid = false;
+        }
+        if (qint_out) *qint_out = qint;
+        return 0;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// TridiagonalIntegrator (src/integrator.cpp)
+// ---------------------------------------------------------------------------
+struct Tridiag {
+    int N;
+    std::vector<double> a, b, c, k, lu_b, lu_c, lu_d, invDenom, y, yprev;
+    double t = 0, h = 0;
+    unsigned stepCount = 0;
+    explicit Tridiag(int n)
+        : N(n), a(n, 0.0), b(n, 0.0), c(n, 0.0), k(n, 0.0), lu_b(n), lu_c(n), lu_d(n), invDenom(n),
+          y(n), yprev(n) {}
+    void initialize(double t0, double h_) { t = t0; h = h_; stepCount = 0; }
+    void factor(double shift) {
+        for (int i = 0; i < N; i++) lu_b[i] = b[i] - shift;
+        lu_c[0] = c[0] / lu_b[0];
+        for (int i = 1; i < N; i++) {
+            invDenom[i] = 1.0 / (lu_b[i] - lu_c[i - 1] * a[i]);
+            lu_c[i] = c[i] * invDenom[i];
+        }
+    }
+    void solve() {
+        lu_d[0] = y[0] / lu_b[0];
+        for (int i = 1; i < N; i++) lu_d[i] = (y[i] - lu_d[i - 1] * a[i]) * invDenom[i];
+        y[N - 1] = lu_d[N - 1];
+        for (int i = N - 2; i >= 0; i--) y[i] = lu_d[i] - lu_c[i] * y[i + 1];
+    }
+    void step() {
+        if (stepCount == 0) {
+            yprev = y;
+            factor(1.0 / (h / 8.0));
+            for (int j = 0; j < 8; j++) {
+                for (int i = 0; i < N; i++) y[i] = -y[i] / (h / 8.0) - k[i];
+                solve();
+            }
+        } else {
+            if (stepCount == 1) factor(3.0 / (2.0 * h));
+            std::vector<double> ynm1 = y;
+            for (int i = 0; i < N; i++) y[i] = -2 * ynm1[i] / h + yprev[i] / (2 * h) - k[i];
+            yprev = ynm1;
+            solve();
+        }
+        stepCount++;
+        t += h;
+    }
+    void integrateToTime(double tEnd) {
+        int nSteps = static_cast<int>((tEnd - t) / h + 1e-5);
+        nSteps = std::max(nSteps, 1);
+        h = (tEnd - t) / nSteps;
+        for (int i = 0; i < nSteps; i++) step();
+    }
+};
+
+// DiffusionSystem::get_A / get_k with OneDimGrid metrics
+void diffusion_coefficients(int N, const double* B, const double* D, const double* sc,
+                            const double* x, int alpha, int leftBC, int rightBC, double yInf,
+                            double wallConst, Tridiag& s) {
+    std::vector<double> hh(N - 1), rphalf(N - 1), r(N), dlj(N, 0.0), c1(N, 0.0), c2(N, 0.0);
+    for (int j = 0; j < N - 1; j++) {
+        hh[j] = x[j + 1] - x[j];
+        rphalf[j] = std::pow(0.5 * (x[j] + x[j + 1]), alpha);
+    }
+    for (int j = 0; j < N; j++) r[j] = std::pow(x[j], alpha);
+    for (int j = 1; j < N - 1; j++) dlj[j] = 0.5 * (x[j + 1] - x[j - 1]);
+    for (int j = 1; j <= N - 2; j++) {
+        c1[j] = 0.5 * B[j] / (dlj[j] * r[j]);
+        c2[j] = rphalf[j] * (D[j] + D[j + 1]) / hh[j];
+    }
+    int jStart;
+    if (leftBC == 0) {
+        jStart = 1;
+    } else if (leftBC == 2) {
+        jStart = 1;
+        double c0 = B[0] * (alpha + 1) * (D[0] + D[1]) / (2 * hh[0] * hh[0]);
+        s.b[0] = -c0;
+        s.c[0] = c0;
+    } else if (leftBC == 3) {
+        jStart = 1;
+        double c0 = B[0] * (alpha + 1) / hh[0];
+        double d = 0.5 * (D[0] + D[1]);
+        s.b[0] = -c0 * (d / hh[0] + wallConst);
+        s.c[0] = d * c0 / hh[0];
+    } else {
+        jStart = 2;
+        s.b[1] = -c1[1] * c2[1];
+        s.c[1] = c1[1] * c2[1];
+    }
+    int jStop;
+    if (rightBC == 0) {
+        jStop = N - 1;
+    } else {
+        jStop = N - 2;
+        s.a[N - 2] = c1[N - 2] * c2[N - 3];
+        s.b[N - 2] = -c1[N - 2] * c2[N - 3];
+    }
+    for (int j = jStart; j < jStop; j++) {
+        s.a[j] = c1[j] * c2[j - 1];
+        s.b[j] = -c1[j] * (c2[j - 1] + c2[j]);
+        s.c[j] = c1[j] * c2[j];
+    }
+    for (int j = 0; j < N; j++) s.k[j] = sc[j];
+    if (leftBC == 3) s.k[0] += B[0] * (alpha + 1) / hh[0] * wallConst * yInf;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C API
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void orc_default_options(orc_options* o) {
+    o->integrator = 0;
+    o->kinetics = 0;
+    o->pressure = 101325.0;
+    o->qss_epsmin = 2e-2;
+    o->qss_epsmax = 10.0;
+    o->qss_dtmin = 1e-16;
+    o->qss_dtmax = 1e-6;
+    o->qss_itermax = 1;
+    o->qss_abstol = 1e-11;
+    o->qss_minval = 1e-60;
+    o->qss_stability_check = 0;
+    o->rtol = 1e-6;
+    o->atol_momentum = 1e-7;
+    o->atol_energy = 1e-8;
+    o->atol_species = 1e-13;
+    o->min_step = 1e-18;
+    o->max_steps = 1000000;
+}
+
+void* orc_mech_load(const char* path) { return load_mech(path); }
+void orc_mech_free(void* mech) { delete static_cast<Mech*>(mech); }
+int orc_mech_nspecies(void* mech) { return static_cast<Mech*>(mech)->K; }
+int orc_mech_nreactions(void* mech) { return static_cast<Mech*>(mech)->I; }
+void orc_mech_set_multipliers(void* mech, const double* m) {
+    Mech* me = static_cast<Mech*>(mech);
+    for (int i = 0; i < me->I; i++) me->perturb[i] = m ? m[i] : 1.0;
+}
+
+int orc_rates(void* mech, const orc_options* o, int n, const double* T, const double* Y, int nonorm,
+              double* wdotQ, double* wdotD, double* wdot_net, double* hk, double* rho, double* cp,
+              double* ropf, double* ropr) {
+    Mech& m = *static_cast<Mech*>(mech);
+    Gas g(m, o->kinetics, o->pressure);
+    if (o->kinetics == 1) {
+        std::vector<double> Y0(m.K, 1.0 / m.K);
+        g.setMassFractions(Y0.data());
+        g.setState_TP(300.0);
+        g.build_interp();
+    }
+    std::vector<double> tmp(m.K);
+    for (int j = 0; j < n; j++) {
+        const double* Yj = Y + (size_t)j * m.K;
+        if (nonorm) g.setMassFractions_NoNorm(Yj);
+        else g.setMassFractions(Yj);
+        g.setState_TP(T[j]);
+        if (!g.creationRates(wdotQ ? wdotQ + (size_t)j * m.K : tmp.data())) return -2;
+        if (wdotD) g.destructionRates(wdotD + (size_t)j * m.K);
+        if (wdot_net) g.netRates(wdot_net + (size_t)j * m.K);
+        if (hk) g.enthalpies(hk + (size_t)j * m.K);
+        if (rho) rho[j] = g.dens;
+        if (cp) cp[j] = g.cp_mass();
+        if (ropf) std::copy(g.ropf.begin(), g.ropf.end(), ropf + (size_t)j * m.I);
+        if (ropr) std::copy(g.ropr.begin(), g.ropr.end(), ropr + (size_t)j * m.I);
+    }
+    return 0;
+}
+
+int orc_source_integrate(void* mech, const orc_options* o, int nPoints, const double* state_in,
+                         const double* split_const, const double* qloss, double dt,
+                         double* state_out, int* status, int* stat1, int* stat2, double* qint,
+                         int nthreads) {
+    Mech& m = *static_cast<Mech*>(mech);
+    const int N = m.K + 2;
+    if (o->kinetics == 1) {
+        Gas g(m, 1, o->pressure);
+        std::vector<double> Y0(m.K, 1.0 / m.K);
+        g.setMassFractions(Y0.data());
+        g.setState_TP(300.0);
+        g.build_interp();
+    }
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        Gas gas(m, o->kinetics, o->pressure);  // per-thread gas (gases.local())
+        SourceQSS qss(gas, *o);
+        SourceBDF bdf(gas, *o);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int j = 0; j < nPoints; j++) {
+            const double* yin = state_in + (size_t)j * N;
+            double* yout = state_out + (size_t)j * N;
+            int err;
+            double qi = 0.0;
+            if (o->integrator == 0) {
+                for (int i = 0; i < N; i++) qss.splitConst[i] = split_const ? split_const[(size_t)j * N + i] : 0.0;
+                qss.qLoss = qloss ? qloss[j] : 0.0;
+                qss.failed_rates = false;
+                qss.setState(yin);
+                err = qss.integrateToTime(dt);
+                if (err >= 0) std::copy(qss.y.begin(), qss.y.end(), yout);
+                if (stat1) stat1[j] = qss.gcount;
+                if (stat2) stat2[j] = qss.rcount;
+                qi = qss.qint;
+            } else {
+                for (int i = 0; i < N; i++) bdf.splitConst[i] = split_const ? split_const[(size_t)j * N + i] : 0.0;
+                bdf.qLoss = qloss ? qloss[j] : 0.0;
+                bdf.failed_rates = false;
+                bdf.nfe = bdf.nsteps = bdf.njac = 0;
+                std::vector<double> y(yin, yin + N);
+                err = bdf.integrate(y.data(), dt, &qi);
+                if (err >= 0) std::copy(y.begin(), y.end(), yout);
+                if (stat1) stat1[j] = bdf.nsteps;
+                if (stat2) stat2[j] = bdf.nfe;
+            }
+            if (err < 0) std::copy(yin, yin + N, yout);  // keep the old state
+            if (status) status[j] = err;
+            if (qint) qint[j] = qi;
+        }
+    }
+    return 0;
+}
+
+int orc_tridiag_integrate(int N, const double* a, const double* b, const double* c, const double* k,
+                          double t0, double h, double t1, int mode, int nsteps, double* y,
+                          double* t_out, double* h_out) {
+    Tridiag s(N);
+    for (int i = 0; i < N; i++) { s.a[i] = a[i]; s.b[i] = b[i]; s.c[i] = c[i]; s.k[i] = k[i]; s.y[i] = y[i]; }
+    s.initialize(t0, h);
+    if (mode == 0) {
+        for (int i = 0; i < nsteps; i++) s.step();
+    } else {
+        s.integrateToTime(t1);
+    }
+    for (int i = 0; i < N; i++) y[i] = s.y[i];
+    if (t_out) *t_out = s.t;
+    if (h_out) *h_out = s.h;
+    return 0;
+}
+
+int orc_diffusion_integrate(int nVars, int N, const double* B, int nB, const int* B_row,
+                            const double* D, const double* split_const, const double* x, int alpha,
+                            int leftBC, int rightBC, double yInf, double wallConst, double multiplier,
+                            double dt_global, const double* h_per_var, double t0, double t1,
+                            double* state, int nthreads) {
+    (void)nB;
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+#endif
+    for (int v = 0; v < nVars; v++) {
+        const double* Bv = B + (size_t)(B_row ? B_row[v] : v) * N;
+        std::vector<double> Dv(N), sc(N);
+        for (int j = 0; j < N; j++) {
+            Dv[j] = D[(size_t)j * nVars + v];
+            sc[j] = split_const ? split_const[(size_t)j * nVars + v] : 0.0;
+        }
+        Tridiag s(N);
+        diffusion_coefficients(N, Bv, Dv.data(), sc.data(), x, alpha, leftBC, rightBC, yInf, wallConst, s);
+        double h;
+        if (h_per_var) {
+            h = h_per_var[v];
+        } else {
+            // FlameSolver::setDiffusionSolverState: min over j = 1..N-2 of dlj^2/(B D)
+            double mn = std::numeric_limits<double>::infinity();
+            for (int j = 1; j <= N - 2; j++) {
+                double dl = 0.5 * (x[j + 1] - x[j - 1]);
+                mn = std::min(mn, dl * dl / (Bv[j] * Dv[j]));
+            }
+            h = std::min(multiplier * mn, dt_global);
+        }
+        for (int j = 0; j < N; j++) s.y[j] = state[(size_t)j * nVars + v];
+        s.initialize(t0, h);
+        s.integrateToTime(t1);
+        for (int j = 0; j < N; j++) state[(size_t)j * nVars + v] = s.y[j];
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+/*
+ * CPU FP64 oracle for the Ember split-step hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain C++17 restatement of Ember's chemistry substep (QssIntegrator,
+ * SourceSystemQSS, SourceSystemCVODE semantics), of the Cantera kinetics and
+ * thermo it calls, and of the diffusion substep (DiffusionSystem,
+ * TridiagonalIntegrator).  Only the tests, the smoke check and the CPU-baseline
+ * leg of the benchmark may load it; the product library never links it.
+ *
+ * Layouts follow Ember: a state is an nVars x nPoints column-major matrix
+ * (one point's U, T, Y_0..Y_{K-1} contiguous), src/splitSolver.h:73.
+ */
+#ifndef EMBER_ORACLE_H
+#define EMBER_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_options {
+    int integrator;        /* 0 = QSS (SourceSystemQSS), 1 = BDF (SourceSystemCVODE semantics) */
+    int kinetics;          /* 0 = standard (Cantera BulkKinetics), 1 = interp (InterpKinetics) */
+    double pressure;       /* Pa */
+    /* QSS tolerances, python/ember/input.py:748-776 */
+    double qss_epsmin, qss_epsmax, qss_dtmin, qss_dtmax;
+    int qss_itermax;
+    double qss_abstol, qss_minval;
+    int qss_stability_check;
+    /* BDF tolerances, python/ember/input.py:729-746 */
+    double rtol, atol_momentum, atol_energy, atol_species, min_step;
+    int max_steps;
+} orc_options;
+
+void orc_default_options(orc_options* o);
+
+void* orc_mech_load(const char* path);
+void orc_mech_free(void* mech);
+int orc_mech_nspecies(void* mech);
+int orc_mech_nreactions(void* mech);
+/* Per-reaction rate multipliers (Cantera setMultiplier / m_perturb). */
+void orc_mech_set_multipliers(void* mech, const double* m);
+
+/* Rates at n states.  nonorm=0: setState_TPY (clip + normalise);
+ * nonorm=1: setMassFractions_NoNorm + setState_TP.  Any output may be NULL. */
+int orc_rates(void* mech, const orc_options* o, int n, const double* T, const double* Y,
+              int nonorm, double* wdotQ, double* wdotD, double* wdot_net, double* hk,
+              double* rho, double* cp, double* ropf, double* ropr);
+
+/* One production stage for nPoints points (FlameSolver::integrateProductionTerms).
+ * status: 0 ok, <0 failure (the point keeps its input state).
+ * stat1/stat2: QSS gcount/rcount, BDF nsteps/nfe.  qint: integral of qdot over the
+ * stage on accepted steps (nullable).  nthreads <= 0: OpenMP default. */
+int orc_source_integrate(void* mech, const orc_options* o, int nPoints,
+                         const double* state_in, const double* split_const,
+                         const double* qloss, double dt, double* state_out,
+                         int* status, int* stat1, int* stat2, double* qint, int nthreads);
+
+/* TridiagonalIntegrator with a constant (a, b, c, k) system
+ * (test/tridiagonalIntegrator.cpp ConstantTridiagonalODE).
+ * mode 0: take nsteps single steps of size h from t0.
+ * mode 1: integrateToTime(t1) from t0 with initial step h.  y is in/out. */
+int orc_tridiag_integrate(int N, const double* a, const double* b, const double* c,
+                          const double* k, double t0, double h, double t1, int mode,
+                          int nsteps, double* y, double* t_out, double* h_out);
+
+/* Diffusion stage for nVars variables on one grid (FlameSolver::integrateDiffusionTerms).
+ * B: nB x N rows (row-contiguous) with B_row[nVars]; D, split_const, state: nVars x N
+ * column-major.  h_per_var NULL -> Ember's rule min(mult*min dlj^2/(B D), dtGlobal).
+ * BC codes: 0 FixedValue, 1 ZeroGradient, 2 ControlVolume, 3 WallFlux. */
+int orc_diffusion_integrate(int nVars, int N, const double* B, int nB, const int* B_row,
+                            const double* D, const double* split_const, const double* x,
+                            int alpha, int leftBC, int rightBC, double yInf, double wallConst,
+                            double multiplier, double dt_global, const double* h_per_var,
+                            double t0, double t1, double* state, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,155 @@
+"""ctypes wrapper for the CPU FP64 oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py``; never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Options(C.Structure):
+    _fields_ = [("integrator", C.c_int), ("kinetics", C.c_int), ("pressure", C.c_double),
+                ("qss_epsmin", C.c_double), ("qss_epsmax", C.c_double), ("qss_dtmin", C.c_double),
+                ("qss_dtmax", C.c_double), ("qss_itermax", C.c_int), ("qss_abstol", C.c_double),
+                ("qss_minval", C.c_double), ("qss_stability_check", C.c_int),
+                ("rtol", C.c_double), ("atol_momentum", C.c_double), ("atol_energy", C.c_double),
+                ("atol_species", C.c_double), ("min_step", C.c_double), ("max_steps", C.c_int)]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_default_options.argtypes = [C.POINTER(Options)]
+        L.orc_mech_load.restype = C.c_void_p
+        L.orc_mech_load.argtypes = [C.c_char_p]
+        L.orc_mech_free.argtypes = [C.c_void_p]
+        L.orc_mech_nspecies.argtypes = [C.c_void_p]
+        L.orc_mech_nreactions.argtypes = [C.c_void_p]
+        L.orc_mech_set_multipliers.argtypes = [C.c_void_p, _dp]
+        L.orc_rates.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, C.c_int,
+                                _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_source_integrate.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, _dp,
+                                           C.c_double, _dp, _ip, _ip, _ip, _dp, C.c_int]
+        L.orc_tridiag_integrate.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                            C.c_double, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_diffusion_integrate.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _ip, _dp, _dp, _dp,
+                                              C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                              C.c_double, C.c_double, _dp, C.c_double, C.c_double,
+                                              _dp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp if a.dtype == np.float64 else _ip)
+
+
+def options(**kw) -> Options:
+    o = Options()
+    lib().orc_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Mech:
+    def __init__(self, table_path: str):
+        self.h = lib().orc_mech_load(table_path.encode())
+        if not self.h:
+            raise RuntimeError(f"oracle could not load {table_path}")
+        self.K = lib().orc_mech_nspecies(self.h)
+        self.I = lib().orc_mech_nreactions(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mech_free(self.h)
+            self.h = None
+
+    def set_multipliers(self, m):
+        m = None if m is None else np.ascontiguousarray(m, dtype=np.float64)
+        lib().orc_mech_set_multipliers(self.h, _p(m))
+
+    def rates(self, T, Y, opts=None, nonorm=False):
+        opts = opts or options()
+        T = np.ascontiguousarray(np.atleast_1d(T), dtype=np.float64)
+        Y = np.ascontiguousarray(np.atleast_2d(Y), dtype=np.float64)
+        n = T.shape[0]
+        out = {k: np.zeros((n, self.K)) for k in ("wdotQ", "wdotD", "wdot", "hk")}
+        out["rho"] = np.zeros(n)
+        out["cp"] = np.zeros(n)
+        out["ropf"] = np.zeros((n, self.I))
+        out["ropr"] = np.zeros((n, self.I))
+        rc = lib().orc_rates(self.h, C.byref(opts), n, _p(T), _p(Y), int(nonorm), _p(out["wdotQ"]),
+                             _p(out["wdotD"]), _p(out["wdot"]), _p(out["hk"]), _p(out["rho"]),
+                             _p(out["cp"]), _p(out["ropf"]), _p(out["ropr"]))
+        if rc:
+            raise RuntimeError(f"orc_rates failed: {rc}")
+        return out
+
+    def integrate(self, state, dt, opts=None, split_const=None, qloss=None, nthreads=0):
+        """One production stage; state is (nPoints, nVars) (Ember column-major)."""
+        opts = opts or options()
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        n = state.shape[0]
+        out = np.empty_like(state)
+        status = np.zeros(n, dtype=np.int32)
+        s1 = np.zeros(n, dtype=np.int32)
+        s2 = np.zeros(n, dtype=np.int32)
+        qint = np.zeros(n)
+        sc = None if split_const is None else np.ascontiguousarray(split_const, dtype=np.float64)
+        ql = None if qloss is None else np.ascontiguousarray(qloss, dtype=np.float64)
+        lib().orc_source_integrate(self.h, C.byref(opts), n, _p(state), _p(sc), _p(ql), dt, _p(out),
+                                   _p(status), _p(s1), _p(s2), _p(qint), nthreads)
+        return out, status, s1, s2, qint
+
+
+def tridiag_integrate(a, b, c, k, y0, t0, h, t1=0.0, mode=1, nsteps=0):
+    a, b, c, k = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, k))
+    y = np.array(y0, dtype=np.float64)
+    t = np.zeros(1)
+    hh = np.zeros(1)
+    lib().orc_tridiag_integrate(len(y), _p(a), _p(b), _p(c), _p(k), t0, h, t1, mode, nsteps, _p(y),
+                                _p(t), _p(hh))
+    return y, float(t[0]), float(hh[0])
+
+
+def diffusion_integrate(B, B_row, D, split_const, x, state, t0, t1, alpha=0, leftBC=0, rightBC=0,
+                        yInf=0.0, wallConst=0.0, multiplier=10.0, dt_global=2e-5, h_per_var=None,
+                        nthreads=0):
+    """B: (nB, N); D, split_const, state: (N, nVars) (Ember column-major). Returns new state."""
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    st = np.array(state, dtype=np.float64, order="C")
+    N, nVars = st.shape
+    Br = np.ascontiguousarray(B_row, dtype=np.int32)
+    sc = None if split_const is None else np.ascontiguousarray(split_const, dtype=np.float64)
+    hv = None if h_per_var is None else np.ascontiguousarray(h_per_var, dtype=np.float64)
+    lib().orc_diffusion_integrate(nVars, N, _p(B), B.shape[0], _p(Br), _p(D), _p(sc), _p(x), alpha,
+                                  leftBC, rightBC, yInf, wallConst, multiplier, dt_global, _p(hv),
+                                  t0, t1, _p(st), nthreads)
+    return st
