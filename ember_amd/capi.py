@@ -1,0 +1,237 @@
+"""ctypes binding of the ember_amd C ABI (include/ember_amd.h).
+
+The binding loads the in-tree ``libember_amd.so``.  There is no fallback: if the
+library is missing or cannot find a GPU, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import mechanism
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libember_amd.so")
+
+EMX_QSS, EMX_BDF = 0, 1
+EMX_KIN_STANDARD, EMX_KIN_INTERP = 0, 1
+BC_FIXED_VALUE, BC_ZERO_GRADIENT, BC_CONTROL_VOLUME, BC_WALL_FLUX = 0, 1, 2, 3
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+_sz = C.c_size_t
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("chemistry_integrator", C.c_int32), ("kinetics_model", C.c_int32),
+        ("qss_epsmin", C.c_double), ("qss_epsmax", C.c_double), ("qss_dtmin", C.c_double),
+        ("qss_dtmax", C.c_double), ("qss_iteration_count", C.c_int32),
+        ("qss_abstol", C.c_double), ("qss_minval", C.c_double),
+        ("qss_stability_check", C.c_int32),
+        ("integrator_rel_tol", C.c_double), ("integrator_momentum_abs_tol", C.c_double),
+        ("integrator_energy_abs_tol", C.c_double), ("integrator_species_abs_tol", C.c_double),
+        ("integrator_min_timestep", C.c_double), ("integrator_max_steps", C.c_int32),
+        ("diffusion_timestep_multiplier", C.c_double), ("global_timestep", C.c_double),
+    ]
+
+
+class Grid(C.Structure):
+    _fields_ = [("x", _dp), ("alpha", C.c_int32), ("left_bc", C.c_int32),
+                ("right_bc", C.c_int32), ("y_inf", C.c_double), ("wall_const", C.c_double)]
+
+
+# (name, restype, argtypes) for every entry point of include/ember_amd.h
+SIGNATURES = [
+    ("emx_default_options", None, [C.POINTER(Options)]),
+    ("emx_create", C.c_int, [C.c_char_p, C.c_double, C.POINTER(Options), C.POINTER(_vp)]),
+    ("emx_destroy", None, [_vp]),
+    ("emx_last_error", C.c_char_p, []),
+    ("emx_num_species", C.c_int, [_vp]),
+    ("emx_num_reactions", C.c_int, [_vp]),
+    ("emx_set_options", C.c_int, [_vp, C.POINTER(Options)]),
+    ("emx_set_rate_multipliers", C.c_int, [_vp, _dp]),
+    ("emx_set_stream", C.c_int, [_vp, _vp]),
+    ("emx_set_launch", C.c_int, [_vp, C.c_int, C.c_int]),
+    ("emx_source_integrate", C.c_int,
+     [_vp, _sz, _sz, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _ip, _ip, _ip]),
+    ("emx_source_integrate_device", C.c_int,
+     [_vp, _sz, _sz, _vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp]),
+    ("emx_rates", C.c_int, [_vp, _sz, _dp, _dp, C.c_int, _dp, _dp, _dp, _ip]),
+    ("emx_diffusion_integrate", C.c_int,
+     [_vp, _sz, _sz, _dp, _sz, _ip, _dp, _dp, C.POINTER(Grid), _dp, C.c_double, C.c_double, _dp]),
+    ("emx_diffusion_integrate_device", C.c_int,
+     [_vp, _sz, _sz, _vp, _sz, _vp, _vp, _vp, C.POINTER(Grid), _vp, C.c_double, C.c_double, _vp]),
+    ("emx_tridiagonal_integrate", C.c_int,
+     [_vp, _sz, _sz, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, _dp]),
+]
+
+_lib = None
+
+
+class EmberAmdError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libember_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EmberAmdError(f"{LIB_PATH} is missing; build it with `python -m ember_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise EmberAmdError(f"ember_amd error {rc}: {lib().emx_last_error().decode()}")
+
+
+def _p(a, t=_dp):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().emx_default_options(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
+
+
+def _tptr(t):
+    """Device pointer of a torch tensor (None passes through)."""
+    if t is None:
+        return None
+    if not t.is_cuda or not t.is_contiguous():
+        raise EmberAmdError("device entry points need contiguous GPU tensors")
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One emx_context: mechanism tables on the current GPU plus options."""
+
+    def __init__(self, mech: str = "gri30", pressure: float = mechanism.ONE_ATM,
+                 options: Optional[Options] = None, **kw):
+        self.table = mechanism.table_path(mech)
+        self.options = options or default_options(**kw)
+        h = _vp()
+        _check(lib().emx_create(self.table.encode(), pressure, C.byref(self.options), C.byref(h)))
+        self.h = h
+        self.n_species = lib().emx_num_species(h)
+        self.n_reactions = lib().emx_num_reactions(h)
+        self.n_vars = self.n_species + 2
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().emx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_options(self, **kw):
+        for k, v in kw.items():
+            setattr(self.options, k, v)
+        _check(lib().emx_set_options(self.h, C.byref(self.options)))
+
+    def set_rate_multipliers(self, m):
+        m = None if m is None else np.ascontiguousarray(m, dtype=np.float64)
+        _check(lib().emx_set_rate_multipliers(self.h, _p(m)))
+
+    def set_stream(self, stream_handle: int):
+        _check(lib().emx_set_stream(self.h, C.c_void_p(stream_handle)))
+
+    def set_launch(self, waves_per_block: int, blocks_per_cu: int):
+        _check(lib().emx_set_launch(self.h, waves_per_block, blocks_per_cu))
+
+    # ---- chemistry
+    def source_integrate(self, state, dt, split_const=None, qloss=None, t_start=0.0):
+        """Host arrays: state (nPoints, nVars).  Returns (state_out, status, stat1, stat2)."""
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        n = st.shape[0]
+        if st.shape[1] != self.n_vars:
+            raise ValueError("state must be (nPoints, nSpecies + 2)")
+        out = np.empty_like(st)
+        status = np.zeros(n, np.int32)
+        s1 = np.zeros(n, np.int32)
+        s2 = np.zeros(n, np.int32)
+        sc = None if split_const is None else np.ascontiguousarray(split_const, dtype=np.float64)
+        ql = None if qloss is None else np.ascontiguousarray(qloss, dtype=np.float64)
+        _check(lib().emx_source_integrate(self.h, n, self.n_vars, _p(st), _p(sc), _p(ql), t_start, dt,
+                                          _p(out), _p(status, _ip), _p(s1, _ip), _p(s2, _ip)))
+        return out, status, s1, s2
+
+    def source_integrate_device(self, state_in, state_out, dt, split_const=None, qloss=None,
+                                status=None, stat1=None, stat2=None, qint=None, t_start=0.0):
+        """torch CUDA tensors, asynchronous on the context stream."""
+        n = state_in.shape[0]
+        _check(lib().emx_source_integrate_device(
+            self.h, n, self.n_vars, _tptr(state_in), _tptr(split_const), _tptr(qloss), t_start, dt,
+            _tptr(state_out), _tptr(status), _tptr(stat1), _tptr(stat2), _tptr(qint)))
+
+    def rates(self, T, Y, nonorm=False):
+        T = np.ascontiguousarray(np.atleast_1d(T), dtype=np.float64)
+        Y = np.ascontiguousarray(np.atleast_2d(Y), dtype=np.float64)
+        n = T.shape[0]
+        wQ = np.zeros((n, self.n_species))
+        wD = np.zeros_like(wQ)
+        wN = np.zeros_like(wQ)
+        status = np.zeros(n, np.int32)
+        _check(lib().emx_rates(self.h, n, _p(T), _p(Y), int(nonorm), _p(wQ), _p(wD), _p(wN),
+                               _p(status, _ip)))
+        return wQ, wD, wN, status
+
+    # ---- diffusion
+    def diffusion_integrate(self, state, B, D, x, t0, t1, B_row=None, split_const=None,
+                            alpha=0, left_bc=0, right_bc=0, y_inf=0.0, wall_const=0.0,
+                            h_per_var=None):
+        """state, D, split_const: (nPoints, nVars); B: (nB, nPoints).  Returns new state."""
+        st = np.array(state, dtype=np.float64, order="C")
+        N, nV = st.shape
+        B = np.ascontiguousarray(np.atleast_2d(B), dtype=np.float64)
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        Br = None if B_row is None else np.ascontiguousarray(B_row, dtype=np.int32)
+        sc = None if split_const is None else np.ascontiguousarray(split_const, dtype=np.float64)
+        hv = None if h_per_var is None else np.ascontiguousarray(h_per_var, dtype=np.float64)
+        g = Grid(_p(x), alpha, left_bc, right_bc, y_inf, wall_const)
+        _check(lib().emx_diffusion_integrate(self.h, nV, N, _p(B), B.shape[0], _p(Br, _ip), _p(D),
+                                             _p(sc), C.byref(g), _p(hv), t0, t1, _p(st)))
+        return st
+
+    def diffusion_integrate_device(self, state, B, D, x, t0, t1, B_row=None, split_const=None,
+                                   alpha=0, left_bc=0, right_bc=0, y_inf=0.0, wall_const=0.0,
+                                   h_per_var=None):
+        """torch CUDA tensors; state is updated in place."""
+        N, nV = state.shape
+        g = Grid(C.cast(C.c_void_p(x.data_ptr()), _dp), alpha, left_bc, right_bc, y_inf, wall_const)
+        _check(lib().emx_diffusion_integrate_device(
+            self.h, nV, N, _tptr(B), B.shape[0], _tptr(B_row), _tptr(D), _tptr(split_const),
+            C.byref(g), _tptr(h_per_var), t0, t1, _tptr(state)))
+
+    def tridiagonal_integrate(self, a, b, c, k, y, h, t0, t1):
+        """Constant systems, (nSys, N) arrays.  Returns y after integrateToTime(t1)."""
+        a, b, c, k = (np.ascontiguousarray(np.atleast_2d(v), dtype=np.float64) for v in (a, b, c, k))
+        yy = np.array(np.atleast_2d(y), dtype=np.float64, order="C")
+        nS, N = yy.shape
+        h = np.ascontiguousarray(np.broadcast_to(np.asarray(h, dtype=np.float64), (nS,)))
+        _check(lib().emx_tridiagonal_integrate(self.h, nS, N, _p(a), _p(b), _p(c), _p(k), _p(h), t0,
+                                               t1, _p(yy)))
+        return yy

@@ -1,0 +1,556 @@
+// C-ABI of the ember_amd library (declared in include/ember_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ember_amd.h"
+#include "emx_common.h"
+#include "kernels.h"
+#include "mech_host.h"
+
+using namespace emx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(-10, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+// growable device buffer
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace
+
+struct emx_context {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    double P = 101325.0;
+    emx_options opts{};
+    DevOpts dopts{};
+    HostMech hm;
+    DevLayout lay;
+    DevMech dm{};
+    void* blob = nullptr;
+    double* perturb = nullptr;  // device, slot order
+    unsigned int* queue = nullptr;
+    int waves_per_block = 4;
+    int blocks_per_cu = 0;  // 0: as many as LDS allows
+    DevBuf b_in, b_out, b_sc, b_ql, b_status, b_s1, b_s2;
+    DevBuf d_B, d_Brow, d_D, d_sc, d_x, d_y, d_h, d_ns, d_work, d_hmin;
+    DevBuf t_a, t_b, t_c, t_k;
+};
+
+namespace {
+
+DevOpts to_dev(const emx_options& o) {
+    DevOpts d;
+    d.qss_epsmin = o.qss_epsmin;
+    d.qss_epsmax = o.qss_epsmax;
+    d.qss_dtmin = o.qss_dtmin;
+    d.qss_dtmax = o.qss_dtmax;
+    d.qss_itermax = o.qss_iteration_count;
+    d.qss_abstol = o.qss_abstol;
+    d.qss_minval = o.qss_minval;
+    d.qss_stability = o.qss_stability_check;
+    d.rtol = o.integrator_rel_tol;
+    d.atol_momentum = o.integrator_momentum_abs_tol;
+    d.atol_energy = o.integrator_energy_abs_tol;
+    d.atol_species = o.integrator_species_abs_tol;
+    d.min_step = o.integrator_min_timestep;
+    d.max_steps = o.integrator_max_steps;
+    return d;
+}
+
+// pack host arrays into one device allocation (256-byte aligned pieces)
+struct Packer {
+    std::vector<char> bytes;
+    std::vector<std::pair<size_t, const void**>> fix;
+    template <class T>
+    void add(const std::vector<T>& v, const T** dst) {
+        size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + std::max<size_t>(v.size(), 1) * sizeof(T), 0);
+        if (!v.empty()) memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+        fix.emplace_back(off, reinterpret_cast<const void**>(dst));
+    }
+};
+
+int upload_mech(emx_context* c) {
+    const HostMech& m = c->hm;
+    const DevLayout& L = c->lay;
+    DevMech& d = c->dm;
+    d.K = m.K;
+    d.I = m.I;
+    d.nVars = m.K + 2;
+    d.nRounds = L.nRounds;
+    d.nFallRounds = L.nFallRounds;
+    d.nGroups = L.nGroups;
+    d.nChunks = L.nChunks;
+    d.nChunkRounds = L.nChunkRounds;
+    d.interp = c->opts.kinetics_model == EMX_KIN_INTERP ? 1 : 0;
+    d.P = c->P;
+    d.Pref = m.Pref;
+    std::vector<double> kf_c, kf_s, kc_c, kc_s;
+    if (d.interp) build_interp_tables(m, L, c->P, kf_c, kf_s, kc_c, kc_s);
+    Packer pk;
+    pk.add(m.W, &d.W);
+    pk.add(L.rW, &d.rW);
+    pk.add(m.Tmid, &d.Tmid);
+    pk.add(m.nasa, &d.nasa);
+    pk.add(L.A, &d.A);
+    pk.add(L.b, &d.b);
+    pk.add(L.EaR, &d.EaR);
+    pk.add(L.dn, &d.dn);
+    pk.add(L.A0, &d.A0);
+    pk.add(L.b0, &d.b0);
+    pk.add(L.EaR0, &d.EaR0);
+    pk.add(L.troeA, &d.troeA);
+    pk.add(L.rT3, &d.rT3);
+    pk.add(L.rT1, &d.rT1);
+    pk.add(L.T2, &d.T2);
+    pk.add(L.reac, &d.reac);
+    pk.add(L.prod, &d.prod);
+    pk.add(L.info, &d.info);
+    pk.add(L.orig, &d.orig);
+    pk.add(L.grp_off, &d.grp_off);
+    pk.add(L.grp_sp, &d.grp_sp);
+    pk.add(L.grp_eff, &d.grp_eff);
+    pk.add(L.grp_def, &d.grp_def);
+    pk.add(L.chunk_off, &d.chunk_off);
+    pk.add(L.entry, &d.entry);
+    pk.add(L.sp_chunk0, &d.sp_chunk0);
+    pk.add(L.sp_nchunk, &d.sp_nchunk);
+    pk.add(kf_c, &d.kf_c);
+    pk.add(kf_s, &d.kf_s);
+    pk.add(kc_c, &d.kc_c);
+    pk.add(kc_s, &d.kc_s);
+    if (c->blob) (void)hipFree(c->blob);
+    c->blob = nullptr;
+    HIP_TRY(hipMalloc(&c->blob, pk.bytes.size()));
+    HIP_TRY(hipMemcpy(c->blob, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice));
+    for (auto& f : pk.fix) *f.second = static_cast<char*>(c->blob) + f.first;
+    return 0;
+}
+
+int check_ctx(emx_context* c) {
+    if (!c) return fail(-1, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    return 0;
+}
+
+int grid_blocks(emx_context* c, int nPoints) {
+    const int wpb = c->waves_per_block;
+    size_t lds_per_block = (size_t)wpb * (3 * 64 + 2 * c->dm.nRounds * 64 + 2 * c->dm.nChunkRounds * 64) * 8;
+    int per_cu = c->blocks_per_cu;
+    if (per_cu <= 0) {
+        per_cu = (int)std::max<size_t>(1, (160 * 1024) / std::max<size_t>(lds_per_block, 1));
+        per_cu = std::min(per_cu, 32 / wpb);
+    }
+    long blocks = (long)per_cu * c->num_cu;
+    long needed = (nPoints + wpb - 1) / wpb;
+    return (int)std::max<long>(1, std::min(blocks, needed));
+}
+
+}  // namespace
+
+extern "C" {
+
+void emx_default_options(emx_options* o) {
+    o->chemistry_integrator = EMX_QSS;
+    o->kinetics_model = EMX_KIN_STANDARD;
+    o->qss_epsmin = 2e-2;
+    o->qss_epsmax = 10.0;
+    o->qss_dtmin = 1e-16;
+    o->qss_dtmax = 1e-6;
+    o->qss_iteration_count = 1;
+    o->qss_abstol = 1e-11;
+    o->qss_minval = 1e-60;
+    o->qss_stability_check = 0;
+    o->integrator_rel_tol = 1e-6;
+    o->integrator_momentum_abs_tol = 1e-7;
+    o->integrator_energy_abs_tol = 1e-8;
+    o->integrator_species_abs_tol = 1e-13;
+    o->integrator_min_timestep = 1e-18;
+    o->integrator_max_steps = 1000000;
+    o->diffusion_timestep_multiplier = 10.0;
+    o->global_timestep = 2e-5;
+}
+
+const char* emx_last_error(void) { return g_err.c_str(); }
+
+int emx_create(const char* path, double pressure, const emx_options* opts, emx_context** out) {
+    if (!out) return fail(-1, "out is NULL");
+    *out = nullptr;
+    auto* c = new emx_context();
+    if (opts) c->opts = *opts; else emx_default_options(&c->opts);
+    c->dopts = to_dev(c->opts);
+    c->P = pressure;
+    std::string err = read_table(path ? path : "", c->hm);
+    if (err.empty()) err = build_layout(c->hm, c->lay);
+    if (!err.empty()) {
+        delete c;
+        return fail(-2, err);
+    }
+    hipError_t e = hipGetDevice(&c->device);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(-10, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    int rc = upload_mech(c);
+    if (rc) { delete c; return rc; }
+    std::vector<double> ones(c->lay.nRounds * 64, 1.0);
+    if (hipMalloc(&c->perturb, ones.size() * sizeof(double)) != hipSuccess ||
+        hipMemcpy(c->perturb, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&c->queue, 256) != hipSuccess) {
+        emx_destroy(c);
+        return fail(-10, "device allocation failed");
+    }
+    *out = c;
+    return 0;
+}
+
+void emx_destroy(emx_context* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->blob) (void)hipFree(c->blob);
+    if (c->perturb) (void)hipFree(c->perturb);
+    if (c->queue) (void)hipFree(c->queue);
+    for (DevBuf* b : {&c->b_in, &c->b_out, &c->b_sc, &c->b_ql, &c->b_status, &c->b_s1, &c->b_s2,
+                      &c->d_B, &c->d_Brow, &c->d_D, &c->d_sc, &c->d_x, &c->d_y, &c->d_h, &c->d_ns,
+                      &c->d_work, &c->d_hmin, &c->t_a, &c->t_b, &c->t_c, &c->t_k})
+        b->release();
+    delete c;
+}
+
+int emx_num_species(const emx_context* c) { return c ? c->hm.K : -1; }
+int emx_num_reactions(const emx_context* c) { return c ? c->hm.I : -1; }
+
+int emx_set_options(emx_context* c, const emx_options* o) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!o) return fail(-1, "options is NULL");
+    bool interp_changed = o->kinetics_model != c->opts.kinetics_model;
+    c->opts = *o;
+    c->dopts = to_dev(*o);
+    if (interp_changed) return upload_mech(c);
+    return 0;
+}
+
+int emx_set_rate_multipliers(emx_context* c, const double* m) {
+    if (int rc = check_ctx(c)) return rc;
+    std::vector<double> p(c->lay.nRounds * 64, 1.0);
+    if (m)
+        for (int i = 0; i < c->hm.I; i++) p[c->lay.slot_of_rxn[i]] = m[i];
+    HIP_TRY(hipMemcpy(c->perturb, p.data(), p.size() * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int emx_set_stream(emx_context* c, void* stream) {
+    if (!c) return fail(-1, "null context");
+    c->stream = static_cast<hipStream_t>(stream);
+    return 0;
+}
+
+int emx_set_launch(emx_context* c, int waves_per_block, int blocks_per_cu) {
+    if (!c) return fail(-1, "null context");
+    if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu < 0)
+        return fail(-1, "invalid launch shape");
+    c->waves_per_block = waves_per_block ? waves_per_block : 4;
+    c->blocks_per_cu = blocks_per_cu;
+    return 0;
+}
+
+int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, const double* state_in,
+                                const double* split_const, const double* qloss, double t_stage_start,
+                                double dt, double* state_out, int32_t* status, int32_t* stat1,
+                                int32_t* stat2, double* qint) {
+    (void)t_stage_start;
+    if (int rc = check_ctx(c)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (nPoints == 0) return 0;
+    if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
+    if (!state_in || !state_out) return fail(-1, "state pointers are NULL");
+    if (c->opts.chemistry_integrator == EMX_BDF) {
+        int rc = launch_bdf_stage(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
+                                  (int)nPoints, nullptr, c->queue, state_out, status, stat1, stat2,
+                                  qint, grid_blocks(c, (int)nPoints), c->waves_per_block, c->stream);
+        if (rc) return fail(-10, std::string("bdf launch: ") + hipGetErrorString((hipError_t)rc));
+        return 0;
+    }
+    int rc = launch_qss_stage(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
+                              (int)nPoints, nullptr, c->queue, state_out, status, stat1, stat2, qint,
+                              grid_blocks(c, (int)nPoints), c->waves_per_block, c->stream);
+    if (rc) return fail(-10, std::string("qss launch: ") + hipGetErrorString((hipError_t)rc));
+    return 0;
+}
+
+int emx_source_integrate(emx_context* c, size_t nPoints, size_t nVars, const double* state_in,
+                         const double* split_const, const double* qloss, double t_stage_start,
+                         double dt, double* state_out, int32_t* status, int32_t* stat1,
+                         int32_t* stat2) {
+    if (int rc = check_ctx(c)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (nPoints == 0) return 0;
+    const size_t bytes = nPoints * nVars * sizeof(double);
+    HIP_TRY(c->b_in.ensure(bytes));
+    HIP_TRY(c->b_out.ensure(bytes));
+    HIP_TRY(c->b_status.ensure(nPoints * sizeof(int32_t)));
+    HIP_TRY(c->b_s1.ensure(nPoints * sizeof(int32_t)));
+    HIP_TRY(c->b_s2.ensure(nPoints * sizeof(int32_t)));
+    hipStream_t st = c->stream;
+    HIP_TRY(hipMemcpyAsync(c->b_in.p, state_in, bytes, hipMemcpyHostToDevice, st));
+    const double* dsc = nullptr;
+    const double* dql = nullptr;
+    if (split_const) {
+        HIP_TRY(c->b_sc.ensure(bytes));
+        HIP_TRY(hipMemcpyAsync(c->b_sc.p, split_const, bytes, hipMemcpyHostToDevice, st));
+        dsc = c->b_sc.as<double>();
+    }
+    if (qloss) {
+        HIP_TRY(c->b_ql.ensure(nPoints * sizeof(double)));
+        HIP_TRY(hipMemcpyAsync(c->b_ql.p, qloss, nPoints * sizeof(double), hipMemcpyHostToDevice, st));
+        dql = c->b_ql.as<double>();
+    }
+    int rc = emx_source_integrate_device(c, nPoints, nVars, c->b_in.as<double>(), dsc, dql,
+                                         t_stage_start, dt, c->b_out.as<double>(),
+                                         c->b_status.as<int32_t>(), c->b_s1.as<int32_t>(),
+                                         c->b_s2.as<int32_t>(), nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(state_out, c->b_out.p, bytes, hipMemcpyDeviceToHost, st));
+    if (status) HIP_TRY(hipMemcpyAsync(status, c->b_status.p, nPoints * 4, hipMemcpyDeviceToHost, st));
+    if (stat1) HIP_TRY(hipMemcpyAsync(stat1, c->b_s1.p, nPoints * 4, hipMemcpyDeviceToHost, st));
+    if (stat2) HIP_TRY(hipMemcpyAsync(stat2, c->b_s2.p, nPoints * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int emx_rates(emx_context* c, size_t n, const double* T, const double* Y, int nonorm, double* wQ,
+              double* wD, double* wnet, int32_t* status) {
+    if (int rc = check_ctx(c)) return rc;
+    if (n == 0) return 0;
+    const int K = c->hm.K;
+    DevBuf dT, dY, dQ, dD, dN, dS;
+    auto cleanup = [&]() { dT.release(); dY.release(); dQ.release(); dD.release(); dN.release(); dS.release(); };
+    size_t by = n * K * sizeof(double);
+    if (dT.ensure(n * 8) || dY.ensure(by) || dQ.ensure(by) || dD.ensure(by) || dN.ensure(by) ||
+        dS.ensure(n * 4)) {
+        cleanup();
+        return fail(-10, "device allocation failed");
+    }
+    hipStream_t st = c->stream;
+    int rc = 0;
+    if (hipMemcpyAsync(dT.p, T, n * 8, hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(dY.p, Y, by, hipMemcpyHostToDevice, st)) rc = fail(-10, "copy failed");
+    if (!rc) {
+        int e = launch_rates(c->dm, dT.as<double>(), dY.as<double>(), nonorm, c->perturb, (int)n,
+                             dQ.as<double>(), dD.as<double>(), dN.as<double>(), dS.as<int>(), st);
+        if (e) rc = fail(-10, std::string("rates launch: ") + hipGetErrorString((hipError_t)e));
+    }
+    if (!rc && ((wQ && hipMemcpyAsync(wQ, dQ.p, by, hipMemcpyDeviceToHost, st)) ||
+                (wD && hipMemcpyAsync(wD, dD.p, by, hipMemcpyDeviceToHost, st)) ||
+                (wnet && hipMemcpyAsync(wnet, dN.p, by, hipMemcpyDeviceToHost, st)) ||
+                (status && hipMemcpyAsync(status, dS.p, n * 4, hipMemcpyDeviceToHost, st)) ||
+                hipStreamSynchronize(st)))
+        rc = fail(-10, "copy back failed");
+    cleanup();
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// diffusion
+// ---------------------------------------------------------------------------
+static int diffusion_run(emx_context* c, const DiffArgs& a, const double* h_host_or_null,
+                         const double* h_dev_or_null, double t0, double t1, double* y_dev) {
+    hipStream_t st = c->stream;
+    const int nV = a.nV, N = a.N;
+    HIP_TRY(c->d_work.ensure(diffusion_work_doubles(nV, N) * sizeof(double)));
+    DiffWork w;
+    diffusion_carve(c->d_work.as<double>(), nV, N, w);
+    std::vector<double> h(nV);
+    if (h_host_or_null) {
+        for (int v = 0; v < nV; v++) h[v] = h_host_or_null[v];
+    } else if (h_dev_or_null) {
+        HIP_TRY(hipMemcpyAsync(h.data(), h_dev_or_null, nV * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    } else {
+        HIP_TRY(c->d_hmin.ensure(nV * sizeof(double)));
+        int e = launch_diffusion_hmin(a, w, c->d_hmin.as<double>(), st);
+        if (e) return fail(-10, "hmin launch failed");
+        HIP_TRY(hipMemcpyAsync(h.data(), c->d_hmin.p, nV * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        // FlameSolver::setDiffusionSolverState (src/flameSolver.cpp:709-711)
+        for (int v = 0; v < nV; v++)
+            h[v] = std::min(c->opts.diffusion_timestep_multiplier * h[v], c->opts.global_timestep);
+    }
+    // Integrator::integrateToTime: equal steps (src/integrator.cpp:41-43)
+    std::vector<int> ns(nV);
+    int maxs = 1;
+    for (int v = 0; v < nV; v++) {
+        int n = static_cast<int>((t1 - t0) / h[v] + 1e-5);
+        n = std::max(n, 1);
+        h[v] = (t1 - t0) / n;
+        ns[v] = n;
+        maxs = std::max(maxs, n);
+    }
+    HIP_TRY(c->d_h.ensure(nV * sizeof(double)));
+    HIP_TRY(c->d_ns.ensure(nV * sizeof(int)));
+    HIP_TRY(hipMemcpyAsync(c->d_h.p, h.data(), nV * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_ns.p, ns.data(), nV * 4, hipMemcpyHostToDevice, st));
+    const int nsolve = 8 + (maxs - 1);
+    for (int q = 0; q < nsolve; q++) {
+        int e = launch_diffusion_solve(a, y_dev, c->d_h.as<double>(), c->d_ns.as<int>(), q, w, st);
+        if (e) return fail(-10, std::string("diffusion launch: ") + hipGetErrorString((hipError_t)e));
+    }
+    // keep host vectors alive until the copies above have completed
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+static int check_diff(emx_context* c, size_t nVars, size_t nPoints, const emx_grid* g) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!g || !g->x) return fail(-1, "grid is NULL");
+    if (nPoints < 4) return fail(-1, "diffusion needs at least 4 points");
+    if (nVars < 1 || nVars > 64) return fail(-1, "nVars must be in [1, 64]");
+    if (g->alpha < 0 || g->alpha > 1) return fail(-1, "alpha must be 0 or 1");
+    if (g->left_bc < 0 || g->left_bc > 3 || g->right_bc < 0 || g->right_bc > 1)
+        return fail(-1, "invalid boundary condition");
+    return 0;
+}
+
+int emx_diffusion_integrate_device(emx_context* c, size_t nVars, size_t nPoints, const double* B,
+                                   size_t nB, const int32_t* B_row, const double* D,
+                                   const double* split_const, const emx_grid* g,
+                                   const double* h_per_var, double t0, double t1, double* state) {
+    if (int rc = check_diff(c, nVars, nPoints, g)) return rc;
+    if (!B_row && nB != nVars) return fail(-1, "B_row is NULL but nB != nVars");
+    DiffArgs a{B, B_row, D, split_const, g->x, (int)nPoints, (int)nVars, g->alpha, g->left_bc,
+               g->right_bc, g->y_inf, g->wall_const};
+    return diffusion_run(c, a, nullptr, h_per_var, t0, t1, state);
+}
+
+int emx_diffusion_integrate(emx_context* c, size_t nVars, size_t nPoints, const double* B, size_t nB,
+                            const int32_t* B_row, const double* D, const double* split_const,
+                            const emx_grid* g, const double* h_per_var, double t0, double t1,
+                            double* state) {
+    if (int rc = check_diff(c, nVars, nPoints, g)) return rc;
+    if (!B_row && nB != nVars) return fail(-1, "B_row is NULL but nB != nVars");
+    if (B_row)
+        for (size_t v = 0; v < nVars; v++)
+            if (B_row[v] < 0 || (size_t)B_row[v] >= nB) return fail(-1, "B_row entry out of range");
+    hipStream_t st = c->stream;
+    const size_t NV = nVars * nPoints * sizeof(double);
+    HIP_TRY(c->d_B.ensure(nB * nPoints * sizeof(double)));
+    HIP_TRY(c->d_D.ensure(NV));
+    HIP_TRY(c->d_y.ensure(NV));
+    HIP_TRY(c->d_x.ensure(nPoints * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(c->d_B.p, B, nB * nPoints * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_D.p, D, NV, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_y.p, state, NV, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_x.p, g->x, nPoints * 8, hipMemcpyHostToDevice, st));
+    const int32_t* brow = nullptr;
+    if (B_row) {
+        HIP_TRY(c->d_Brow.ensure(nVars * 4));
+        HIP_TRY(hipMemcpyAsync(c->d_Brow.p, B_row, nVars * 4, hipMemcpyHostToDevice, st));
+        brow = c->d_Brow.as<int32_t>();
+    }
+    const double* sc = nullptr;
+    if (split_const) {
+        HIP_TRY(c->d_sc.ensure(NV));
+        HIP_TRY(hipMemcpyAsync(c->d_sc.p, split_const, NV, hipMemcpyHostToDevice, st));
+        sc = c->d_sc.as<double>();
+    }
+    DiffArgs a{c->d_B.as<double>(), brow, c->d_D.as<double>(), sc, c->d_x.as<double>(),
+               (int)nPoints, (int)nVars, g->alpha, g->left_bc, g->right_bc, g->y_inf, g->wall_const};
+    int rc = diffusion_run(c, a, h_per_var, nullptr, t0, t1, c->d_y.as<double>());
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(state, c->d_y.p, NV, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int emx_tridiagonal_integrate(emx_context* c, size_t nSys, size_t N, const double* a, const double* b,
+                              const double* cc, const double* k, const double* h_in, double t0,
+                              double t1, double* y) {
+    if (int rc = check_ctx(c)) return rc;
+    if (N < 2 || nSys < 1 || nSys > 64) return fail(-1, "need N >= 2 and 1 <= nSys <= 64");
+    if (!h_in) return fail(-1, "h is NULL");
+    hipStream_t st = c->stream;
+    const size_t bytes = nSys * N * sizeof(double);
+    HIP_TRY(c->t_a.ensure(bytes));
+    HIP_TRY(c->t_b.ensure(bytes));
+    HIP_TRY(c->t_c.ensure(bytes));
+    HIP_TRY(c->t_k.ensure(bytes));
+    HIP_TRY(c->d_y.ensure(bytes));
+    HIP_TRY(hipMemcpyAsync(c->t_a.p, a, bytes, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->t_b.p, b, bytes, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->t_c.p, cc, bytes, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->t_k.p, k, bytes, hipMemcpyHostToDevice, st));
+    // y: system-major on the host, point-major on the device
+    std::vector<double> yt(nSys * N);
+    for (size_t s = 0; s < nSys; s++)
+        for (size_t j = 0; j < N; j++) yt[j * nSys + s] = y[s * N + j];
+    HIP_TRY(hipMemcpyAsync(c->d_y.p, yt.data(), bytes, hipMemcpyHostToDevice, st));
+    HIP_TRY(c->d_work.ensure(diffusion_work_doubles((int)nSys, (int)N) * sizeof(double)));
+    DiffWork w;
+    diffusion_carve(c->d_work.as<double>(), (int)nSys, (int)N, w);
+    std::vector<double> h(nSys);
+    std::vector<int> ns(nSys);
+    int maxs = 1;
+    for (size_t s = 0; s < nSys; s++) {
+        int n = std::max(static_cast<int>((t1 - t0) / h_in[s] + 1e-5), 1);
+        h[s] = (t1 - t0) / n;
+        ns[s] = n;
+        maxs = std::max(maxs, n);
+    }
+    HIP_TRY(c->d_h.ensure(nSys * 8));
+    HIP_TRY(c->d_ns.ensure(nSys * 4));
+    HIP_TRY(hipMemcpyAsync(c->d_h.p, h.data(), nSys * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_ns.p, ns.data(), nSys * 4, hipMemcpyHostToDevice, st));
+    for (int q = 0; q < 8 + maxs - 1; q++) {
+        int e = launch_explicit_solve(c->t_a.as<double>(), c->t_b.as<double>(), c->t_c.as<double>(),
+                                      c->t_k.as<double>(), (int)nSys, (int)N, c->d_y.as<double>(),
+                                      c->d_h.as<double>(), c->d_ns.as<int>(), q, w, st);
+        if (e) return fail(-10, "tridiagonal launch failed");
+    }
+    HIP_TRY(hipMemcpyAsync(yt.data(), c->d_y.p, bytes, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (size_t s = 0; s < nSys; s++)
+        for (size_t j = 0; j < N; j++) y[s * N + j] = yt[j * nSys + s];
+    return 0;
+}
+
+}  // extern "C"

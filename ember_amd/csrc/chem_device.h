@@ -1,0 +1,312 @@
+// Wavefront-cooperative gas kinetics and thermo for one grid point.
+//
+// One wavefront holds one point.  Lane v holds state variable v (0 = U,
+// 1 = T, 2 + k = Y_k, Ember's kMomentum/kEnergy/kSpecies rows,
+// src/readConfig.h:7-9).  For the reaction work, lane l owns reaction slots
+// l, l + 64, ... ("rounds"); the device slot order groups falloff, then
+// three-body, then elementary reactions so that each round runs one code path.
+//
+// The arithmetic restates Cantera 3.2 as called by Ember's CanteraGas
+// (src/chemistry0d.cpp:487-672): Phase::setMassFractions /
+// setMassFractions_NoNorm, IdealGasPhase density and standard chemical
+// potentials, NasaPoly2, BulkKinetics::updateROP (third bodies, falloff
+// Lindemann/Troe, reverse rates through 1/Kc capped at BigNumber), and
+// Kinetics::getCreationRates / getDestructionRates / getNetProductionRates.
+// Ember's InterpKinetics (src/chemistry0d.cpp:271-436) is the INTERP=true
+// variant: kf of Arrhenius/three-body reactions and 1/Kc come from linear
+// tables over [273, 3500] K.
+#pragma once
+
+#include "emx_common.h"
+#include "wave.h"
+
+namespace emx {
+
+// per-wave LDS scratch (doubles)
+struct WaveLds {
+    double* C;      // [64] concentrations
+    double* mu;     // [64] standard chemical potentials
+    double* Mg;     // [64] third-body concentrations per efficiency group
+    double* ropf;   // [nRounds*64]
+    double* ropr;   // [nRounds*64]
+    double* partq;  // [nChunkRounds*64]
+    double* partd;  // [nChunkRounds*64]
+};
+
+__host__ __device__ inline int wave_lds_doubles(int nRounds, int nChunkRounds) {
+    return 3 * 64 + 2 * nRounds * 64 + 2 * nChunkRounds * 64;
+}
+
+__device__ inline WaveLds carve_lds(double* base, int nRounds, int nChunkRounds) {
+    WaveLds w;
+    w.C = base;
+    w.mu = base + 64;
+    w.Mg = base + 128;
+    w.ropf = base + 192;
+    w.ropr = w.ropf + nRounds * 64;
+    w.partq = w.ropr + nRounds * 64;
+    w.partd = w.partq + nChunkRounds * 64;
+    return w;
+}
+
+// T-dependent reaction data held in registers (one set per owned slot).
+// NR = compile-time bound on reaction rounds (>= DevMech::nRounds).
+template <int NR>
+struct RateRegs {
+    static constexpr int NF = NR < kMaxFallRounds ? NR : kMaxFallRounds;
+    double kf[NR];
+    double rkc[NR];
+    double klow[NF];
+    double khigh[NF];
+    double work[NF];  // log10(Fcent) for Troe
+    double T;         // temperature these were computed at (-1: none)
+};
+
+// Composition state produced by set_state (uniform values are identical in all lanes)
+struct GasState {
+    double T, mmw, dens, ctot;
+    double ym;  // this lane's Y_k / W_k (species lanes), 0 otherwise
+};
+
+__device__ __forceinline__ bool is_species_lane(int lane, const DevMech& m) {
+    return lane >= 2 && lane < m.nVars;
+}
+
+// Phase::setMassFractions (+ setState_TP): clip negatives, normalise.
+// nonorm: Phase::setMassFractions_NoNorm.  y = this lane's state variable.
+__device__ inline GasState set_state(const DevMech& m, int lane, double y, double rW, double T,
+                                     bool nonorm, const WaveLds& L) {
+    GasState g;
+    g.T = T;
+    const bool sp = is_species_lane(lane, m);
+    double yk = sp ? y : 0.0;
+    if (!nonorm) {
+        double yc = fmax(yk, 0.0);
+        double norm = wave_sum(yc);
+        yk = yc * (1.0 / norm);
+    }
+    g.ym = sp ? yk * rW : 0.0;
+    double sum = wave_sum(g.ym);
+    g.mmw = 1.0 / sum;
+    g.dens = m.P * g.mmw / (T * kGasConstant);
+    g.ctot = g.dens / g.mmw;
+    if (sp) L.C[lane - 2] = g.ym * g.dens;
+    return g;
+}
+
+// NasaPoly1::updateProperties for this lane's species
+__device__ inline void nasa(const DevMech& m, int k, double T, double logT, double& cpR,
+                            double& hRT, double& sR) {
+    const double* c = m.nasa + 14 * k + ((T <= m.Tmid[k]) ? 0 : 7);
+    double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+    double ct0 = c[0];
+    double ct1 = c[1] * T;
+    double ct2 = c[2] * T2;
+    double ct3 = c[3] * T3;
+    double ct4 = c[4] * T4;
+    cpR = ct0 + ct1 + ct2 + ct3 + ct4;
+    hRT = ct0 + 0.5 * ct1 + 1.0 / 3.0 * ct2 + 0.25 * ct3 + 0.2 * ct4 + c[5] * (1.0 / T);
+    sR = ct0 * logT + ct1 + 0.5 * ct2 + 1.0 / 3.0 * ct3 + 0.25 * ct4 + c[6];
+}
+
+__device__ __forceinline__ double arrhenius(double A, double b, double EaR, double logT,
+                                            double recipT) {
+    return A * exp(b * logT - EaR * recipT);
+}
+
+__device__ __forceinline__ double conc_product(uint32_t st, const double* C) {
+    int n = st_n(st);
+    if (n == 1) return C[st_k(st, 0)];
+    if (n == 2) return C[st_k(st, 0)] * C[st_k(st, 1)];
+    return C[st_k(st, 0)] * C[st_k(st, 1)] * C[st_k(st, 2)];
+}
+
+__device__ __forceinline__ double mu_sum(uint32_t st, const double* mu) {
+    int n = st_n(st);
+    if (n == 1) return mu[st_k(st, 0)];
+    if (n == 2) return mu[st_k(st, 0)] + mu[st_k(st, 1)];
+    return mu[st_k(st, 0)] + mu[st_k(st, 1)] + mu[st_k(st, 2)];
+}
+
+// Thermo for this lane's species plus the T-dependent rate data, cached on T.
+// Returns false when T leaves the interpolation table range (INTERP only).
+// Outputs hRT/cpR for the lane's species.
+template <int NR, bool INTERP>
+__device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, RateRegs<NR>& R,
+                                const WaveLds& L, double& hRT, double& cpR) {
+    const double T = g.T;
+    const double logT = log(T);
+    const double recipT = 1.0 / T;
+    const double RT = T * kGasConstant;
+    const bool sp = is_species_lane(lane, m);
+    double sR = 0.0;
+    hRT = 0.0;
+    cpR = 0.0;
+    if (sp) nasa(m, lane - 2, T, logT, cpR, hRT, sR);
+    if (R.T == T) return true;
+    if (INTERP) {
+        if (!(T >= kInterpTmin && T < kInterpTmax)) return false;
+        size_t n = (size_t)floor((T - kInterpTmin) / (kInterpTmax - kInterpTmin) * (kInterpN - 1));
+        double dT = T - n * (kInterpTmax - kInterpTmin) / (kInterpN - 1) - kInterpTmin;
+        const int stride = m.nRounds * 64;
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            if (j < m.nRounds) {
+                int s = j * 64 + lane;
+                size_t o = n * stride + s;
+                R.rkc[j] = m.kc_c[o] + m.kc_s[o] * dT;
+                R.kf[j] = m.kf_c[o] + m.kf_s[o] * dT;
+            }
+        }
+    } else {
+        // IdealGasPhase::getStandardChemPotentials; pressure() = R * molarDensity * T
+        const double pressure = kGasConstant * (g.dens / g.mmw) * T;
+        const double tmp = log(pressure / m.Pref) * RT;
+        if (sp) L.mu[lane - 2] = RT * (hRT - sR) + tmp;
+        const double logStandConc = log(pressure / RT);
+        const double rrt = 1.0 / RT;
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            if (j < m.nRounds) {
+                int s = j * 64 + lane;
+                int w = m.info[s];
+                int kind = info_kind(w);
+                double rkc = 0.0, kf = 0.0;
+                if (kind != kNone) {
+                    if (info_rev(w)) {
+                        double dg = 0.0;
+                        dg += mu_sum(m.prod[s], L.mu);
+                        dg -= mu_sum(m.reac[s], L.mu);
+                        rkc = fmin(exp(dg * rrt - m.dn[s] * logStandConc), kBigNumber);
+                    }
+                    if (kind <= kThreeBody) kf = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT);
+                }
+                R.rkc[j] = rkc;
+                R.kf[j] = kf;
+            }
+        }
+    }
+    // falloff: k_low, k_high and log10(Fcent) (FalloffRate/TroeRate::updateTemp)
+#pragma unroll
+    for (int j = 0; j < RateRegs<NR>::NF; j++) {
+        if (j < m.nFallRounds) {
+            int s = j * 64 + lane;
+            int kind = info_kind(m.info[s]);
+            double klow = 0.0, khigh = 0.0, work = 0.0;
+            if (kind == kLindemann || kind == kTroe) {
+                klow = arrhenius(m.A0[s], m.b0[s], m.EaR0[s], logT, recipT);
+                khigh = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT);
+                if (kind == kTroe) {
+                    double Fcent = (1.0 - m.troeA[s]) * exp(-T * m.rT3[s]) + m.troeA[s] * exp(-T * m.rT1[s]);
+                    if (m.T2[s] != 0.0) Fcent += exp(-m.T2[s] / T);
+                    work = log10(fmax(Fcent, kSmallNumber));
+                }
+            }
+            R.klow[j] = klow;
+            R.khigh[j] = khigh;
+            R.work[j] = work;
+        }
+    }
+    R.T = T;
+    return true;
+}
+
+// Rates of progress into LDS (composition-dependent part of updateROP).
+// perturb: per-slot rate multipliers (Cantera m_perturb).
+template <int NR>
+__device__ inline void rates_of_progress(const DevMech& m, int lane, const GasState& g,
+                                         const RateRegs<NR>& R, const double* perturb,
+                                         const WaveLds& L) {
+    wave_lds_sync();  // concentrations written by set_state
+    // ThirdBodyCalc::update per efficiency group
+    if (lane < m.nGroups) {
+        double sum = 0.0;
+        for (int e = m.grp_off[lane]; e < m.grp_off[lane + 1]; e++)
+            sum += m.grp_eff[e] * L.C[m.grp_sp[e]];
+        L.Mg[lane] = m.grp_def[lane] * g.ctot + sum;
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+        if (j < m.nRounds) {
+            int s = j * 64 + lane;
+            int w = m.info[s];
+            int kind = info_kind(w);
+            double ropf = 0.0, ropr = 0.0;
+            if (kind != kNone) {
+                double kf = R.kf[j];
+                if (j < RateRegs<NR>::NF && (kind == kLindemann || kind == kTroe)) {
+                    double khigh = R.khigh[j];
+                    double pr = L.Mg[info_group(w)] * R.klow[j] / (khigh + kSmallNumber);
+                    double F = 1.0;
+                    if (kind == kTroe) {
+                        double work = R.work[j];
+                        double lpr = log10(fmax(pr, kSmallNumber));
+                        double cc = -0.4 - 0.67 * work;
+                        double nn = 0.75 - 1.27 * work;
+                        double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
+                        double lf = work / (1.0 + f1 * f1);
+                        F = pow(10.0, lf);
+                    }
+                    pr *= F / (1.0 + pr);
+                    kf = pr * khigh;
+                }
+                double rfn = kf * perturb[s];
+                ropf = rfn;
+                if (kind == kThreeBody) ropf *= L.Mg[info_group(w)];
+                ropr = ropf;
+                ropf *= conc_product(m.reac[s], L.C);
+                ropr *= R.rkc[j];
+                if (info_rev(w)) ropr *= conc_product(m.prod[s], L.C);
+            }
+            L.ropf[s] = ropf;
+            L.ropr[s] = ropr;
+        }
+    }
+    wave_lds_sync();
+}
+
+// Species creation / destruction rates: chunked sums, then per-species totals.
+// Returns (wQ, wD) for this lane's species (0 elsewhere).
+__device__ inline void species_rates(const DevMech& m, int lane, const WaveLds& L, double& wQ,
+                                     double& wD) {
+#pragma unroll
+    for (int cr = 0; cr < kMaxChunkRounds; cr++) {
+        if (cr < m.nChunkRounds) {
+            int c = cr * 64 + lane;
+            if (c < m.nChunks) {
+                double q = 0.0, d = 0.0;
+                for (int e = m.chunk_off[c]; e < m.chunk_off[c + 1]; e++) {
+                    uint32_t en = m.entry[e];
+                    int s = en_slot(en);
+                    double nu = (double)en_nu(en);
+                    double f = L.ropf[s], r = L.ropr[s];
+                    if (en_prod(en)) {
+                        q += nu * f;
+                        d += nu * r;
+                    } else {
+                        q += nu * r;
+                        d += nu * f;
+                    }
+                }
+                L.partq[c] = q;
+                L.partd[c] = d;
+            }
+        }
+    }
+    wave_lds_sync();
+    wQ = 0.0;
+    wD = 0.0;
+    if (is_species_lane(lane, m)) {
+        int k = lane - 2;
+        int c0 = m.sp_chunk0[k], nc = m.sp_nchunk[k];
+        for (int c = c0; c < c0 + nc; c++) {
+            wQ += L.partq[c];
+            wD += L.partd[c];
+        }
+    }
+    wave_lds_sync();
+}
+
+}  // namespace emx

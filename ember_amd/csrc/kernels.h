@@ -1,0 +1,57 @@
+// Kernel launchers (device code in *.hip), called by the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "emx_common.h"
+
+namespace emx {
+
+int launch_qss_stage(const DevMech& m, const DevOpts& o, const double* state_in,
+                     const double* split_const, const double* qloss, const double* perturb,
+                     double dt, int nPoints, const int* order, unsigned int* queue,
+                     double* state_out, int* status, int* stat1, int* stat2, double* qint,
+                     int grid_blocks, int waves_per_block, hipStream_t stream);
+
+int launch_rates(const DevMech& m, const double* T, const double* Y, int nonorm,
+                 const double* perturb, int n, double* wQ, double* wD, double* wnet, int* status,
+                 hipStream_t stream);
+
+}  // namespace emx
+
+namespace emx {
+
+struct DiffArgs {
+    const double* B;
+    const int32_t* B_row;
+    const double* D;
+    const double* sc;
+    const double* x;
+    int N, nV, alpha, left, right;
+    double yInf, wallConst;
+};
+
+struct DiffWork {
+    double *own1, *nxt1, *X1, *own2, *nxt2, *X2, *cp, *dp, *cmin, *yprev;
+};
+
+size_t diffusion_work_doubles(int nV, int N);
+void diffusion_carve(double* base, int nV, int N, DiffWork& w);
+int launch_diffusion_hmin(const DiffArgs& a, DiffWork& w, double* hmin, hipStream_t st);
+int launch_diffusion_solve(const DiffArgs& a, double* y, const double* h, const int* nsteps, int q,
+                           DiffWork& w, hipStream_t st);
+int launch_explicit_solve(const double* A, const double* B, const double* C, const double* K,
+                          int nSys, int N, double* y, const double* h, const int* nsteps, int q,
+                          DiffWork& w, hipStream_t st);
+
+}  // namespace emx
+
+namespace emx {
+
+int launch_bdf_stage(const DevMech& m, const DevOpts& o, const double* state_in,
+                     const double* split_const, const double* qloss, const double* perturb,
+                     double dt, int nPoints, const int* order, unsigned int* queue,
+                     double* state_out, int* status, int* stat1, int* stat2, double* qint,
+                     int grid_blocks, int waves_per_block, hipStream_t stream);
+
+}  // namespace emx

@@ -1,0 +1,256 @@
+// Mechanism table reader and device layout builder.
+#include "mech_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+
+namespace emx {
+
+std::string read_table(const std::string& path, HostMech& m) {
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return "cannot open mechanism table " + path;
+    char tag[64], buf[256];
+    int E = 0;
+    std::string err;
+    if (fscanf(f, "%63s %255s", tag, buf) != 2 || strcmp(tag, "EMXM") != 0) err = "not an EMXM table";
+    while (err.empty() && fscanf(f, "%63s", tag) == 1) {
+        if (!strcmp(tag, "END")) break;
+        if (!strcmp(tag, "name")) {
+            if (fscanf(f, "%255s", buf) != 1) err = "bad name";
+            m.name = buf;
+        } else if (!strcmp(tag, "counts")) {
+            if (fscanf(f, "%d %d %d", &m.K, &m.I, &E) != 3) err = "bad counts";
+        } else if (!strcmp(tag, "pref")) {
+            if (fscanf(f, "%lf", &m.Pref) != 1) err = "bad pref";
+        } else if (!strcmp(tag, "elements")) {
+            for (int e = 0; e < E && err.empty(); e++)
+                if (fscanf(f, "%255s", buf) != 1) err = "bad elements";
+        } else if (!strcmp(tag, "S")) {
+            double v[18];
+            if (fscanf(f, "%255s", buf) != 1) { err = "bad species"; break; }
+            for (int j = 0; j < 18 && err.empty(); j++)
+                if (fscanf(f, "%lf", &v[j]) != 1) err = "bad species data";
+            for (int e = 0; e < E && err.empty(); e++) {
+                double c;
+                if (fscanf(f, "%lf", &c) != 1) err = "bad composition";
+            }
+            m.species.push_back(buf);
+            m.W.push_back(v[0]);
+            m.Tmid.push_back(v[2]);
+            for (int j = 0; j < 14; j++) m.nasa.push_back(v[4 + j]);
+        } else if (!strcmp(tag, "R")) {
+            HostRxn r;
+            int neff = 0;
+            if (fscanf(f, "%d %d %d %d %d %d %d %d %d %d", &r.type, &r.rev, &r.nr, &r.r[0], &r.r[1],
+                       &r.r[2], &r.np, &r.p[0], &r.p[1], &r.p[2]) != 10 ||
+                fscanf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %d %lf %d", &r.A, &r.b, &r.EaR,
+                       &r.A0, &r.b0, &r.EaR0, &r.troeA, &r.T3, &r.T1, &r.T2, &r.hasT2, &r.defeff,
+                       &neff) != 13) {
+                err = "bad reaction";
+                break;
+            }
+            for (int j = 0; j < neff; j++) {
+                int k;
+                double e;
+                if (fscanf(f, "%d %lf", &k, &e) != 2) { err = "bad efficiencies"; break; }
+                r.eff.emplace_back(k, e);
+            }
+            m.rxn.push_back(r);
+        } else {
+            err = std::string("unknown record ") + tag;
+        }
+    }
+    fclose(f);
+    if (err.empty() && ((int)m.species.size() != m.K || (int)m.rxn.size() != m.I))
+        err = "table counts do not match records";
+    return err;
+}
+
+static uint32_t pack_stoich(int n, const int* k) {
+    uint32_t s = (uint32_t)n;
+    for (int j = 0; j < n; j++) s |= ((uint32_t)k[j] & 0xff) << (8 * (j + 1));
+    return s;
+}
+
+std::string build_layout(const HostMech& m, DevLayout& L) {
+    if (m.K + 2 > kMaxVars) return "mechanism has more than 62 species";
+    if (m.K > 255) return "too many species";
+    // device order: Troe, Lindemann, three-body, elementary (stable)
+    std::vector<int> order(m.I);
+    for (int i = 0; i < m.I; i++) order[i] = i;
+    auto rank = [&](int i) {
+        int t = m.rxn[i].type;
+        return t == 3 ? 0 : t == 2 ? 1 : t == 1 ? 2 : 3;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rank(a) < rank(b); });
+    int nFall = 0;
+    for (int i = 0; i < m.I; i++) nFall += (m.rxn[i].type >= 2);
+    L.nRounds = (m.I + 63) / 64;
+    L.nFallRounds = (nFall + 63) / 64;
+    if (L.nRounds > kMaxRounds) return "mechanism has more than 512 reactions";
+    if (L.nFallRounds > kMaxFallRounds) return "mechanism has more than 128 falloff reactions";
+    const int S = L.nRounds * 64;
+    L.orig.assign(S, -1);
+    L.slot_of_rxn.assign(m.I, -1);
+    for (int s = 0; s < m.I; s++) {
+        L.orig[s] = order[s];
+        L.slot_of_rxn[order[s]] = s;
+    }
+    L.A.assign(S, 0); L.b.assign(S, 0); L.EaR.assign(S, 0); L.dn.assign(S, 0);
+    L.A0.assign(S, 0); L.b0.assign(S, 0); L.EaR0.assign(S, 0);
+    L.troeA.assign(S, 0); L.rT3.assign(S, 0); L.rT1.assign(S, 0); L.T2.assign(S, 0);
+    L.reac.assign(S, 0); L.prod.assign(S, 0); L.info.assign(S, kNone);
+    // third-body efficiency groups
+    std::map<std::pair<double, std::vector<std::pair<int, double>>>, int> groups;
+    std::vector<std::pair<double, std::vector<std::pair<int, double>>>> glist;
+    for (int s = 0; s < m.I; s++) {
+        const HostRxn& r = m.rxn[order[s]];
+        L.A[s] = r.A; L.b[s] = r.b; L.EaR[s] = r.EaR;
+        L.dn[s] = double(r.np) - double(r.nr);
+        L.A0[s] = r.A0; L.b0[s] = r.b0; L.EaR0[s] = r.EaR0;
+        L.troeA[s] = r.troeA;
+        // TroeRate: 1/T3, 1/T1 (an zero T3/T1 means an infinite rate constant)
+        L.rT3[s] = (std::fabs(r.T3) < 1e-300) ? INFINITY : 1.0 / r.T3;
+        L.rT1[s] = (std::fabs(r.T1) < 1e-300) ? INFINITY : 1.0 / r.T1;
+        L.T2[s] = r.T2;
+        L.reac[s] = pack_stoich(r.nr, r.r);
+        L.prod[s] = pack_stoich(r.np, r.p);
+        int group = 255;
+        if (r.type != 0) {
+            std::vector<std::pair<int, double>> e;
+            for (auto& pe : r.eff) e.emplace_back(pe.first, pe.second - r.defeff);
+            auto key = std::make_pair(r.defeff, e);
+            auto it = groups.find(key);
+            if (it == groups.end()) {
+                group = (int)glist.size();
+                groups[key] = group;
+                glist.push_back(key);
+            } else {
+                group = it->second;
+            }
+        }
+        int kind = r.type == 3 ? kTroe : r.type == 2 ? kLindemann : r.type == 1 ? kThreeBody : kElementary;
+        L.info[s] = kind | (r.rev ? 8 : 0) | (r.hasT2 ? 16 : 0) | (group << 8);
+    }
+    if ((int)glist.size() > 64) return "more than 64 distinct third-body efficiency sets";
+    L.nGroups = (int)glist.size();
+    L.grp_off.assign(1, 0);
+    for (auto& g : glist) {
+        L.grp_def.push_back(g.first);
+        for (auto& e : g.second) {
+            L.grp_sp.push_back(e.first);
+            L.grp_eff.push_back(e.second);
+        }
+        L.grp_off.push_back((int)L.grp_sp.size());
+    }
+    if (L.grp_sp.empty()) { L.grp_sp.push_back(0); L.grp_eff.push_back(0.0); }
+    if (L.grp_def.empty()) L.grp_def.push_back(0.0);
+    // species-sum entries: products then reactants, slot order
+    std::vector<std::vector<uint32_t>> ent(m.K);
+    for (int s = 0; s < m.I; s++) {
+        const HostRxn& r = m.rxn[order[s]];
+        std::map<int, int> cnt;
+        for (int j = 0; j < r.np; j++) cnt[r.p[j]]++;
+        for (auto& c : cnt) ent[c.first].push_back((uint32_t)s | ((uint32_t)c.second << 16) | (1u << 24));
+    }
+    for (int s = 0; s < m.I; s++) {
+        const HostRxn& r = m.rxn[order[s]];
+        std::map<int, int> cnt;
+        for (int j = 0; j < r.nr; j++) cnt[r.r[j]]++;
+        for (auto& c : cnt) ent[c.first].push_back((uint32_t)s | ((uint32_t)c.second << 16));
+    }
+    // chunk length minimising (rounds x length) with rounds <= kMaxChunkRounds
+    size_t maxcnt = 1;
+    for (auto& e : ent) maxcnt = std::max(maxcnt, e.size());
+    int bestS = -1;
+    long bestCost = 1L << 40;
+    for (int Sz = 1; Sz <= (int)maxcnt; Sz++) {
+        long nch = 0;
+        for (auto& e : ent) nch += std::max<long>(1, ((long)e.size() + Sz - 1) / Sz);
+        long rounds = (nch + 63) / 64;
+        if (rounds > kMaxChunkRounds) continue;
+        long cost = rounds * Sz;
+        if (cost < bestCost) { bestCost = cost; bestS = Sz; }
+    }
+    if (bestS < 0) return "species-sum chunking does not fit";
+    L.chunk_off.assign(1, 0);
+    L.sp_chunk0.assign(m.K, 0);
+    L.sp_nchunk.assign(m.K, 0);
+    for (int k = 0; k < m.K; k++) {
+        L.sp_chunk0[k] = (int)L.chunk_off.size() - 1;
+        size_t n = ent[k].size();
+        int nc = std::max<int>(1, (int)((n + bestS - 1) / bestS));
+        for (int c = 0; c < nc; c++) {
+            size_t a = (size_t)c * bestS, b = std::min(n, a + bestS);
+            for (size_t e = a; e < b; e++) L.entry.push_back(ent[k][e]);
+            L.chunk_off.push_back((int)L.entry.size());
+        }
+        L.sp_nchunk[k] = nc;
+    }
+    if (L.entry.empty()) L.entry.push_back(0);
+    L.nChunks = (int)L.chunk_off.size() - 1;
+    L.nChunkRounds = (L.nChunks + 63) / 64;
+    L.rW.resize(m.K);
+    for (int k = 0; k < m.K; k++) L.rW[k] = 1.0 / m.W[k];
+    return "";
+}
+
+static void host_nasa(const HostMech& m, int k, double T, double& cpR, double& hRT, double& sR) {
+    const double* c = &m.nasa[14 * k + ((T <= m.Tmid[k]) ? 0 : 7)];
+    double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+    double ct0 = c[0], ct1 = c[1] * T, ct2 = c[2] * T2, ct3 = c[3] * T3, ct4 = c[4] * T4;
+    cpR = ct0 + ct1 + ct2 + ct3 + ct4;
+    hRT = ct0 + 0.5 * ct1 + 1.0 / 3.0 * ct2 + 0.25 * ct3 + 0.2 * ct4 + c[5] * (1.0 / T);
+    sR = ct0 * std::log(T) + ct1 + 0.5 * ct2 + 1.0 / 3.0 * ct3 + 0.25 * ct4 + c[6];
+}
+
+void build_interp_tables(const HostMech& m, const DevLayout& L, double P, std::vector<double>& kf_c,
+                         std::vector<double>& kf_s, std::vector<double>& kc_c,
+                         std::vector<double>& kc_s) {
+    const int S = L.nRounds * 64;
+    kf_c.assign((size_t)kInterpN * S, 0.0);
+    kf_s.assign((size_t)kInterpN * S, 0.0);
+    kc_c.assign((size_t)kInterpN * S, 0.0);
+    kc_s.assign((size_t)kInterpN * S, 0.0);
+    std::vector<double> mu(m.K);
+    for (int n = 0; n < kInterpN; n++) {
+        double T = kInterpTmin + ((double)n) / (kInterpN - 1) * (kInterpTmax - kInterpTmin);
+        double RT = T * kGasConstant;
+        double tmp = std::log(P / m.Pref) * RT;
+        for (int k = 0; k < m.K; k++) {
+            double cpR, hRT, sR;
+            host_nasa(m, k, T, cpR, hRT, sR);
+            mu[k] = RT * (hRT - sR) + tmp;
+        }
+        double logStandConc = std::log(P / RT), rrt = 1.0 / RT;
+        double logT = std::log(T), recipT = 1.0 / T;
+        for (int s = 0; s < m.I; s++) {
+            const HostRxn& r = m.rxn[L.orig[s]];
+            double rkc = 0.0;
+            if (r.rev) {
+                double dg = 0.0;
+                if (r.np == 1) dg += mu[r.p[0]];
+                else if (r.np == 2) dg += mu[r.p[0]] + mu[r.p[1]];
+                else dg += mu[r.p[0]] + mu[r.p[1]] + mu[r.p[2]];
+                if (r.nr == 1) dg -= mu[r.r[0]];
+                else if (r.nr == 2) dg -= (mu[r.r[0]] + mu[r.r[1]]);
+                else dg -= (mu[r.r[0]] + mu[r.r[1]] + mu[r.r[2]]);
+                rkc = std::min(std::exp(dg * rrt - L.dn[s] * logStandConc), kBigNumber);
+            }
+            kc_c[(size_t)n * S + s] = rkc;
+            if (r.type <= 1) kf_c[(size_t)n * S + s] = r.A * std::exp(r.b * logT - r.EaR * recipT);
+        }
+    }
+    double dT = (kInterpTmax - kInterpTmin) / (kInterpN - 1);
+    for (int n = 0; n < kInterpN - 1; n++)
+        for (int s = 0; s < S; s++) {
+            size_t a = (size_t)n * S + s, b = (size_t)(n + 1) * S + s;
+            kf_s[a] = (kf_c[b] - kf_c[a]) / dT;
+            kc_s[a] = (kc_c[b] - kc_c[a]) / dT;
+        }
+}
+
+}  // namespace emx

@@ -1,0 +1,58 @@
+// Host-side mechanism tables: reads a compiled ``.emxm`` table and lays it out for
+// the wavefront kernels (device slot order, efficiency groups, species-sum chunks,
+// interpolation tables).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "emx_common.h"
+
+namespace emx {
+
+struct HostRxn {
+    int type = 0, rev = 1, nr = 0, np = 0;
+    int r[3] = {-1, -1, -1}, p[3] = {-1, -1, -1};
+    double A = 0, b = 0, EaR = 0, A0 = 0, b0 = 0, EaR0 = 0;
+    double troeA = 0, T3 = 0, T1 = 0, T2 = 0;
+    int hasT2 = 0;
+    double defeff = 1.0;
+    std::vector<std::pair<int, double>> eff;
+};
+
+struct HostMech {
+    std::string name;
+    int K = 0, I = 0;
+    double Pref = 101325.0;
+    std::vector<std::string> species;
+    std::vector<double> W, Tmid, nasa;  // nasa: K x 14
+    std::vector<HostRxn> rxn;
+};
+
+// Returns an error message, empty on success.
+std::string read_table(const std::string& path, HostMech& out);
+
+// Flat arrays in device order, ready to upload into one allocation.
+struct DevLayout {
+    int nRounds = 0, nFallRounds = 0, nGroups = 0, nChunks = 0, nChunkRounds = 0;
+    std::vector<int> slot_of_rxn;  // reaction -> slot
+    std::vector<int> orig;         // slot -> reaction (-1 padding)
+    std::vector<double> A, b, EaR, dn, A0, b0, EaR0, troeA, rT3, rT1, T2;
+    std::vector<uint32_t> reac, prod;
+    std::vector<int32_t> info;
+    std::vector<int32_t> grp_off, grp_sp;
+    std::vector<double> grp_eff, grp_def;
+    std::vector<int32_t> chunk_off;
+    std::vector<uint32_t> entry;
+    std::vector<int32_t> sp_chunk0, sp_nchunk;
+    std::vector<double> rW;
+};
+
+std::string build_layout(const HostMech& m, DevLayout& L);
+
+// InterpKinetics / MultiArrheniusInterp tables at pressure P, [kInterpN][nRounds*64]
+void build_interp_tables(const HostMech& m, const DevLayout& L, double P, std::vector<double>& kf_c,
+                         std::vector<double>& kf_s, std::vector<double>& kc_c,
+                         std::vector<double>& kc_s);
+
+}  // namespace emx

@@ -1,0 +1,157 @@
+/*
+ * ember_amd: MI355X-native batched split-step kernels for Ember.
+ *
+ * C ABI (plain pointers and sizes).  Each entry point replaces one per-point or
+ * per-variable loop of Ember's FlameSolver with one batched call:
+ *
+ *   chemistry  FlameSolver::integrateProductionTerms()  src/flameSolver.cpp:754-761
+ *              tbb::parallel_for over points             src/flameSolver.cpp:758-759
+ *              -> SourceSystem{QSS,CVODE}::setState / integrateToTime / unroll_y
+ *                 (src/sourceSystem.h:26-41, src/sourceSystem.cpp:295-338, 434-526)
+ *   diffusion  FlameSolver::integrateDiffusionTerms()   src/flameSolver.cpp:838-854
+ *              tbb::parallel_for over variables          src/flameSolver.cpp:842-843
+ *              -> setDiffusionSolverState (src/flameSolver.cpp:702-718) +
+ *                 TridiagonalIntegrator::integrateToTime (src/integrator.cpp:37-47,133-204)
+ *                 over DiffusionSystem::get_A / get_k   (src/diffusionSystem.cpp:10-76)
+ *
+ * Layouts are Ember's: a state is the nVars x nPoints column-major matrix of
+ * SplitSolver::state (src/splitSolver.h:73), i.e. the nVars values of one point
+ * are contiguous, rows U, T, Y_0..Y_{K-1} (src/readConfig.h:7-9).
+ *
+ * Return codes: 0 = success, < 0 = API or HIP error (text in emx_last_error()).
+ * Per-point integration failures are reported in `status` (0 ok, < 0 failed) and the
+ * failed point keeps its input state, as FlameSolver does (src/flameSolver.cpp:794-834).
+ *
+ * Calls on one context are not re-entrant; use one context per host thread.
+ */
+#ifndef EMBER_AMD_H
+#define EMBER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct emx_context emx_context;
+
+enum { EMX_QSS = 0, EMX_BDF = 1 };                 /* chemistryIntegrator "qss" / "cvode" */
+enum { EMX_KIN_STANDARD = 0, EMX_KIN_INTERP = 1 };  /* kineticsModel "standard" / "interp" */
+enum { EMX_BC_FIXED_VALUE = 0, EMX_BC_ZERO_GRADIENT = 1, EMX_BC_CONTROL_VOLUME = 2,
+       EMX_BC_WALL_FLUX = 3 };                      /* BoundaryCondition, src/grid.h:7-25 */
+
+/* Hot-path subset of ConfigOptions (src/readConfig.h:29-217); defaults are
+ * python/ember/input.py:371, 686-776. */
+typedef struct emx_options {
+    int32_t chemistry_integrator;
+    int32_t kinetics_model;
+    double qss_epsmin, qss_epsmax, qss_dtmin, qss_dtmax;
+    int32_t qss_iteration_count;
+    double qss_abstol, qss_minval;
+    int32_t qss_stability_check;
+    double integrator_rel_tol, integrator_momentum_abs_tol, integrator_energy_abs_tol,
+        integrator_species_abs_tol, integrator_min_timestep;
+    int32_t integrator_max_steps;
+    double diffusion_timestep_multiplier;
+    double global_timestep;
+} emx_options;
+
+/* One-dimensional grid for the diffusion systems (OneDimGrid, src/grid.h:120-126). */
+typedef struct emx_grid {
+    const double* x;      /* nPoints positions (host pointer for host entry points) */
+    int32_t alpha;        /* 0 planar, 1 cylindrical */
+    int32_t left_bc, right_bc;
+    double y_inf;         /* DiffusionSystem::yInf (WallFlux) */
+    double wall_const;    /* DiffusionSystem::wallConst (WallFlux) */
+} emx_grid;
+
+/* Fills `o` with Ember's defaults. */
+void emx_default_options(emx_options* o);
+
+/* Creates a context on the current HIP device.  `mech_table_path` is a compiled
+ * mechanism table (ember_amd.mechanism.table_path); replaces CanteraGas::setOptions +
+ * initialize (src/chemistry0d.cpp:445-485).  pressure in Pa. */
+int emx_create(const char* mech_table_path, double pressure, const emx_options* opts,
+               emx_context** out);
+void emx_destroy(emx_context* ctx);
+const char* emx_last_error(void);
+int emx_num_species(const emx_context* ctx);
+int emx_num_reactions(const emx_context* ctx);
+int emx_set_options(emx_context* ctx, const emx_options* opts);
+
+/* Per-reaction rate multipliers, Cantera Kinetics::setMultiplier as used by
+ * CanteraGas::setRateMultiplier (src/chemistry0d.cpp:634-642).  NULL resets to 1. */
+int emx_set_rate_multipliers(emx_context* ctx, const double* multipliers);
+
+/* HIP stream (hipStream_t) for the *_device entry points; NULL = default stream. */
+int emx_set_stream(emx_context* ctx, void* stream);
+
+/* Launch shape for the chemistry kernels: waves per workgroup and workgroups per CU
+ * (0 = built-in default).  The autotuner in ember_amd.autotune picks these. */
+int emx_set_launch(emx_context* ctx, int waves_per_block, int blocks_per_cu);
+
+/* Chemistry production stage over [t_stage_start, t_stage_start + dt] for all points:
+ * FlameSolver::integrateProductionTerms (src/flameSolver.cpp:754-836) with
+ * SourceSystemQSS or SourceSystemCVODE per point.
+ *   state_in / state_out   nVars x nPoints (nVars = nSpecies + 2), host memory
+ *   split_const            nVars x nPoints, SplitSolver::splitConstProd (NULL = zeros)
+ *   qloss                  nPoints heat loss [W/m^3] evaluated at setState (NULL = zeros)
+ *   status                 nPoints, 0 ok / <0 failed (may be NULL)
+ *   stat1, stat2           nPoints: QSS gcount/rcount, BDF steps/RHS evaluations (NULL ok)
+ * Synchronous. */
+int emx_source_integrate(emx_context* ctx, size_t nPoints, size_t nVars, const double* state_in,
+                         const double* split_const, const double* qloss, double t_stage_start,
+                         double dt, double* state_out, int32_t* status, int32_t* stat1,
+                         int32_t* stat2);
+
+/* Same with device pointers; asynchronous on the context stream.  qint (nullable): the
+ * heat release integrated over the stage on accepted internal steps [J/m^3]. */
+int emx_source_integrate_device(emx_context* ctx, size_t nPoints, size_t nVars,
+                                const double* state_in, const double* split_const,
+                                const double* qloss, double t_stage_start, double dt,
+                                double* state_out, int32_t* status, int32_t* stat1,
+                                int32_t* stat2, double* qint);
+
+/* Creation, destruction and net molar production rates [kmol/m^3/s] at n states
+ * (CanteraGas::getCreationRates / getDestructionRates / getReactionRates,
+ * src/chemistry0d.cpp:644-672).  T: n, Y: n x nSpecies, outputs n x nSpecies.
+ * nonorm = 0: setState_TPY (QSS path); 1: setMassFractions_NoNorm (CVODE path).
+ * status (nullable): 0 ok, -2 if T is outside the interpolation table (interp only). */
+int emx_rates(emx_context* ctx, size_t n, const double* T, const double* Y, int nonorm,
+              double* wdot_creation, double* wdot_destruction, double* wdot_net,
+              int32_t* status);
+
+/* Diffusion stage over [t0, t1] for nVars variables on one grid:
+ * FlameSolver::setDiffusionSolverState + integrateDiffusionTerms.
+ *   B         nB x nPoints, row b contiguous; B_row[v] selects the row used by variable v
+ *             (NULL: nB == nVars, row v).  DiffusionSystem::B (1/rho, 1/(rho cp), ...)
+ *   D         nVars x nPoints (Ember layout): mu, lambda, rhoD_k
+ *   split_const  nVars x nPoints, splitConstDiff (NULL = zeros)
+ *   h_per_var nVars initial step sizes (NULL: Ember's rule
+ *             min(diffusion_timestep_multiplier * min_j dlj^2/(B D), global_timestep))
+ *   state     nVars x nPoints, in/out.
+ * Synchronous (host pointers). */
+int emx_diffusion_integrate(emx_context* ctx, size_t nVars, size_t nPoints, const double* B,
+                            size_t nB, const int32_t* B_row, const double* D,
+                            const double* split_const, const emx_grid* grid,
+                            const double* h_per_var, double t0, double t1, double* state);
+
+/* Same with device pointers (grid->x and B_row also device); asynchronous. */
+int emx_diffusion_integrate_device(emx_context* ctx, size_t nVars, size_t nPoints,
+                                   const double* B, size_t nB, const int32_t* B_row,
+                                   const double* D, const double* split_const,
+                                   const emx_grid* grid, const double* h_per_var, double t0,
+                                   double t1, double* state);
+
+/* TridiagonalIntegrator::integrateToTime for nSys constant systems y' = A y + k
+ * (test/tridiagonalIntegrator.cpp ConstantTridiagonalODE).  a, b, c, k, y: nSys x N,
+ * row-major (system s contiguous).  h: initial step per system.  Host pointers. */
+int emx_tridiagonal_integrate(emx_context* ctx, size_t nSys, size_t N, const double* a,
+                              const double* b, const double* c, const double* k,
+                              const double* h, double t0, double t1, double* y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EMBER_AMD_H */

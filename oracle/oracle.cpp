@@ -417,31 +417,166 @@ struct Gas {
 // ---------------------------------------------------------------------------
 int isign(double x) { return (x > 0) ? 1 : (x < 0) ? -1 : 0; }
 
-struct SourceQSS {
-    Gas& gas;
-    const orc_options& o;
-    int K, N;
-    // integrator state
+// QssOde (src/qssintegrator.h:6-21)
+struct QssOde {
+    virtual ~QssOde() {}
+    virtual void odefun(double t, const double* y, double* q, double* d, bool corrector) = 0;
+    virtual bool failed() const { return false; }
+};
+
+// QssIntegrator (src/qssintegrator.h:24-110, src/qssintegrator.cpp)
+struct QssIntegrator {
+    QssOde* ode = nullptr;
+    int N = 0;
     std::vector<double> y, q, d, rtau, rtaus, y1, ys, qs, scratch, ym1, ym2, ymin, rtaub, alpha;
     std::vector<char> enforce;
     double tn = 0, ts = 0, dt = 0, tstart = 0, tfd = 1.000008;
     bool firstStep = true;
     int gcount = 0, rcount = 0;
-    // system state
+    // options (constructor defaults of src/qssintegrator.cpp:9-30)
+    double epsmax = 20, epsmin = 1e-2, dtmin = 1e-15, dtmax = 1e-6, abstol = 1e-8;
+    int itermax = 2;
+    bool stabilityCheck = true;
+    // heat-release bookkeeping for accepted steps (set by the ODE on predictor calls)
+    double qint = 0, qdot_step = 0;
+    double* qdot_src = nullptr;
+
+    void initialize(int n) {
+        N = n;
+        y.assign(N, 0); q.assign(N, 0); d.assign(N, 0); rtau.assign(N, 0); rtaus.assign(N, 0);
+        y1.assign(N, 0); ys.assign(N, 0); qs.assign(N, 0); scratch.assign(N, 0);
+        ym1.assign(N, 0); ym2.assign(N, 0); rtaub.assign(N, 0); alpha.assign(N, 0);
+        ymin.assign(N, 1e-20);
+        enforce.assign(N, 1);
+    }
+    void setState(const double* yIn, double tstart_) {
+        for (int i = 0; i < N; i++) y[i] = enforce[i] ? std::max(yIn[i], ymin[i]) : yIn[i];
+        gcount = 0; rcount = 0; tstart = tstart_; tn = 0; firstStep = true;
+        qint = 0;
+    }
+    void getInitialStepSize(double tf) {
+        firstStep = false;
+        double scrtch = 1.0e-25;
+        for (int i = 0; i < N; i++) {
+            if (std::abs(y[i]) > abstol) {
+                double absq = std::abs(q[i]);
+                double scr2 = std::abs(1 / y[i]) * isign(0.1 * epsmin * absq - d[i]);
+                double scr1 = scr2 * d[i];
+                scrtch = std::max(std::max(scr1, -std::abs(absq - d[i]) * scr2), scrtch);
+            }
+        }
+        double sqreps = 0.5;
+        dt = std::min(sqreps / scrtch, tf);
+        dt = std::min(dt, dtmax);
+    }
+    int integrateToTime(double tf) {
+        while (tfd * tn < tf) {
+            int ret = integrateOneStep(tf);
+            if (ode->failed()) return -2;
+            if (ret) return ret;
+        }
+        return 0;
+    }
+    int integrateOneStep(double tf) {
+        ode->odefun(tn + tstart, y.data(), q.data(), d.data(), false);
+        gcount += 1;
+        qdot_step = qdot_src ? *qdot_src : 0.0;
+        if (firstStep) getInitialStepSize(tf);
+        ts = tn;
+        for (int i = 0; i < N; i++) rtau[i] = enforce[i] ? dt * d[i] / y[i] : 0.0;
+        qs = q; ys = y; rtaus = rtau;
+        while (true) {
+            for (int i = 0; i < N; i++) {
+                double r = rtau[i];
+                scratch[i] = (q[i] - d[i]) /
+                             (1.0 + r * (180 + r * (60 + r * (11 + r))) / (360 + r * (60 + r * (12 + r))));
+            }
+            double eps = 1e-10;
+            for (int iter = 0; iter < itermax; iter++) {
+                if (stabilityCheck) { ym2 = ym1; ym1 = y; }
+                for (int i = 0; i < N; i++) y[i] = std::max(ymin[i], ys[i] + dt * scratch[i]);
+                if (iter == 0) { tn = ts + dt; y1 = y; }
+                ode->odefun(tn + tstart, y.data(), q.data(), d.data(), true);
+                gcount += 1;
+                eps = 1.0e-10;
+                for (int i = 0; i < N; i++) rtaub[i] = enforce[i] ? 0.5 * (rtaus[i] + dt * d[i] / y[i]) : 0.0;
+                for (int i = 0; i < N; i++) {
+                    double r = rtaub[i];
+                    alpha[i] = (180. + r * (60. + r * (11. + r))) / (360. + r * (60. + r * (12. + r)));
+                }
+                for (int i = 0; i < N; i++)
+                    scratch[i] = (qs[i] * (1.0 - alpha[i]) + q[i] * alpha[i] - ys[i] * rtaub[i] / dt) /
+                                 (1.0 + alpha[i] * rtaub[i]);
+            }
+            for (int i = 0; i < N; i++) {
+                double scr2 = ys[i] + dt * scratch[i];
+                if (enforce[i]) scr2 = std::max(scr2, ymin[i]);
+                double scr1 = std::abs(scr2 - y1[i]);
+                y[i] = std::max(scr2, ymin[i]);
+                if (std::abs(y[i]) > abstol && 0.25 * (ys[i] + y[i]) > ymin[i]) {
+                    scr1 = scr1 / y[i];
+                    eps = std::max(.5 * (scr1 + std::min(std::abs(q[i] - d[i]) / (q[i] + d[i] + 1e-30), scr1)), eps);
+                }
+            }
+            if (stabilityCheck) { ym2 = ym1; ym1 = y; }
+            eps /= epsmin;
+            if (dt <= dtmin + 1e-16 * tn) return -1;
+            double stab = 0;
+            if (stabilityCheck && itermax >= 3) {
+                stab = 0.01;
+                for (int i = 0; i < N; i++)
+                    if (std::abs(y[i]) > abstol)
+                        stab = std::max(stab, std::abs(y[i] - ym1[i]) / (std::abs(ym1[i] - ym2[i]) + 1e-20 * y[i]));
+            }
+            if (eps <= epsmax && stab <= 1.0) {
+                qint += dt * qdot_step;
+                if (tf <= tn * tfd) return 0;
+            } else {
+                tn = ts;
+            }
+            double rteps = 0.5 * (eps + 1.0);
+            rteps = 0.5 * (rteps + eps / rteps);
+            rteps = 0.5 * (rteps + eps / rteps);
+            double dto = dt;
+            dt = std::min(dt * (1.0 / rteps + 0.005), tfd * (tf - tn));
+            dt = std::min(dt, dtmax);
+            if (stabilityCheck) dt = std::min(dt, dto / (stab + .001));
+            if (eps > epsmax || stab > 1) {
+                rcount += 1;
+                for (int i = 0; i < N; i++) rtaus[i] *= dt / dto;
+            } else {
+                return 0;
+            }
+        }
+    }
+};
+
+// SourceSystemQSS (src/sourceSystem.cpp:398-531)
+struct SourceQSS : QssOde {
+    Gas& gas;
+    int K, N;
+    QssIntegrator integrator;
     std::vector<double> splitConst, Yv, hk, wQ, wD;
     double U = 0, T = 0, rho = 0, cp = 0, qLoss = 0, qDot = 0;
     bool failed_rates = false;
-    double qint = 0, qdot_step = 0;
 
-    SourceQSS(Gas& g, const orc_options& opt)
-        : gas(g), o(opt), K(g.m.K), N(g.m.K + 2), y(N), q(N), d(N), rtau(N), rtaus(N), y1(N), ys(N),
-          qs(N), scratch(N), ym1(N), ym2(N), ymin(N), rtaub(N), alpha(N), enforce(N, 1),
-          splitConst(N), Yv(K), hk(K), wQ(K), wD(K) {
-        // SourceSystemQSS::initialize / setOptions
-        enforce[0] = 0;
-        std::fill(ymin.begin(), ymin.end(), o.qss_minval);
-        ymin[0] = -1e4;
+    SourceQSS(Gas& g, const orc_options& o)
+        : gas(g), K(g.m.K), N(g.m.K + 2), splitConst(N), Yv(K), hk(K), wQ(K), wD(K) {
+        integrator.ode = this;
+        integrator.initialize(N);
+        integrator.enforce[0] = 0;                 // initialize: enforce_ymin[kMomentum] = false
+        integrator.epsmin = o.qss_epsmin;          // setOptions
+        integrator.epsmax = o.qss_epsmax;
+        integrator.dtmin = o.qss_dtmin;
+        integrator.dtmax = o.qss_dtmax;
+        integrator.itermax = o.qss_itermax;
+        integrator.abstol = o.qss_abstol;
+        integrator.stabilityCheck = o.qss_stability_check != 0;
+        std::fill(integrator.ymin.begin(), integrator.ymin.end(), o.qss_minval);
+        integrator.ymin[0] = -1e4;
+        integrator.qdot_src = &qDot;
     }
+    bool failed() const override { return failed_rates; }
 
     void unroll_y(const double* yy, bool corrector) {
         if (!corrector) T = yy[1];
@@ -449,7 +584,7 @@ struct SourceQSS {
         for (int k = 0; k < K; k++) Yv[k] = yy[2 + k];
     }
 
-    void odefun(double t, const double* yy, double* qq, double* dd, bool corrector) {
+    void odefun(double t, const double* yy, double* qq, double* dd, bool corrector) override {
         (void)t;
         unroll_y(yy, corrector);
         gas.setMassFractions(Yv.data());
@@ -474,110 +609,36 @@ struct SourceQSS {
             dd[2 + k] = wD[k] * gas.m.W[k] / rho;
         }
     }
+    void setState(const double* yIn) { integrator.setState(yIn, 0.0); }
+    int integrateToTime(double tf) { return integrator.integrateToTime(tf); }
+};
 
-    void setState(const double* yIn) {
-        for (int i = 0; i < N; i++) y[i] = enforce[i] ? std::max(yIn[i], ymin[i]) : yIn[i];
-        gcount = 0; rcount = 0; tstart = 0; tn = 0; firstStep = true;
-        qint = 0;
+// CHEMEQ2 cesium/ion relaxation problem (test/old/test_qssintegrator.cpp:11-83)
+struct CesiumOde : QssOde {
+    void odefun(double, const double* y, double* q, double* d, bool) override {
+        double o2m = y[0], csp = y[1], cs = y[2], cso2 = y[3], o2 = y[4], n2 = y[5];
+        double ne = std::max(csp - o2m, 0.0);
+        double cr1 = 5.00e-08 * o2m * csp;
+        double cr2 = 1.00e-12 * csp * ne;
+        double cr3 = 3.24e-03 * cs;
+        double cr4 = 4.00e-01 * o2m;
+        double cr5 = 1.00e-31 * o2 * cs * (cs + cso2 + n2 + o2);
+        double cr6 = 1.24e-30 * o2 * o2 * ne;
+        double cr7 = 1.00e-31 * o2 * n2 * ne;
+        q[0] = cr6 + cr7; d[0] = cr1 + cr4;
+        q[1] = cr3;       d[1] = cr1 + cr2;
+        q[2] = cr1 + cr2; d[2] = cr3 + cr5;
+        q[3] = cr5;       d[3] = 0.0;
+        q[4] = cr1 + cr4; d[4] = cr5 + cr6 + cr7;
+        q[5] = 0.0;       d[5] = 0.0;
     }
+};
 
-    void getInitialStepSize(double tf) {
-        firstStep = false;
-        double scrtch = 1.0e-25;
-        for (int i = 0; i < N; i++) {
-            if (std::abs(y[i]) > o.qss_abstol) {
-                double absq = std::abs(q[i]);
-                double scr2 = std::abs(1 / y[i]) * isign(0.1 * o.qss_epsmin * absq - d[i]);
-                double scr1 = scr2 * d[i];
-                scrtch = std::max(std::max(scr1, -std::abs(absq - d[i]) * scr2), scrtch);
-            }
-        }
-        double sqreps = 0.5;
-        dt = std::min(sqreps / scrtch, tf);
-        dt = std::min(dt, o.qss_dtmax);
-    }
-
-    int integrateOneStep(double tf) {
-        odefun(tn + tstart, y.data(), q.data(), d.data(), false);
-        gcount += 1;
-        qdot_step = qDot;
-        if (firstStep) getInitialStepSize(tf);
-        ts = tn;
-        for (int i = 0; i < N; i++) rtau[i] = enforce[i] ? dt * d[i] / y[i] : 0.0;
-        qs = q; ys = y; rtaus = rtau;
-        const bool stab_on = o.qss_stability_check != 0;
-        while (true) {
-            for (int i = 0; i < N; i++) {
-                double r = rtau[i];
-                scratch[i] = (q[i] - d[i]) /
-                             (1.0 + r * (180 + r * (60 + r * (11 + r))) / (360 + r * (60 + r * (12 + r))));
-            }
-            double eps = 1e-10;
-            for (int iter = 0; iter < o.qss_itermax; iter++) {
-                if (stab_on) { ym2 = ym1; ym1 = y; }
-                for (int i = 0; i < N; i++) y[i] = std::max(ymin[i], ys[i] + dt * scratch[i]);
-                if (iter == 0) { tn = ts + dt; y1 = y; }
-                odefun(tn + tstart, y.data(), q.data(), d.data(), true);
-                gcount += 1;
-                eps = 1.0e-10;
-                for (int i = 0; i < N; i++) rtaub[i] = enforce[i] ? 0.5 * (rtaus[i] + dt * d[i] / y[i]) : 0.0;
-                for (int i = 0; i < N; i++) {
-                    double r = rtaub[i];
-                    alpha[i] = (180. + r * (60. + r * (11. + r))) / (360. + r * (60. + r * (12. + r)));
-                }
-                for (int i = 0; i < N; i++)
-                    scratch[i] = (qs[i] * (1.0 - alpha[i]) + q[i] * alpha[i] - ys[i] * rtaub[i] / dt) /
-                                 (1.0 + alpha[i] * rtaub[i]);
-            }
-            for (int i = 0; i < N; i++) {
-                double scr2 = ys[i] + dt * scratch[i];
-                if (enforce[i]) scr2 = std::max(scr2, ymin[i]);
-                double scr1 = std::abs(scr2 - y1[i]);
-                y[i] = std::max(scr2, ymin[i]);
-                if (std::abs(y[i]) > o.qss_abstol && 0.25 * (ys[i] + y[i]) > ymin[i]) {
-                    scr1 = scr1 / y[i];
-                    eps = std::max(.5 * (scr1 + std::min(std::abs(q[i] - d[i]) / (q[i] + d[i] + 1e-30), scr1)), eps);
-                }
-            }
-            if (stab_on) { ym2 = ym1; ym1 = y; }
-            eps /= o.qss_epsmin;
-            if (dt <= o.qss_dtmin + 1e-16 * tn) return -1;
-            double stab = 0;
-            if (stab_on && o.qss_itermax >= 3) {
-                stab = 0.01;
-                for (int i = 0; i < N; i++)
-                    if (std::abs(y[i]) > o.qss_abstol)
-                        stab = std::max(stab, std::abs(y[i] - ym1[i]) / (std::abs(ym1[i] - ym2[i]) + 1e-20 * y[i]));
-            }
-            if (eps <= o.qss_epsmax && stab <= 1.0) {
-                qint += dt * qdot_step;
-                if (tf <= tn * tfd) return 0;
-            } else {
-                tn = ts;
-            }
-            double rteps = 0.5 * (eps + 1.0);
-            rteps = 0.5 * (rteps + eps / rteps);
-            rteps = 0.5 * (rteps + eps / rteps);
-            double dto = dt;
-            dt = std::min(dt * (1.0 / rteps + 0.005), tfd * (tf - tn));
-            dt = std::min(dt, o.qss_dtmax);
-            if (stab_on) dt = std::min(dt, dto / (stab + .001));
-            if (eps > o.qss_epsmax || stab > 1) {
-                rcount += 1;
-                for (int i = 0; i < N; i++) rtaus[i] *= dt / dto;
-            } else {
-                return 0;
-            }
-        }
-    }
-
-    int integrateToTime(double tf) {
-        while (tfd * tn < tf) {
-            int ret = integrateOneStep(tf);
-            if (failed_rates) return -2;
-            if (ret) return ret;
-        }
-        return 0;
+// y_i' = q_i - k_i y_i with constant q, k
+struct LinearOde : QssOde {
+    std::vector<double> qc, kc;
+    void odefun(double, const double* y, double* q, double* d, bool) override {
+        for (size_t i = 0; i < qc.size(); i++) { q[i] = qc[i]; d[i] = kc[i] * y[i]; }
     }
 };
 
@@ -1104,10 +1165,10 @@ int orc_source_integrate(void* mech, const orc_options* o, int nPoints, const do
                 qss.failed_rates = false;
                 qss.setState(yin);
                 err = qss.integrateToTime(dt);
-                if (err >= 0) std::copy(qss.y.begin(), qss.y.end(), yout);
-                if (stat1) stat1[j] = qss.gcount;
-                if (stat2) stat2[j] = qss.rcount;
-                qi = qss.qint;
+                if (err >= 0) std::copy(qss.integrator.y.begin(), qss.integrator.y.end(), yout);
+                if (stat1) stat1[j] = qss.integrator.gcount;
+                if (stat2) stat2[j] = qss.integrator.rcount;
+                qi = qss.integrator.qint;
             } else {
                 for (int i = 0; i < N; i++) bdf.splitConst[i] = split_const ? split_const[(size_t)j * N + i] : 0.0;
                 bdf.qLoss = qloss ? qloss[j] : 0.0;
@@ -1125,6 +1186,49 @@ int orc_source_integrate(void* mech, const orc_options* o, int nPoints, const do
         }
     }
     return 0;
+}
+
+int orc_qss_test_problem(int problem, int n, const double* qc, const double* kc, double epsmin,
+                         double epsmax, double dtmin, double dtmax, int itermax, double abstol,
+                         double ymin, int stability, double tf, double* y, int* gcount, int* rcount) {
+    CesiumOde ces;
+    LinearOde lin;
+    QssIntegrator it;
+    if (problem == 0) {
+        n = 6;
+        it.ode = &ces;
+    } else {
+        lin.qc.assign(qc, qc + n);
+        lin.kc.assign(kc, kc + n);
+        it.ode = &lin;
+    }
+    it.initialize(n);
+    std::fill(it.ymin.begin(), it.ymin.end(), ymin);
+    it.epsmin = epsmin; it.epsmax = epsmax; it.dtmin = dtmin; it.dtmax = dtmax;
+    it.itermax = itermax; it.abstol = abstol; it.stabilityCheck = stability != 0;
+    it.setState(y, 0.0);
+    int err = it.integrateToTime(tf);
+    for (int i = 0; i < n; i++) y[i] = it.y[i];
+    if (gcount) *gcount = it.gcount;
+    if (rcount) *rcount = it.rcount;
+    return err;
+}
+
+int orc_source_rhs(void* mech, const orc_options* o, const double* y, const double* split_const,
+                   double qloss, double* ydot) {
+    Mech& m = *static_cast<Mech*>(mech);
+    Gas gas(m, o->kinetics, o->pressure);
+    if (o->kinetics == 1) {
+        std::vector<double> Y0(m.K, 1.0 / m.K);
+        gas.setMassFractions(Y0.data());
+        gas.setState_TP(300.0);
+        gas.build_interp();
+    }
+    SourceBDF bdf(gas, *o);
+    for (int i = 0; i < m.K + 2; i++) bdf.splitConst[i] = split_const ? split_const[i] : 0.0;
+    bdf.qLoss = qloss;
+    bdf.f(y, ydot);
+    return bdf.failed_rates ? -2 : 0;
 }
 
 int orc_tridiag_integrate(int N, const double* a, const double* b, const double* c, const double* k,

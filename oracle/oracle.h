@@ -55,6 +55,16 @@ int orc_source_integrate(void* mech, const orc_options* o, int nPoints,
                          const double* qloss, double dt, double* state_out,
                          int* status, int* stat1, int* stat2, double* qint, int nthreads);
 
+/* QssIntegrator on a test ODE.  problem 0: CHEMEQ2 cesium problem (n = 6);
+ * problem 1: y' = qc - kc*y (n components).  y in/out.  Returns the integrator status. */
+int orc_qss_test_problem(int problem, int n, const double* qc, const double* kc, double epsmin,
+                         double epsmax, double dtmin, double dtmax, int itermax, double abstol,
+                         double ymin, int stability, double tf, double* y, int* gcount, int* rcount);
+
+/* SourceSystemCVODE::f at one state y (nVars), for independent ODE solvers. */
+int orc_source_rhs(void* mech, const orc_options* o, const double* y, const double* split_const,
+                   double qloss, double* ydot);
+
 /* TridiagonalIntegrator with a constant (a, b, c, k) system
  * (test/tridiagonalIntegrator.cpp ConstantTridiagonalODE).
  * mode 0: take nsteps single steps of size h from t0.

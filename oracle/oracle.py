@@ -52,6 +52,10 @@ def lib():
                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_source_integrate.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, _dp,
                                            C.c_double, _dp, _ip, _ip, _ip, _dp, C.c_int]
+        L.orc_qss_test_problem.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double,
+                                           C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                           C.c_int, C.c_double, _dp, _ip, _ip]
+        L.orc_source_rhs.argtypes = [C.c_void_p, C.POINTER(Options), _dp, _dp, C.c_double, _dp]
         L.orc_tridiag_integrate.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
                                             C.c_double, C.c_int, C.c_int, _dp, _dp, _dp]
         L.orc_diffusion_integrate.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _ip, _dp, _dp, _dp,
@@ -110,6 +114,17 @@ class Mech:
             raise RuntimeError(f"orc_rates failed: {rc}")
         return out
 
+    def rhs(self, y, opts=None, split_const=None, qloss=0.0):
+        """SourceSystemCVODE::f at state y (nVars)."""
+        opts = opts or options()
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros_like(y)
+        sc = None if split_const is None else np.ascontiguousarray(split_const, dtype=np.float64)
+        rc = lib().orc_source_rhs(self.h, C.byref(opts), _p(y), _p(sc), qloss, _p(out))
+        if rc:
+            raise RuntimeError(f"orc_source_rhs failed: {rc}")
+        return out
+
     def integrate(self, state, dt, opts=None, split_const=None, qloss=None, nthreads=0):
         """One production stage; state is (nPoints, nVars) (Ember column-major)."""
         opts = opts or options()
@@ -125,6 +140,19 @@ class Mech:
         lib().orc_source_integrate(self.h, C.byref(opts), n, _p(state), _p(sc), _p(ql), dt, _p(out),
                                    _p(status), _p(s1), _p(s2), _p(qint), nthreads)
         return out, status, s1, s2, qint
+
+
+def qss_test_problem(problem, y0, tf, epsmin=1e-2, epsmax=20.0, dtmin=1e-15, dtmax=1e-6,
+                     itermax=2, abstol=1e-8, ymin=1e-20, stability=True, qc=None, kc=None):
+    """QssIntegrator on the cesium problem (problem=0) or y' = qc - kc*y (problem=1)."""
+    y = np.array(y0, dtype=np.float64)
+    qc = None if qc is None else np.ascontiguousarray(qc, dtype=np.float64)
+    kc = None if kc is None else np.ascontiguousarray(kc, dtype=np.float64)
+    g = np.zeros(1, np.int32)
+    r = np.zeros(1, np.int32)
+    err = lib().orc_qss_test_problem(problem, len(y), _p(qc), _p(kc), epsmin, epsmax, dtmin, dtmax,
+                                     itermax, abstol, ymin, int(stability), tf, _p(y), _p(g), _p(r))
+    return y, int(err), int(g[0]), int(r[0])
 
 
 def tridiag_integrate(a, b, c, k, y0, t0, h, t1=0.0, mode=1, nsteps=0):
