@@ -77,5 +77,20 @@ def _wait(item):
     os.replace(obj + ".tmp", obj)
 
 
+def build_test_programs() -> str:
+    """C++ test driver for the split-step sequence (links the library and the oracle)."""
+    src = os.path.join(ROOT, "tests", "cpp", "split_flame.cpp")
+    out_dir = os.path.join(ROOT, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "split_flame")
+    oracle_lib = os.path.join(ROOT, "oracle", "_build")
+    _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", exe + ".tmp",
+          "-L", PKG, "-lember_amd", "-L", oracle_lib, "-loracle", "-fopenmp",
+          f"-Wl,-rpath,{PKG}", f"-Wl,-rpath,{oracle_lib}", "-Wl,-rpath,$ORIGIN/../../../ember_amd",
+          "-Wl,-rpath,$ORIGIN/../../../oracle/_build"])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(force="--force" in sys.argv))

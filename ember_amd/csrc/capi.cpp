@@ -72,6 +72,8 @@ struct emx_context {
     DevBuf b_in, b_out, b_sc, b_ql, b_status, b_s1, b_s2;
     DevBuf d_B, d_Brow, d_D, d_sc, d_x, d_y, d_h, d_ns, d_work, d_hmin;
     DevBuf t_a, t_b, t_c, t_k;
+    DevBuf jac, plain_ws;
+    int variant = EMX_VARIANT_WAVE;
 };
 
 namespace {
@@ -171,9 +173,10 @@ int check_ctx(emx_context* c) {
     return 0;
 }
 
-int grid_blocks(emx_context* c, int nPoints) {
+int grid_blocks(emx_context* c, int nPoints, bool bdf) {
     const int wpb = c->waves_per_block;
-    size_t lds_per_block = (size_t)wpb * (3 * 64 + 2 * c->dm.nRounds * 64 + 2 * c->dm.nChunkRounds * 64) * 8;
+    size_t lds_per_block = bdf ? bdf_lds_bytes(c->dm, wpb)
+                               : (size_t)wpb * (3 * 64 + 2 * c->dm.nRounds * 64 + 2 * c->dm.nChunkRounds * 64) * 8;
     int per_cu = c->blocks_per_cu;
     if (per_cu <= 0) {
         per_cu = (int)std::max<size_t>(1, (160 * 1024) / std::max<size_t>(lds_per_block, 1));
@@ -252,7 +255,7 @@ void emx_destroy(emx_context* c) {
     if (c->queue) (void)hipFree(c->queue);
     for (DevBuf* b : {&c->b_in, &c->b_out, &c->b_sc, &c->b_ql, &c->b_status, &c->b_s1, &c->b_s2,
                       &c->d_B, &c->d_Brow, &c->d_D, &c->d_sc, &c->d_x, &c->d_y, &c->d_h, &c->d_ns,
-                      &c->d_work, &c->d_hmin, &c->t_a, &c->t_b, &c->t_c, &c->t_k})
+                      &c->d_work, &c->d_hmin, &c->t_a, &c->t_b, &c->t_c, &c->t_k, &c->jac, &c->plain_ws})
         b->release();
     delete c;
 }
@@ -294,6 +297,13 @@ int emx_set_launch(emx_context* c, int waves_per_block, int blocks_per_cu) {
     return 0;
 }
 
+int emx_set_kernel_variant(emx_context* c, int variant) {
+    if (!c) return fail(-1, "null context");
+    if (variant != EMX_VARIANT_WAVE && variant != EMX_VARIANT_PLAIN) return fail(-1, "unknown variant");
+    c->variant = variant;
+    return 0;
+}
+
 int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, const double* state_in,
                                 const double* split_const, const double* qloss, double t_stage_start,
                                 double dt, double* state_out, int32_t* status, int32_t* stat1,
@@ -304,16 +314,30 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
     if (nPoints == 0) return 0;
     if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
     if (!state_in || !state_out) return fail(-1, "state pointers are NULL");
+    if (c->variant == EMX_VARIANT_PLAIN) {
+        if (c->opts.chemistry_integrator != EMX_QSS || c->opts.qss_stability_check)
+            return fail(-1, "the plain baseline kernel implements QSS without stability check only");
+        HIP_TRY(c->plain_ws.ensure(plain_workspace_doubles(c->dm, (int)nPoints) * sizeof(double)));
+        int rc = launch_qss_plain(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
+                                  (int)nPoints, state_out, status, stat1, stat2,
+                                  c->plain_ws.as<double>(), c->stream);
+        if (rc) return fail(-10, std::string("plain launch: ") + hipGetErrorString((hipError_t)rc));
+        return 0;
+    }
     if (c->opts.chemistry_integrator == EMX_BDF) {
+        if (bdf_lds_bytes(c->dm, c->waves_per_block) > 160 * 1024)
+            return fail(-1, "BDF workgroup needs more than 160 KB of LDS; use fewer waves per block");
+        const int blocks = grid_blocks(c, (int)nPoints, true);
+        HIP_TRY(c->jac.ensure((size_t)blocks * c->waves_per_block * 64 * 64 * sizeof(double)));
         int rc = launch_bdf_stage(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
                                   (int)nPoints, nullptr, c->queue, state_out, status, stat1, stat2,
-                                  qint, grid_blocks(c, (int)nPoints), c->waves_per_block, c->stream);
+                                  qint, blocks, c->waves_per_block, c->jac.as<double>(), c->stream);
         if (rc) return fail(-10, std::string("bdf launch: ") + hipGetErrorString((hipError_t)rc));
         return 0;
     }
     int rc = launch_qss_stage(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
                               (int)nPoints, nullptr, c->queue, state_out, status, stat1, stat2, qint,
-                              grid_blocks(c, (int)nPoints), c->waves_per_block, c->stream);
+                              grid_blocks(c, (int)nPoints, false), c->waves_per_block, c->stream);
     if (rc) return fail(-10, std::string("qss launch: ") + hipGetErrorString((hipError_t)rc));
     return 0;
 }

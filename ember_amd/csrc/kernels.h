@@ -52,6 +52,17 @@ int launch_bdf_stage(const DevMech& m, const DevOpts& o, const double* state_in,
                      const double* split_const, const double* qloss, const double* perturb,
                      double dt, int nPoints, const int* order, unsigned int* queue,
                      double* state_out, int* status, int* stat1, int* stat2, double* qint,
-                     int grid_blocks, int waves_per_block, hipStream_t stream);
+                     int grid_blocks, int waves_per_block, double* jac_scratch, hipStream_t stream);
+size_t bdf_lds_bytes(const DevMech& m, int waves_per_block);
+
+}  // namespace emx
+
+namespace emx {
+
+size_t plain_workspace_doubles(const DevMech& m, int nPoints);
+int launch_qss_plain(const DevMech& m, const DevOpts& o, const double* state_in,
+                     const double* split_const, const double* qloss, const double* perturb,
+                     double dt, int nPoints, double* state_out, int* status, int* stat1, int* stat2,
+                     double* ws, hipStream_t stream);
 
 }  // namespace emx

@@ -107,13 +107,14 @@ def _smooth(rng, n, npts, lo, hi):
 
 def diffusion_case(n_vars: int = 53, n_points: int = 65536, seed: int = 4,
                    length: float = 0.02) -> DiffusionCase:
-    """Species diffusion systems on a non-uniform grid (spacing ratio <= 1.5)."""
+    """Species diffusion systems on a smooth non-uniform grid: log(spacing) follows a
+    seeded smooth profile spanning a factor of 8, so the ratio of neighbouring spacings
+    stays within [1/1.5, 1.5] (in fact within 1 +- 1e-3 for large N)."""
     rng = np.random.default_rng(seed)
-    ratio = np.exp(np.convolve(rng.uniform(np.log(1 / 1.5), np.log(1.5), n_points - 1),
-                               np.ones(64) / 64, mode="same"))
-    ratio = np.clip(ratio, 1 / 1.5, 1.5)
-    hh = np.cumprod(ratio)
+    log_h = _smooth(rng, n_points - 1, int(np.clip(n_points // 16, 2, 12)), np.log(1.0 / 8.0), 0.0)
+    hh = np.exp(log_h)
     hh = hh / hh.sum() * length
+    assert (hh[1:] / hh[:-1]).max() <= 1.5 and (hh[1:] / hh[:-1]).min() >= 1 / 1.5
     x = np.concatenate([[0.0], np.cumsum(hh)])
     s = (x - 0.5 * length) / (0.05 * length)
     T = 300.0 + 1900.0 * 0.5 * (1 + np.tanh(s))
@@ -128,3 +129,47 @@ def diffusion_case(n_vars: int = 53, n_points: int = 65536, seed: int = 4,
         w = rng.uniform(0.02, 0.2)
         y0[:, k] = amp * 0.5 * (1 + np.tanh((x / length - rng.uniform(0.3, 0.7)) / w))
     return DiffusionCase(x, B, np.zeros(n_vars, np.int32), D, y0, np.zeros_like(y0))
+
+
+@dataclass
+class FlameCase:
+    x: np.ndarray        # (N,)
+    state: np.ndarray    # (N, nVars)
+    B: np.ndarray        # (2, N): 1/rho, 1/(rho cp)
+    B_row: np.ndarray    # (nVars,)
+    D: np.ndarray        # (N, nVars): mu, lambda, rho D_k
+
+
+def flame_profile(n_points: int = 150, seed: int = 1, name: str = "h2o2",
+                  width: float = 0.01, delta: float = 5e-4) -> FlameCase:
+    """Premixed-flame-like profile (tanh in T between the unburned mixture at 300 K and
+    its adiabatic equilibrium) with frozen seeded transport coefficients."""
+    mech = mechanism.load(name)
+    rng = np.random.default_rng(seed)
+    u = np.linspace(-1.0, 1.0, n_points)
+    x = 0.5 * width * np.sinh(2.5 * u) / np.sinh(2.5)   # finer spacing at the centre
+    Tu, Yu = unburned(mech, name)
+    Tb, Yb = burned(mech, name)
+    c = 0.5 * (1 + np.tanh(x / delta))
+    T = Tu + c * (Tb - Tu)
+    Y = (1 - c)[:, None] * Yu + c[:, None] * Yb
+    K = mech.n_species
+    state = np.zeros((n_points, K + 2))
+    state[:, 0] = 0.0
+    state[:, 1] = T
+    state[:, 2:] = Y
+    Wmix = 1.0 / (Y / mech.W).sum(axis=1)
+    rho = mechanism.ONE_ATM * Wmix / (mechanism.GAS_CONSTANT * T)
+    cp = np.array([(mech.cp_R(t) * y / mech.W).sum() * mechanism.GAS_CONSTANT for t, y in zip(T, Y)])
+    B = np.vstack([1.0 / rho, 1.0 / (rho * cp)])
+    B_row = np.zeros(K + 2, np.int32)
+    B_row[1] = 1
+    D = np.empty((n_points, K + 2))
+    D[:, 0] = 1.8e-5 * (T / 300.0) ** 0.7
+    D[:, 1] = 0.026 * (T / 300.0) ** 0.8
+    D0 = rng.uniform(1e-5, 1e-4, K)
+    for k, s in enumerate(mech.species):
+        if s in ("H", "H2"):
+            D0[k] *= 4.0
+        D[:, 2 + k] = rho * D0[k] * (T / 300.0) ** 1.75
+    return FlameCase(x, state, B, B_row, D)

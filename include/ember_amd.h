@@ -91,6 +91,13 @@ int emx_set_stream(emx_context* ctx, void* stream);
  * (0 = built-in default).  The autotuner in ember_amd.autotune picks these. */
 int emx_set_launch(emx_context* ctx, int waves_per_block, int blocks_per_cu);
 
+/* Chemistry kernel variant for the *_integrate entry points.  EMX_VARIANT_WAVE (default):
+ * one wavefront per point, persistent work queue.  EMX_VARIANT_PLAIN: one thread per point
+ * running the serial QssIntegrator loop (QSS only), kept as the plain-GPU comparison
+ * baseline for benchmarks; it has no counterpart in the reference. */
+enum { EMX_VARIANT_WAVE = 0, EMX_VARIANT_PLAIN = 1 };
+int emx_set_kernel_variant(emx_context* ctx, int variant);
+
 /* Chemistry production stage over [t_stage_start, t_stage_start + dt] for all points:
  * FlameSolver::integrateProductionTerms (src/flameSolver.cpp:754-836) with
  * SourceSystemQSS or SourceSystemCVODE per point.
