@@ -27,6 +27,7 @@ def parse():
     p.add_argument("--kinetics", choices=["standard", "interp"], default="standard")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--no-plain", action="store_true", help="skip the plain thread-per-point kernel")
     return p.parse_args()
 
 
@@ -80,6 +81,23 @@ def main():
     g = s1.cpu().numpy().astype(np.float64)
     r = s2.cpu().numpy().astype(np.float64)
 
+    if not a.no_plain and integ == capi.EMX_QSS:
+        # plain GPU port (one thread per point) on the same states, same stream
+        ctx.set_kernel_variant(capi.EMX_VARIANT_PLAIN)
+        ctx.source_integrate_device(d_in, d_out, dt, status=status, stat1=s1, stat2=s2)
+        torch.cuda.synchronize()
+        np_steps = max(1, min(a.steps, 3))
+        e0.record(stream)
+        for _ in range(np_steps):
+            ctx.source_integrate_device(d_in, d_out, dt, status=status, stat1=s1, stat2=s2)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_plain = e0.elapsed_time(e1) / np_steps
+        ctx.set_kernel_variant(capi.EMX_VARIANT_WAVE)
+        print(f"[bench] plain kernel: {ms_plain:.3f} ms/stage", file=sys.stderr, flush=True)
+    else:
+        ms_plain = None
+
     mech = mechanism.load("gri30")
     fl = flops.rhs_flops(mech)
     res = {
@@ -92,6 +110,9 @@ def main():
                    "pressure_Pa": mechanism.ONE_ATM, "seed": 2},
         "failed_points": int((st != 0).sum()),
     }
+    if ms_plain:
+        res["plain_gpu"] = {"value": N / (ms_plain * 1e-3), "ms_per_step": ms_plain,
+                            "speedup": ms_plain / ms}
 
     cpu = None
     if not a.no_cpu_baseline:
