@@ -122,6 +122,8 @@ int upload_mech(emx_context* c) {
     d.nGroups = L.nGroups;
     d.nChunks = L.nChunks;
     d.nChunkRounds = L.nChunkRounds;
+    d.nEntries = (int)L.entry.size();
+    d.nGrpEntries = (int)L.grp_sp.size();
     d.interp = c->opts.kinetics_model == EMX_KIN_INTERP ? 1 : 0;
     d.P = c->P;
     d.Pref = m.Pref;
@@ -175,8 +177,7 @@ int check_ctx(emx_context* c) {
 
 int grid_blocks(emx_context* c, int nPoints, bool bdf) {
     const int wpb = c->waves_per_block;
-    size_t lds_per_block = bdf ? bdf_lds_bytes(c->dm, wpb)
-                               : (size_t)wpb * (3 * 64 + 2 * c->dm.nRounds * 64 + 2 * c->dm.nChunkRounds * 64) * 8;
+    size_t lds_per_block = bdf ? bdf_lds_bytes(c->dm, wpb) : qss_lds_bytes(c->dm, wpb);
     int per_cu = c->blocks_per_cu;
     if (per_cu <= 0) {
         per_cu = (int)std::max<size_t>(1, (160 * 1024) / std::max<size_t>(lds_per_block, 1));

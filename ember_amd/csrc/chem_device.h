@@ -49,6 +49,59 @@ __device__ inline WaveLds carve_lds(double* base, int nRounds, int nChunkRounds)
     return w;
 }
 
+// ---------------------------------------------------------------------------
+// Mechanism tables staged in LDS once per workgroup (read on every right-hand side):
+// per-slot info/stoichiometry/multipliers, species data, third-body groups and the
+// species-sum entries.  Arrhenius and falloff parameters (used once per step) stay in
+// global memory.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int mech_lds_doubles(const DevMech& m) {
+    const int S = m.nRounds * 64;
+    int n = S + 3 * m.K + 14 * m.K + m.nGrpEntries + (m.nGroups > 0 ? m.nGroups : 1);
+    int ints = 3 * S + m.nEntries + (m.nChunks + 1) + 2 * m.K + (m.nGroups + 1) + m.nGrpEntries;
+    return n + (ints + 1) / 2;
+}
+
+// Copies the tables into `base` (block-cooperative; the caller synchronises the block)
+// and returns a view whose staged pointers refer to LDS.
+__device__ inline DevMech stage_mech(const DevMech& m, const double* perturb, double* base,
+                                     const double*& perturb_lds) {
+    DevMech v = m;
+    const int S = m.nRounds * 64;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double* p = base;
+    auto cpd = [&](const double* src, int n) {
+        for (int i = tid; i < n; i += nt) p[i] = src[i];
+        const double* r = p;
+        p += n;
+        return r;
+    };
+    perturb_lds = cpd(perturb, S);
+    v.W = cpd(m.W, m.K);
+    v.rW = cpd(m.rW, m.K);
+    v.Tmid = cpd(m.Tmid, m.K);
+    v.nasa = cpd(m.nasa, 14 * m.K);
+    v.grp_eff = cpd(m.grp_eff, m.nGrpEntries);
+    v.grp_def = cpd(m.grp_def, m.nGroups > 0 ? m.nGroups : 1);
+    int32_t* q = reinterpret_cast<int32_t*>(p);
+    auto cpi = [&](const int32_t* src, int n) {
+        for (int i = tid; i < n; i += nt) q[i] = src[i];
+        const int32_t* r = q;
+        q += n;
+        return r;
+    };
+    v.info = cpi(m.info, S);
+    v.reac = reinterpret_cast<const uint32_t*>(cpi(reinterpret_cast<const int32_t*>(m.reac), S));
+    v.prod = reinterpret_cast<const uint32_t*>(cpi(reinterpret_cast<const int32_t*>(m.prod), S));
+    v.entry = reinterpret_cast<const uint32_t*>(cpi(reinterpret_cast<const int32_t*>(m.entry), m.nEntries));
+    v.chunk_off = cpi(m.chunk_off, m.nChunks + 1);
+    v.sp_chunk0 = cpi(m.sp_chunk0, m.K);
+    v.sp_nchunk = cpi(m.sp_nchunk, m.K);
+    v.grp_off = cpi(m.grp_off, m.nGroups + 1);
+    v.grp_sp = cpi(m.grp_sp, m.nGrpEntries);
+    return v;
+}
+
 // T-dependent reaction data held in registers (one set per owned slot).
 // NR = compile-time bound on reaction rounds (>= DevMech::nRounds).
 template <int NR>

@@ -44,6 +44,8 @@ struct DevMech {
     int nGroups;           // distinct third-body efficiency sets
     int nChunks;           // species-sum chunks
     int nChunkRounds;
+    int nEntries;          // species-sum entries
+    int nGrpEntries;       // third-body efficiency entries
     int interp;            // 1: InterpKinetics tables
     double P, Pref;
     // species [K]

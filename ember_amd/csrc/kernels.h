@@ -54,6 +54,7 @@ int launch_bdf_stage(const DevMech& m, const DevOpts& o, const double* state_in,
                      double* state_out, int* status, int* stat1, int* stat2, double* qint,
                      int grid_blocks, int waves_per_block, double* jac_scratch, hipStream_t stream);
 size_t bdf_lds_bytes(const DevMech& m, int waves_per_block);
+size_t qss_lds_bytes(const DevMech& m, int waves_per_block);
 
 }  // namespace emx
 

@@ -218,16 +218,19 @@ __device__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
 
 template <int NR, bool INTERP>
 __global__ void __launch_bounds__(256, 2) qss_stage_kernel(
-    DevMech m, DevOpts o, const double* __restrict__ state_in,
+    DevMech mg, DevOpts o, const double* __restrict__ state_in,
     const double* __restrict__ split_const, const double* __restrict__ qloss,
-    const double* __restrict__ perturb, double dt, int nPoints, const int* __restrict__ order,
+    const double* __restrict__ perturb_g, double dt, int nPoints, const int* __restrict__ order,
     unsigned int* __restrict__ queue, double* __restrict__ state_out, int* __restrict__ status,
     int* __restrict__ stat1, int* __restrict__ stat2, double* __restrict__ qint_out) {
     extern __shared__ double lds[];
+    const double* perturb;
+    const DevMech m = stage_mech(mg, perturb_g, lds, perturb);
+    __syncthreads();
     const int lane = lane_id();
     const int wave = threadIdx.x / 64;
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
-    WaveLds L = carve_lds(lds + wave * per_wave, m.nRounds, m.nChunkRounds);
+    WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
     const int nv = m.nVars;
     const bool active = lane < nv;
     QssSystem S;
@@ -271,16 +274,19 @@ __global__ void reset_queue_kernel(unsigned int* q) {
 
 // Rates probe: creation / destruction / net production rates at given states
 template <int NR, bool INTERP>
-__global__ void __launch_bounds__(256) rates_kernel(DevMech m, const double* __restrict__ T,
+__global__ void __launch_bounds__(256) rates_kernel(DevMech mg, const double* __restrict__ T,
                                                     const double* __restrict__ Y, int nonorm,
-                                                    const double* __restrict__ perturb, int n,
+                                                    const double* __restrict__ perturb_g, int n,
                                                     double* wQ_out, double* wD_out,
                                                     double* wnet_out, int* status) {
     extern __shared__ double lds[];
+    const double* perturb;
+    const DevMech m = stage_mech(mg, perturb_g, lds, perturb);
+    __syncthreads();
     const int lane = lane_id();
     const int wave = threadIdx.x / 64;
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
-    WaveLds L = carve_lds(lds + wave * per_wave, m.nRounds, m.nChunkRounds);
+    WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
     const int p = blockIdx.x * (blockDim.x / 64) + wave;
     if (p >= n) return;
     const int K = m.K;
@@ -315,12 +321,16 @@ __global__ void __launch_bounds__(256) rates_kernel(DevMech m, const double* __r
         default: { constexpr int NR = 8; BODY; } break; \
     }
 
+size_t qss_lds_bytes(const DevMech& m, int waves_per_block) {
+    return ((size_t)mech_lds_doubles(m) + (size_t)waves_per_block * wave_lds_doubles(m.nRounds, m.nChunkRounds)) * sizeof(double);
+}
+
 int launch_qss_stage(const DevMech& m, const DevOpts& o, const double* state_in,
                      const double* split_const, const double* qloss, const double* perturb,
                      double dt, int nPoints, const int* order, unsigned int* queue,
                      double* state_out, int* status, int* stat1, int* stat2, double* qint,
                      int grid_blocks, int waves_per_block, hipStream_t stream) {
-    size_t lds = (size_t)waves_per_block * wave_lds_doubles(m.nRounds, m.nChunkRounds) * sizeof(double);
+    size_t lds = ((size_t)mech_lds_doubles(m) + (size_t)waves_per_block * wave_lds_doubles(m.nRounds, m.nChunkRounds)) * sizeof(double);
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, stream, queue);
     EMX_DISPATCH_NR(m.nRounds, {
         if (m.interp)
@@ -339,7 +349,7 @@ int launch_rates(const DevMech& m, const double* T, const double* Y, int nonorm,
                  const double* perturb, int n, double* wQ, double* wD, double* wnet, int* status,
                  hipStream_t stream) {
     const int wpb = 4;
-    size_t lds = (size_t)wpb * wave_lds_doubles(m.nRounds, m.nChunkRounds) * sizeof(double);
+    size_t lds = ((size_t)mech_lds_doubles(m) + (size_t)wpb * wave_lds_doubles(m.nRounds, m.nChunkRounds)) * sizeof(double);
     int blocks = (n + wpb - 1) / wpb;
     EMX_DISPATCH_NR(m.nRounds, {
         if (m.interp)

@@ -3,39 +3,62 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 namespace emx {
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-// All 64 lanes receive the same (bitwise identical) sum: a xor butterfly adds
-// the same pair of operands on both partners at every level.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+// ---- DPP data movement on 64-bit values (two 32-bit halves moved with one pattern).
+// `identity` is written where the pattern has no source lane.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v, double identity) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t o = __builtin_bit_cast(uint64_t, identity);
+    const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)o, (int)(uint32_t)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(o >> 32), (int)(uint32_t)(b >> 32), CTRL,
+                                               0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
+// value of `v` held by lane `src` (src uniform across the wave): v_readlane
+__device__ __forceinline__ double bcast(double v, int src) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), src);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ int bcast_i(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
+
+// Reductions over all 64 lanes: quad permutes, row shifts and row broadcasts leave the
+// total in lane 63, which is then read into a scalar, so every lane receives the same
+// bits.  The combination order is fixed (independent of the data).
+template <class Op>
+__device__ __forceinline__ double wave_reduce(double v, double identity, Op op) {
+    v = op(v, dpp_move<0xB1>(v, identity));   // quad_perm [1,0,3,2]
+    v = op(v, dpp_move<0x4E>(v, identity));   // quad_perm [2,3,0,1]
+    v = op(v, dpp_move<0x114>(v, identity));  // row_shr:4
+    v = op(v, dpp_move<0x118>(v, identity));  // row_shr:8
+    v = op(v, dpp_move<0x142>(v, identity));  // row_bcast:15
+    v = op(v, dpp_move<0x143>(v, identity));  // row_bcast:31
+    return bcast(v, 63);
 }
 
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
+struct AddOp {
+    __device__ __forceinline__ double operator()(double a, double b) const { return a + b; }
+};
+struct MaxOp {
+    __device__ __forceinline__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+struct MinOp {
+    __device__ __forceinline__ double operator()(double a, double b) const { return fmin(a, b); }
+};
 
-__device__ __forceinline__ int wave_or(int v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o, 64);
-    return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return wave_reduce(v, 0.0, AddOp()); }
+__device__ __forceinline__ double wave_max(double v) { return wave_reduce(v, -__builtin_huge_val(), MaxOp()); }
+__device__ __forceinline__ double wave_min(double v) { return wave_reduce(v, __builtin_huge_val(), MinOp()); }
 
-// value of `v` held by lane `src` (src uniform across the wave)
-__device__ __forceinline__ double bcast(double v, int src) { return __shfl(v, src, 64); }
-__device__ __forceinline__ int bcast_i(int v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ int wave_or(int v) { return __ballot(v != 0) != 0ull; }
 
 // Order LDS traffic between lanes of one wavefront: wait for this wave's LDS
 // operations and keep the compiler from moving LDS accesses across the point.

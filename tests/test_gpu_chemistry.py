@@ -22,11 +22,11 @@ def _opts(O, kinetics=0, integrator=0, **kw):
     return O.options(kinetics=kinetics, integrator=integrator, **kw)
 
 
-def _assert_state_close(got, ref, label=""):
+def _assert_state_close(got, ref, label="", atol_y=ATOL_Y):
     dT = np.abs(got[:, 1] - ref[:, 1])
     assert (dT <= RTOL * np.abs(ref[:, 1])).all(), (label, dT.max())
     dY = np.abs(got[:, 2:] - ref[:, 2:])
-    bad = dY > RTOL * np.abs(ref[:, 2:]) + ATOL_Y
+    bad = dY > RTOL * np.abs(ref[:, 2:]) + atol_y
     assert not bad.any(), (label, dY[bad].max(), np.argwhere(bad)[:5])
     np.testing.assert_allclose(got[:, 0], ref[:, 0], rtol=RTOL, atol=1e-12)
 
@@ -137,7 +137,10 @@ def test_bdf_stage_parity(gpu, oracle_mod, name, n):
     om = oracle_mod.Mech(mechanism.table_path(name))
     ref, rstat, rnst, rnfe, _ = om.integrate(st, 2e-5, _opts(oracle_mod, integrator=1, atol_species=1e-12))
     assert (status == 0).all() and (rstat == 0).all()
-    _assert_state_close(out, ref, f"bdf/{name}")
+    # floor = 10x the species absolute tolerance the integrator was given (1e-12): below
+    # it the BDF error control does not define Y, and ULP-level differences between the
+    # two implementations move accept/reject decisions
+    _assert_state_close(out, ref, f"bdf/{name}", atol_y=1e-11)
     assert np.mean(nst == rnst) > 0.8
 
 
