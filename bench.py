@@ -28,6 +28,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--no-plain", action="store_true", help="skip the plain thread-per-point kernel")
+    p.add_argument("--wpb", type=int, default=0, help="waves per workgroup (0: library default)")
+    p.add_argument("--bpc", type=int, default=0, help="workgroups per CU (0: as many as fit)")
+    p.add_argument("--schedule", choices=["cost", "fifo"], default="cost")
     return p.parse_args()
 
 
@@ -54,26 +57,46 @@ def main():
     ctx = capi.Context("gri30", **opts)
     stream = torch.cuda.current_stream(dev)
     ctx.set_stream(stream.cuda_stream)
+    if a.wpb or a.bpc:
+        ctx.set_launch(a.wpb, a.bpc)
     d_in = torch.from_numpy(states).to(dev)
     d_out = torch.empty_like(d_in)
     status = torch.zeros(N, dtype=torch.int32, device=dev)
     s1 = torch.zeros(N, dtype=torch.int32, device=dev)
     s2 = torch.zeros(N, dtype=torch.int32, device=dev)
     dt = 2e-5
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+
+    def timed(steps):
+        e0.record(stream)
+        for _ in range(steps):
+            ctx.source_integrate_device(d_in, d_out, dt, status=status, stat1=s1, stat2=s2)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    # index-order work queue (what a first stage over new points gets)
+    ctx.set_schedule(capi.EMX_SCHED_FIFO)
     print(f"[bench] warmup {a.warmup} stages, N={N}, {a.integrator}/{a.kinetics}", file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         ctx.source_integrate_device(d_in, d_out, dt, status=status, stat1=s1, stat2=s2)
     torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
+    ms_fifo = timed(a.steps)
+    print(f"[bench] index-order queue: {ms_fifo:.3f} ms/stage", file=sys.stderr, flush=True)
+    # cost-ordered queue: each stage takes the points in descending order of the previous
+    # stage's right-hand-side evaluation counts
+    ms = ms_fifo
     t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(a.steps):
+    wall = 0.0
+    if a.schedule == "cost":
+        ctx.set_schedule(capi.EMX_SCHED_COST)
         ctx.source_integrate_device(d_in, d_out, dt, status=status, stat1=s1, stat2=s2)
-    e1.record(stream)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ms = timed(a.steps)
+        print(f"[bench] cost-ordered queue: {ms:.3f} ms/stage", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1) / a.steps
     print(f"[bench] timed {a.steps} stages: {ms:.3f} ms/stage", file=sys.stderr, flush=True)
     value = N / (ms * 1e-3)
     out = d_out.cpu().numpy()
@@ -87,12 +110,7 @@ def main():
         ctx.source_integrate_device(d_in, d_out, dt, status=status, stat1=s1, stat2=s2)
         torch.cuda.synchronize()
         np_steps = max(1, min(a.steps, 3))
-        e0.record(stream)
-        for _ in range(np_steps):
-            ctx.source_integrate_device(d_in, d_out, dt, status=status, stat1=s1, stat2=s2)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms_plain = e0.elapsed_time(e1) / np_steps
+        ms_plain = timed(np_steps)
         ctx.set_kernel_variant(capi.EMX_VARIANT_WAVE)
         print(f"[bench] plain kernel: {ms_plain:.3f} ms/stage", file=sys.stderr, flush=True)
     else:
@@ -109,6 +127,7 @@ def main():
                    "mechanism": "gri30 (53 species, 325 reactions)", "points": N, "dt_s": dt,
                    "pressure_Pa": mechanism.ONE_ATM, "seed": 2},
         "failed_points": int((st != 0).sum()),
+        "schedule": a.schedule, "index_order_ms_per_step": ms_fifo,
     }
     if ms_plain:
         res["plain_gpu"] = {"value": N / (ms_plain * 1e-3), "ms_per_step": ms_plain,

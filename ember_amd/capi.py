@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libember_am
 EMX_QSS, EMX_BDF = 0, 1
 EMX_KIN_STANDARD, EMX_KIN_INTERP = 0, 1
 EMX_VARIANT_WAVE, EMX_VARIANT_PLAIN = 0, 1
+EMX_SCHED_FIFO, EMX_SCHED_COST = 0, 1
 BC_FIXED_VALUE, BC_ZERO_GRADIENT, BC_CONTROL_VOLUME, BC_WALL_FLUX = 0, 1, 2, 3
 
 _dp = C.POINTER(C.c_double)
@@ -58,6 +59,7 @@ SIGNATURES = [
     ("emx_set_stream", C.c_int, [_vp, _vp]),
     ("emx_set_launch", C.c_int, [_vp, C.c_int, C.c_int]),
     ("emx_set_kernel_variant", C.c_int, [_vp, C.c_int]),
+    ("emx_set_schedule", C.c_int, [_vp, C.c_int]),
     ("emx_source_integrate", C.c_int,
      [_vp, _sz, _sz, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _ip, _ip, _ip]),
     ("emx_source_integrate_device", C.c_int,
@@ -166,6 +168,11 @@ class Context:
     def set_kernel_variant(self, variant: int):
         """0: wavefront-per-point kernel (default), 1: plain thread-per-point baseline."""
         _check(lib().emx_set_kernel_variant(self.h, variant))
+
+    def set_schedule(self, mode: int):
+        """EMX_SCHED_COST (default): most expensive points of the previous stage first;
+        EMX_SCHED_FIFO: index order."""
+        _check(lib().emx_set_schedule(self.h, mode))
 
     # ---- chemistry
     def source_integrate(self, state, dt, split_const=None, qloss=None, t_start=0.0):

@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(256) bdf_stage_kernel(
     const double* __restrict__ qloss, const double* __restrict__ perturb, double dt, int nPoints,
     const int* __restrict__ order, unsigned int* __restrict__ queue, double* __restrict__ state_out,
     int* __restrict__ status, int* __restrict__ stat1, int* __restrict__ stat2,
-    double* __restrict__ qint_out, double* __restrict__ jac_scratch) {
+    int* __restrict__ cost, double* __restrict__ qint_out, double* __restrict__ jac_scratch) {
     extern __shared__ double lds[];
     const int lane = lane_id();
     const int wave = threadIdx.x / 64;
@@ -377,6 +377,7 @@ __global__ void __launch_bounds__(256) bdf_stage_kernel(
     const int per_wave = chem + nv * 64 + 64;
     double* base = lds + wave * per_wave;
     WaveLds L = carve_lds(base, m.nRounds, m.nChunkRounds);
+    init_wave_lds(L, lane, m.nRounds);
     double* M = base + chem;
     double* ru = M + nv * 64;
     double* J = jac_scratch + (size_t)(blockIdx.x * wpb + wave) * 64 * 64;
@@ -406,6 +407,7 @@ __global__ void __launch_bounds__(256) bdf_stage_kernel(
             if (status) status[p] = err;
             if (stat1) stat1[p] = nsteps;
             if (stat2) stat2[p] = S.nfe;
+            if (cost) cost[p] = S.nfe;
         }
     }
 }
@@ -424,7 +426,7 @@ size_t bdf_lds_bytes(const DevMech& m, int waves_per_block) {
 int launch_bdf_stage(const DevMech& m, const DevOpts& o, const double* state_in,
                      const double* split_const, const double* qloss, const double* perturb,
                      double dt, int nPoints, const int* order, unsigned int* queue,
-                     double* state_out, int* status, int* stat1, int* stat2, double* qint,
+                     double* state_out, int* status, int* stat1, int* stat2, int* cost, double* qint,
                      int grid_blocks, int waves_per_block, double* jac_scratch, hipStream_t stream) {
     const size_t lds = bdf_lds_bytes(m, waves_per_block);
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, stream, queue);
@@ -433,12 +435,12 @@ int launch_bdf_stage(const DevMech& m, const DevOpts& o, const double* state_in,
         hipLaunchKernelGGL((bdf_stage_kernel<NRV, true>), dim3(grid_blocks),                    \
                            dim3(64 * waves_per_block), lds, stream, m, o, state_in, split_const, \
                            qloss, perturb, dt, nPoints, order, queue, state_out, status, stat1,  \
-                           stat2, qint, jac_scratch);                                           \
+                           stat2, cost, qint, jac_scratch);                                           \
     else                                                                                        \
         hipLaunchKernelGGL((bdf_stage_kernel<NRV, false>), dim3(grid_blocks),                   \
                            dim3(64 * waves_per_block), lds, stream, m, o, state_in, split_const, \
                            qloss, perturb, dt, nPoints, order, queue, state_out, status, stat1,  \
-                           stat2, qint, jac_scratch);
+                           stat2, cost, qint, jac_scratch);
     if (m.nRounds <= 1) { EMX_BDF_LAUNCH(1) }
     else if (m.nRounds <= 2) { EMX_BDF_LAUNCH(2) }
     else if (m.nRounds <= 4) { EMX_BDF_LAUNCH(4) }

@@ -74,6 +74,10 @@ struct emx_context {
     DevBuf t_a, t_b, t_c, t_k;
     DevBuf jac, plain_ws;
     int variant = EMX_VARIANT_WAVE;
+    // cost-ordered scheduling: per-point cost of the last stage and the order built from it
+    int schedule = EMX_SCHED_COST;
+    DevBuf cost, order;
+    size_t cost_n = 0;  // number of points the recorded costs belong to (0: none)
 };
 
 namespace {
@@ -153,7 +157,6 @@ int upload_mech(emx_context* c) {
     pk.add(L.grp_sp, &d.grp_sp);
     pk.add(L.grp_eff, &d.grp_eff);
     pk.add(L.grp_def, &d.grp_def);
-    pk.add(L.chunk_off, &d.chunk_off);
     pk.add(L.entry, &d.entry);
     pk.add(L.sp_chunk0, &d.sp_chunk0);
     pk.add(L.sp_nchunk, &d.sp_nchunk);
@@ -256,7 +259,8 @@ void emx_destroy(emx_context* c) {
     if (c->queue) (void)hipFree(c->queue);
     for (DevBuf* b : {&c->b_in, &c->b_out, &c->b_sc, &c->b_ql, &c->b_status, &c->b_s1, &c->b_s2,
                       &c->d_B, &c->d_Brow, &c->d_D, &c->d_sc, &c->d_x, &c->d_y, &c->d_h, &c->d_ns,
-                      &c->d_work, &c->d_hmin, &c->t_a, &c->t_b, &c->t_c, &c->t_k, &c->jac, &c->plain_ws})
+                      &c->d_work, &c->d_hmin, &c->t_a, &c->t_b, &c->t_c, &c->t_k, &c->jac, &c->plain_ws,
+                      &c->cost, &c->order})
         b->release();
     delete c;
 }
@@ -305,6 +309,13 @@ int emx_set_kernel_variant(emx_context* c, int variant) {
     return 0;
 }
 
+int emx_set_schedule(emx_context* c, int mode) {
+    if (!c) return fail(-1, "null context");
+    if (mode != EMX_SCHED_FIFO && mode != EMX_SCHED_COST) return fail(-1, "unknown schedule");
+    c->schedule = mode;
+    return 0;
+}
+
 int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, const double* state_in,
                                 const double* split_const, const double* qloss, double t_stage_start,
                                 double dt, double* state_out, int32_t* status, int32_t* stat1,
@@ -325,21 +336,34 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
         if (rc) return fail(-10, std::string("plain launch: ") + hipGetErrorString((hipError_t)rc));
         return 0;
     }
+    // work queue order: descending cost of the previous stage over the same points
+    HIP_TRY(c->cost.ensure(nPoints * sizeof(int)));
+    HIP_TRY(c->order.ensure(nPoints * sizeof(int)));
+    const int* order = nullptr;
+    if (c->schedule == EMX_SCHED_COST && c->cost_n == nPoints) {
+        int rc = launch_cost_order(c->cost.as<int>(), (int)nPoints, c->order.as<int>(), c->stream);
+        if (rc) return fail(-10, std::string("order launch: ") + hipGetErrorString((hipError_t)rc));
+        order = c->order.as<int>();
+    }
     if (c->opts.chemistry_integrator == EMX_BDF) {
         if (bdf_lds_bytes(c->dm, c->waves_per_block) > 160 * 1024)
             return fail(-1, "BDF workgroup needs more than 160 KB of LDS; use fewer waves per block");
         const int blocks = grid_blocks(c, (int)nPoints, true);
         HIP_TRY(c->jac.ensure((size_t)blocks * c->waves_per_block * 64 * 64 * sizeof(double)));
         int rc = launch_bdf_stage(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
-                                  (int)nPoints, nullptr, c->queue, state_out, status, stat1, stat2,
-                                  qint, blocks, c->waves_per_block, c->jac.as<double>(), c->stream);
+                                  (int)nPoints, order, c->queue, state_out, status, stat1, stat2,
+                                  c->cost.as<int>(), qint, blocks, c->waves_per_block,
+                                  c->jac.as<double>(), c->stream);
         if (rc) return fail(-10, std::string("bdf launch: ") + hipGetErrorString((hipError_t)rc));
+        c->cost_n = nPoints;
         return 0;
     }
     int rc = launch_qss_stage(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
-                              (int)nPoints, nullptr, c->queue, state_out, status, stat1, stat2, qint,
-                              grid_blocks(c, (int)nPoints, false), c->waves_per_block, c->stream);
+                              (int)nPoints, order, c->queue, state_out, status, stat1, stat2,
+                              c->cost.as<int>(), qint, grid_blocks(c, (int)nPoints, false),
+                              c->waves_per_block, c->stream);
     if (rc) return fail(-10, std::string("qss launch: ") + hipGetErrorString((hipError_t)rc));
+    c->cost_n = nPoints;
     return 0;
 }
 

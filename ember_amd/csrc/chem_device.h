@@ -27,14 +27,14 @@ struct WaveLds {
     double* C;      // [64] concentrations
     double* mu;     // [64] standard chemical potentials
     double* Mg;     // [64] third-body concentrations per efficiency group
-    double* ropf;   // [nRounds*64]
-    double* ropr;   // [nRounds*64]
+    double* ropf;   // [nRounds*64 + 1]: the last entry is the zero slot
+    double* ropr;   // [nRounds*64 + 1]
     double* partq;  // [nChunkRounds*64]
     double* partd;  // [nChunkRounds*64]
 };
 
 __host__ __device__ inline int wave_lds_doubles(int nRounds, int nChunkRounds) {
-    return 3 * 64 + 2 * nRounds * 64 + 2 * nChunkRounds * 64;
+    return 3 * 64 + 2 * (nRounds * 64 + 1) + 2 * nChunkRounds * 64;
 }
 
 __device__ inline WaveLds carve_lds(double* base, int nRounds, int nChunkRounds) {
@@ -43,10 +43,23 @@ __device__ inline WaveLds carve_lds(double* base, int nRounds, int nChunkRounds)
     w.mu = base + 64;
     w.Mg = base + 128;
     w.ropf = base + 192;
-    w.ropr = w.ropf + nRounds * 64;
-    w.partq = w.ropr + nRounds * 64;
+    w.ropr = w.ropf + nRounds * 64 + 1;
+    w.partq = w.ropr + nRounds * 64 + 1;
     w.partd = w.partq + nChunkRounds * 64;
     return w;
+}
+
+// Constant cells of the per-wave scratch: C[kPadSpecies] = 1 and mu[kPadSpecies] = 0
+// (stoichiometry padding), ropf/ropr zero slot (species-sum padding).  Species lanes
+// never write index kPadSpecies since K <= 62.
+__device__ inline void init_wave_lds(const WaveLds& L, int lane, int nRounds) {
+    if (lane == 0) {
+        L.C[kPadSpecies] = 1.0;
+        L.mu[kPadSpecies] = 0.0;
+        L.ropf[nRounds * 64] = 0.0;
+        L.ropr[nRounds * 64] = 0.0;
+    }
+    wave_lds_sync();
 }
 
 // ---------------------------------------------------------------------------
@@ -58,7 +71,7 @@ __device__ inline WaveLds carve_lds(double* base, int nRounds, int nChunkRounds)
 __host__ __device__ inline int mech_lds_doubles(const DevMech& m) {
     const int S = m.nRounds * 64;
     int n = S + 3 * m.K + 14 * m.K + m.nGrpEntries + (m.nGroups > 0 ? m.nGroups : 1);
-    int ints = 3 * S + m.nEntries + (m.nChunks + 1) + 2 * m.K + (m.nGroups + 1) + m.nGrpEntries;
+    int ints = 3 * S + m.nEntries + 2 * m.K + (m.nGroups + 1) + m.nGrpEntries;
     return n + (ints + 1) / 2;
 }
 
@@ -94,7 +107,6 @@ __device__ inline DevMech stage_mech(const DevMech& m, const double* perturb, do
     v.reac = reinterpret_cast<const uint32_t*>(cpi(reinterpret_cast<const int32_t*>(m.reac), S));
     v.prod = reinterpret_cast<const uint32_t*>(cpi(reinterpret_cast<const int32_t*>(m.prod), S));
     v.entry = reinterpret_cast<const uint32_t*>(cpi(reinterpret_cast<const int32_t*>(m.entry), m.nEntries));
-    v.chunk_off = cpi(m.chunk_off, m.nChunks + 1);
     v.sp_chunk0 = cpi(m.sp_chunk0, m.K);
     v.sp_nchunk = cpi(m.sp_nchunk, m.K);
     v.grp_off = cpi(m.grp_off, m.nGroups + 1);
@@ -167,17 +179,13 @@ __device__ __forceinline__ double arrhenius(double A, double b, double EaR, doub
     return A * exp(b * logT - EaR * recipT);
 }
 
+// Products and sums over the (padded) three stoichiometry indices: the padding cells
+// (C = 1, mu = 0) leave the one- and two-species results unchanged bit for bit.
 __device__ __forceinline__ double conc_product(uint32_t st, const double* C) {
-    int n = st_n(st);
-    if (n == 1) return C[st_k(st, 0)];
-    if (n == 2) return C[st_k(st, 0)] * C[st_k(st, 1)];
     return C[st_k(st, 0)] * C[st_k(st, 1)] * C[st_k(st, 2)];
 }
 
 __device__ __forceinline__ double mu_sum(uint32_t st, const double* mu) {
-    int n = st_n(st);
-    if (n == 1) return mu[st_k(st, 0)];
-    if (n == 2) return mu[st_k(st, 0)] + mu[st_k(st, 1)];
     return mu[st_k(st, 0)] + mu[st_k(st, 1)] + mu[st_k(st, 2)];
 }
 
@@ -320,43 +328,43 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
     wave_lds_sync();
 }
 
-// Species creation / destruction rates: chunked sums, then per-species totals.
-// Returns (wQ, wD) for this lane's species (0 elsewhere).
+// Species creation / destruction rates: fixed-length chunk sums (chunk c = cr*64 + lane,
+// kChunkLen entries, padding entries add exact zeros), then per-species totals over the
+// species' consecutive chunks.  Returns (wQ, wD) for this lane's species (0 elsewhere).
 __device__ inline void species_rates(const DevMech& m, int lane, const WaveLds& L, double& wQ,
                                      double& wD) {
 #pragma unroll
     for (int cr = 0; cr < kMaxChunkRounds; cr++) {
         if (cr < m.nChunkRounds) {
-            int c = cr * 64 + lane;
-            if (c < m.nChunks) {
-                double q = 0.0, d = 0.0;
-                for (int e = m.chunk_off[c]; e < m.chunk_off[c + 1]; e++) {
-                    uint32_t en = m.entry[e];
-                    int s = en_slot(en);
-                    double nu = (double)en_nu(en);
-                    double f = L.ropf[s], r = L.ropr[s];
-                    if (en_prod(en)) {
-                        q += nu * f;
-                        d += nu * r;
-                    } else {
-                        q += nu * r;
-                        d += nu * f;
-                    }
-                }
-                L.partq[c] = q;
-                L.partd[c] = d;
+            const int c = cr * 64 + lane;
+            const uint32_t* ep = m.entry + c * kChunkLen;
+            uint32_t en[kChunkLen];
+#pragma unroll
+            for (int e = 0; e < kChunkLen; e++) en[e] = ep[e];
+            double q = 0.0, d = 0.0;
+#pragma unroll
+            for (int e = 0; e < kChunkLen; e++) {
+                const int s = en_slot(en[e]);
+                const double nu = (double)en_nu(en[e]);
+                const double f = L.ropf[s], r = L.ropr[s];
+                const bool pr = en_prod(en[e]);
+                q += nu * (pr ? f : r);
+                d += nu * (pr ? r : f);
             }
+            L.partq[c] = q;
+            L.partd[c] = d;
         }
     }
     wave_lds_sync();
     wQ = 0.0;
     wD = 0.0;
     if (is_species_lane(lane, m)) {
-        int k = lane - 2;
-        int c0 = m.sp_chunk0[k], nc = m.sp_nchunk[k];
-        for (int c = c0; c < c0 + nc; c++) {
-            wQ += L.partq[c];
-            wD += L.partd[c];
+        const int k = lane - 2;
+        const int c0 = m.sp_chunk0[k], nc = m.sp_nchunk[k];
+#pragma unroll 4
+        for (int i = 0; i < nc; i++) {
+            wQ += L.partq[c0 + i];
+            wD += L.partd[c0 + i];
         }
     }
     wave_lds_sync();

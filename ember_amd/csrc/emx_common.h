@@ -10,6 +10,8 @@ constexpr int kMaxRounds = 8;      // reactions per lane: up to 8 * 64 = 512 rea
 constexpr int kMaxFallRounds = 2;  // falloff reactions occupy the first rounds (<= 128)
 constexpr int kMaxVars = 64;       // nSpecies + 2 <= 64: one lane per state variable
 constexpr int kMaxChunkRounds = 4; // species-sum chunks: <= 256 chunks
+constexpr int kChunkLen = 8;       // entries per species-sum chunk (padded with zero entries)
+constexpr int kPadSpecies = 63;    // stoichiometry padding index: C[63] = 1, mu[63] = 0
 constexpr int kInterpN = 200;      // InterpKinetics default table size (src/chemistry0d.h:70-72)
 constexpr double kInterpTmin = 273.0;
 constexpr double kInterpTmax = 3500.0;
@@ -28,10 +30,12 @@ __host__ __device__ inline int info_group(int32_t w) { return (w >> 8) & 0xff; }
 __host__ __device__ inline int info_hasT2(int32_t w) { return (w >> 4) & 1; }
 
 // packed stoichiometry: byte0 = molecule count, bytes 1..3 = species indices
+// (unused index bytes hold kPadSpecies)
 __host__ __device__ inline int st_n(uint32_t s) { return s & 0xff; }
 __host__ __device__ inline int st_k(uint32_t s, int j) { return (s >> (8 * (j + 1))) & 0xff; }
 
-// species-sum entry: slot (bits 0..15), multiplicity (16..23), side (24: 1 = product)
+// species-sum entry: slot (bits 0..15), multiplicity (16..23), side (24: 1 = product);
+// padding entries have multiplicity 0 and point at the zero slot nRounds * 64
 __host__ __device__ inline int en_slot(uint32_t e) { return e & 0xffff; }
 __host__ __device__ inline int en_nu(uint32_t e) { return (e >> 16) & 0xff; }
 __host__ __device__ inline int en_prod(uint32_t e) { return (e >> 24) & 1; }
@@ -44,7 +48,7 @@ struct DevMech {
     int nGroups;           // distinct third-body efficiency sets
     int nChunks;           // species-sum chunks
     int nChunkRounds;
-    int nEntries;          // species-sum entries
+    int nEntries;          // species-sum entries, nChunkRounds * 64 * kChunkLen
     int nGrpEntries;       // third-body efficiency entries
     int interp;            // 1: InterpKinetics tables
     double P, Pref;
@@ -74,8 +78,7 @@ struct DevMech {
     const int32_t* grp_sp;
     const double* grp_eff;    // efficiency - default
     const double* grp_def;    // [nGroups]
-    // species sums
-    const int32_t* chunk_off; // [nChunks + 1]
+    // species sums: chunk c holds entries [c * kChunkLen, (c + 1) * kChunkLen)
     const uint32_t* entry;
     const int32_t* sp_chunk0; // [K]
     const int32_t* sp_nchunk; // [K]

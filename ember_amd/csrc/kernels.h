@@ -10,7 +10,7 @@ namespace emx {
 int launch_qss_stage(const DevMech& m, const DevOpts& o, const double* state_in,
                      const double* split_const, const double* qloss, const double* perturb,
                      double dt, int nPoints, const int* order, unsigned int* queue,
-                     double* state_out, int* status, int* stat1, int* stat2, double* qint,
+                     double* state_out, int* status, int* stat1, int* stat2, int* cost, double* qint,
                      int grid_blocks, int waves_per_block, hipStream_t stream);
 
 int launch_rates(const DevMech& m, const double* T, const double* Y, int nonorm,
@@ -51,10 +51,12 @@ namespace emx {
 int launch_bdf_stage(const DevMech& m, const DevOpts& o, const double* state_in,
                      const double* split_const, const double* qloss, const double* perturb,
                      double dt, int nPoints, const int* order, unsigned int* queue,
-                     double* state_out, int* status, int* stat1, int* stat2, double* qint,
+                     double* state_out, int* status, int* stat1, int* stat2, int* cost, double* qint,
                      int grid_blocks, int waves_per_block, double* jac_scratch, hipStream_t stream);
 size_t bdf_lds_bytes(const DevMech& m, int waves_per_block);
 size_t qss_lds_bytes(const DevMech& m, int waves_per_block);
+// order[0..n) = point indices by descending cost (single workgroup)
+int launch_cost_order(const int* cost, int n, int* order, hipStream_t stream);
 
 }  // namespace emx
 

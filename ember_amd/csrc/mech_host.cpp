@@ -71,7 +71,7 @@ std::string read_table(const std::string& path, HostMech& m) {
 
 static uint32_t pack_stoich(int n, const int* k) {
     uint32_t s = (uint32_t)n;
-    for (int j = 0; j < n; j++) s |= ((uint32_t)k[j] & 0xff) << (8 * (j + 1));
+    for (int j = 0; j < 3; j++) s |= ((uint32_t)(j < n ? k[j] : kPadSpecies) & 0xff) << (8 * (j + 1));
     return s;
 }
 
@@ -162,37 +162,28 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
         for (int j = 0; j < r.nr; j++) cnt[r.r[j]]++;
         for (auto& c : cnt) ent[c.first].push_back((uint32_t)s | ((uint32_t)c.second << 16));
     }
-    // chunk length minimising (rounds x length) with rounds <= kMaxChunkRounds
-    size_t maxcnt = 1;
-    for (auto& e : ent) maxcnt = std::max(maxcnt, e.size());
-    int bestS = -1;
-    long bestCost = 1L << 40;
-    for (int Sz = 1; Sz <= (int)maxcnt; Sz++) {
-        long nch = 0;
-        for (auto& e : ent) nch += std::max<long>(1, ((long)e.size() + Sz - 1) / Sz);
-        long rounds = (nch + 63) / 64;
-        if (rounds > kMaxChunkRounds) continue;
-        long cost = rounds * Sz;
-        if (cost < bestCost) { bestCost = cost; bestS = Sz; }
-    }
-    if (bestS < 0) return "species-sum chunking does not fit";
-    L.chunk_off.assign(1, 0);
+    // fixed-length chunks of kChunkLen entries; a species' chunks are consecutive
+    const uint32_t zero_entry = (uint32_t)(L.nRounds * 64);  // zero slot, multiplicity 0
     L.sp_chunk0.assign(m.K, 0);
     L.sp_nchunk.assign(m.K, 0);
+    L.entry.clear();
+    int nch = 0;
     for (int k = 0; k < m.K; k++) {
-        L.sp_chunk0[k] = (int)L.chunk_off.size() - 1;
+        L.sp_chunk0[k] = nch;
         size_t n = ent[k].size();
-        int nc = std::max<int>(1, (int)((n + bestS - 1) / bestS));
-        for (int c = 0; c < nc; c++) {
-            size_t a = (size_t)c * bestS, b = std::min(n, a + bestS);
-            for (size_t e = a; e < b; e++) L.entry.push_back(ent[k][e]);
-            L.chunk_off.push_back((int)L.entry.size());
-        }
+        int nc = std::max<int>(1, (int)((n + kChunkLen - 1) / kChunkLen));
+        for (int c = 0; c < nc; c++)
+            for (int e = 0; e < kChunkLen; e++) {
+                size_t i = (size_t)c * kChunkLen + e;
+                L.entry.push_back(i < n ? ent[k][i] : zero_entry);
+            }
         L.sp_nchunk[k] = nc;
+        nch += nc;
     }
-    if (L.entry.empty()) L.entry.push_back(0);
-    L.nChunks = (int)L.chunk_off.size() - 1;
-    L.nChunkRounds = (L.nChunks + 63) / 64;
+    L.nChunks = nch;
+    L.nChunkRounds = (nch + 63) / 64;
+    if (L.nChunkRounds > kMaxChunkRounds) return "species-sum chunking does not fit";
+    L.entry.resize((size_t)L.nChunkRounds * 64 * kChunkLen, zero_entry);
     L.rW.resize(m.K);
     for (int k = 0; k < m.K; k++) L.rW[k] = 1.0 / m.W[k];
     return "";

@@ -42,8 +42,7 @@ struct DevLayout {
     std::vector<int32_t> info;
     std::vector<int32_t> grp_off, grp_sp;
     std::vector<double> grp_eff, grp_def;
-    std::vector<int32_t> chunk_off;
-    std::vector<uint32_t> entry;
+    std::vector<uint32_t> entry;     // nChunkRounds * 64 * kChunkLen
     std::vector<int32_t> sp_chunk0, sp_nchunk;
     std::vector<double> rW;
 };

@@ -149,8 +149,9 @@ __device__ void plain_odefun(const DevMech& m, const Ws& w, const Slots& s, cons
     for (int k = 0; k < K; k++) {
         double q = 0.0, d = 0.0;
         for (int c = m.sp_chunk0[k]; c < m.sp_chunk0[k] + m.sp_nchunk[k]; c++) {
-            for (int e = m.chunk_off[c]; e < m.chunk_off[c + 1]; e++) {
+            for (int e = c * kChunkLen; e < (c + 1) * kChunkLen; e++) {
                 uint32_t en = m.entry[e];
+                if (en_nu(en) == 0) continue;  // chunk padding
                 double nu = (double)en_nu(en), f = w(s.ropf + en_slot(en)), r = w(s.ropr + en_slot(en));
                 if (en_prod(en)) { q += nu * f; d += nu * r; } else { q += nu * r; d += nu * f; }
             }

@@ -216,13 +216,16 @@ __device__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
     }
 }
 
-template <int NR, bool INTERP>
-__global__ void __launch_bounds__(256, 2) qss_stage_kernel(
+// OCC = waves per SIMD the register allocation targets: 2 (up to 8 waves per workgroup,
+// no spills) or 3 (9..12 waves per workgroup, register budget 168).
+template <int NR, bool INTERP, int OCC>
+__global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
     DevMech mg, DevOpts o, const double* __restrict__ state_in,
     const double* __restrict__ split_const, const double* __restrict__ qloss,
     const double* __restrict__ perturb_g, double dt, int nPoints, const int* __restrict__ order,
     unsigned int* __restrict__ queue, double* __restrict__ state_out, int* __restrict__ status,
-    int* __restrict__ stat1, int* __restrict__ stat2, double* __restrict__ qint_out) {
+    int* __restrict__ stat1, int* __restrict__ stat2, int* __restrict__ cost,
+    double* __restrict__ qint_out) {
     extern __shared__ double lds[];
     const double* perturb;
     const DevMech m = stage_mech(mg, perturb_g, lds, perturb);
@@ -231,6 +234,7 @@ __global__ void __launch_bounds__(256, 2) qss_stage_kernel(
     const int wave = threadIdx.x / 64;
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
     WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
+    init_wave_lds(L, lane, m.nRounds);
     const int nv = m.nVars;
     const bool active = lane < nv;
     QssSystem S;
@@ -263,6 +267,7 @@ __global__ void __launch_bounds__(256, 2) qss_stage_kernel(
             if (status) status[p] = err;
             if (stat1) stat1[p] = g;
             if (stat2) stat2[p] = r;
+            if (cost) cost[p] = g;
             if (qint_out) qint_out[p] = qi;
         }
     }
@@ -287,6 +292,7 @@ __global__ void __launch_bounds__(256) rates_kernel(DevMech mg, const double* __
     const int wave = threadIdx.x / 64;
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
     WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
+    init_wave_lds(L, lane, m.nRounds);
     const int p = blockIdx.x * (blockDim.x / 64) + wave;
     if (p >= n) return;
     const int K = m.K;
@@ -328,20 +334,24 @@ size_t qss_lds_bytes(const DevMech& m, int waves_per_block) {
 int launch_qss_stage(const DevMech& m, const DevOpts& o, const double* state_in,
                      const double* split_const, const double* qloss, const double* perturb,
                      double dt, int nPoints, const int* order, unsigned int* queue,
-                     double* state_out, int* status, int* stat1, int* stat2, double* qint,
+                     double* state_out, int* status, int* stat1, int* stat2, int* cost, double* qint,
                      int grid_blocks, int waves_per_block, hipStream_t stream) {
     size_t lds = ((size_t)mech_lds_doubles(m) + (size_t)waves_per_block * wave_lds_doubles(m.nRounds, m.nChunkRounds)) * sizeof(double);
+    if (waves_per_block < 1 || waves_per_block > 12) return (int)hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, stream, queue);
+#define EMX_QSS_LAUNCH(INTERP, OCC)                                                                    \
+    hipLaunchKernelGGL((qss_stage_kernel<NR, INTERP, OCC>), dim3(grid_blocks), dim3(64 * waves_per_block), \
+                       lds, stream, m, o, state_in, split_const, qloss, perturb, dt, nPoints, order,  \
+                       queue, state_out, status, stat1, stat2, cost, qint)
+    const bool occ3 = waves_per_block > 8;
     EMX_DISPATCH_NR(m.nRounds, {
-        if (m.interp)
-            hipLaunchKernelGGL((qss_stage_kernel<NR, true>), dim3(grid_blocks), dim3(64 * waves_per_block),
-                               lds, stream, m, o, state_in, split_const, qloss, perturb, dt, nPoints,
-                               order, queue, state_out, status, stat1, stat2, qint);
-        else
-            hipLaunchKernelGGL((qss_stage_kernel<NR, false>), dim3(grid_blocks), dim3(64 * waves_per_block),
-                               lds, stream, m, o, state_in, split_const, qloss, perturb, dt, nPoints,
-                               order, queue, state_out, status, stat1, stat2, qint);
+        if (m.interp) {
+            if (occ3) EMX_QSS_LAUNCH(true, 3); else EMX_QSS_LAUNCH(true, 2);
+        } else {
+            if (occ3) EMX_QSS_LAUNCH(false, 3); else EMX_QSS_LAUNCH(false, 2);
+        }
     });
+#undef EMX_QSS_LAUNCH
     return (int)hipGetLastError();
 }
 

@@ -98,6 +98,15 @@ int emx_set_launch(emx_context* ctx, int waves_per_block, int blocks_per_cu);
 enum { EMX_VARIANT_WAVE = 0, EMX_VARIANT_PLAIN = 1 };
 int emx_set_kernel_variant(emx_context* ctx, int variant);
 
+/* Work-queue order of the chemistry kernels.  EMX_SCHED_COST (default): when the
+ * previous stage ran over the same number of points, the points are taken in
+ * descending order of their previous right-hand-side evaluation counts (most expensive
+ * first); otherwise, and with EMX_SCHED_FIFO, in index order.  Results do not depend on
+ * the order.  No counterpart in the reference: TBB's dynamic grain-1 scheduling over
+ * points (src/flameSolver.cpp:758-759) is the analogue. */
+enum { EMX_SCHED_FIFO = 0, EMX_SCHED_COST = 1 };
+int emx_set_schedule(emx_context* ctx, int mode);
+
 /* Chemistry production stage over [t_stage_start, t_stage_start + dt] for all points:
  * FlameSolver::integrateProductionTerms (src/flameSolver.cpp:754-836) with
  * SourceSystemQSS or SourceSystemCVODE per point.
