@@ -183,6 +183,7 @@ struct Gas {
     std::vector<double> cpR, hRT, sR, grt, mu0;
     std::vector<double> kf0, rkcn, concm, rfn, ropf, ropr, ropnet;
     double thermoT = -1.0, rateT = -1.0;
+    const double* perturb = nullptr;  // per-thread rate multipliers (NULL: the mechanism's)
 
     Gas(Mech& mech, int kin, double pressure)
         : m(mech), kinetics(kin), P(pressure), y(m.K), ym(m.K), conc(m.K), cpR(m.K), hRT(m.K),
@@ -363,7 +364,8 @@ struct Gas {
             pr *= F / (1.0 + pr);
             kf0[i] = pr * khigh;
         }
-        for (int i = 0; i < m.I; i++) rfn[i] = kf0[i] * m.perturb[i];
+        const double* pt = perturb ? perturb : m.perturb.data();
+        for (int i = 0; i < m.I; i++) rfn[i] = kf0[i] * pt[i];
         for (int i = 0; i < m.I; i++) ropf[i] = rfn[i];
         for (int i = 0; i < m.I; i++)
             if (m.rxn[i].type == 1) ropf[i] *= concm[m.tb_slot[i]];
@@ -424,6 +426,19 @@ struct QssOde {
     virtual bool failed() const { return false; }
 };
 
+// Record of one accepted internal step, kept for the discrete adjoint: the state and the
+// predictor outputs at the step start, then every attempt (predictor value yp and the
+// corrector outputs at it); the last attempt is the accepted one.
+struct QssAttempt {
+    double dt;
+    std::vector<double> yp, qc, dc;
+};
+struct QssStepRecord {
+    std::vector<double> y, q, d;
+    double qdot;
+    std::vector<QssAttempt> att;
+};
+
 // QssIntegrator (src/qssintegrator.h:24-110, src/qssintegrator.cpp)
 struct QssIntegrator {
     QssOde* ode = nullptr;
@@ -440,6 +455,7 @@ struct QssIntegrator {
     // heat-release bookkeeping for accepted steps (set by the ODE on predictor calls)
     double qint = 0, qdot_step = 0;
     double* qdot_src = nullptr;
+    std::vector<QssStepRecord>* tape = nullptr;  // adjoint record (itermax 1), or none
 
     void initialize(int n) {
         N = n;
@@ -481,6 +497,7 @@ struct QssIntegrator {
         ode->odefun(tn + tstart, y.data(), q.data(), d.data(), false);
         gcount += 1;
         qdot_step = qdot_src ? *qdot_src : 0.0;
+        if (tape) tape->push_back({y, q, d, qdot_step, {}});
         if (firstStep) getInitialStepSize(tf);
         ts = tn;
         for (int i = 0; i < N; i++) rtau[i] = enforce[i] ? dt * d[i] / y[i] : 0.0;
@@ -498,6 +515,7 @@ struct QssIntegrator {
                 if (iter == 0) { tn = ts + dt; y1 = y; }
                 ode->odefun(tn + tstart, y.data(), q.data(), d.data(), true);
                 gcount += 1;
+                if (tape && iter == 0) tape->back().att.push_back({dt, y, q, d});
                 eps = 1.0e-10;
                 for (int i = 0; i < N; i++) rtaub[i] = enforce[i] ? 0.5 * (rtaus[i] + dt * d[i] / y[i]) : 0.0;
                 for (int i = 0; i < N; i++) {
@@ -1065,6 +1083,8 @@ void diffusion_coefficients(int N, const double* B, const double* D, const doubl
     if (leftBC == 3) s.k[0] += B[0] * (alpha + 1) / hh[0] * wallConst * yInf;
 }
 
+#include "qss_adjoint.inc"
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1283,6 +1303,157 @@ int orc_diffusion_integrate(int nVars, int N, const double* B, int nB, const int
         s.initialize(t0, h);
         s.integrateToTime(t1);
         for (int j = 0; j < N; j++) state[(size_t)j * nVars + v] = s.y[j];
+    }
+    return 0;
+}
+
+
+static bool adjoint_options_ok(const orc_options* o) {
+    return o->integrator == 0 && o->qss_itermax == 1 && o->qss_stability_check == 0;
+}
+
+static void prepare_interp(Mech& m, const orc_options* o) {
+    if (o->kinetics == 1) {
+        Gas g(m, 1, o->pressure);
+        std::vector<double> Y0(m.K, 1.0 / m.K);
+        g.setMassFractions(Y0.data());
+        g.setState_TP(300.0);
+        g.build_interp();
+    }
+}
+
+int orc_qss_adjoint(void* mech, const orc_options* o, int nPoints, const double* state_in,
+                    const double* split_const, const double* qloss, double dt, const double* Qbar,
+                    const double* ybar_final, double* state_out, double* Q, double* grad_y0,
+                    double* grad_lnk, int* status, int nthreads) {
+    Mech& m = *static_cast<Mech*>(mech);
+    if (!adjoint_options_ok(o)) return -6;
+    const int N = m.K + 2;
+    prepare_interp(m, o);
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        Gas gas(m, o->kinetics, o->pressure);
+        SourceQSS qss(gas, *o);
+        QssAdjoint adj(gas, qss);
+        std::vector<QssStepRecord> tape;
+        std::vector<double> yb(N), thb(m.I);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int j = 0; j < nPoints; j++) {
+            const double* yin = state_in + (size_t)j * N;
+            for (int i = 0; i < N; i++) qss.splitConst[i] = split_const ? split_const[(size_t)j * N + i] : 0.0;
+            qss.qLoss = qloss ? qloss[j] : 0.0;
+            qss.failed_rates = false;
+            tape.clear();
+            qss.integrator.tape = &tape;
+            qss.setState(yin);
+            int err = qss.integrateToTime(dt);
+            qss.integrator.tape = nullptr;
+            double* yout = state_out ? state_out + (size_t)j * N : nullptr;
+            if (yout) {
+                if (err >= 0) std::copy(qss.integrator.y.begin(), qss.integrator.y.end(), yout);
+                else std::copy(yin, yin + N, yout);
+            }
+            if (Q) Q[j] = qss.integrator.qint;
+            for (int i = 0; i < N; i++) yb[i] = ybar_final ? ybar_final[(size_t)j * N + i] : 0.0;
+            std::fill(thb.begin(), thb.end(), 0.0);
+            if (err >= 0 && !adj.reverse(tape, yin, Qbar ? Qbar[j] : 1.0, yb.data(), thb.data())) err = -2;
+            if (err < 0) {
+                std::fill(yb.begin(), yb.end(), 0.0);
+                std::fill(thb.begin(), thb.end(), 0.0);
+            }
+            if (grad_y0) std::copy(yb.begin(), yb.end(), grad_y0 + (size_t)j * N);
+            if (grad_lnk) std::copy(thb.begin(), thb.end(), grad_lnk + (size_t)j * m.I);
+            if (status) status[j] = err;
+        }
+    }
+    return 0;
+}
+
+int orc_qss_fd_gradient(void* mech, const orc_options* o, int nPoints, const double* state_in,
+                        const double* split_const, const double* qloss, double dt, const double* Qbar,
+                        const double* ybar_final, double h, int frozen, double* grad_y0,
+                        double* grad_lnk, int nthreads) {
+    Mech& m = *static_cast<Mech*>(mech);
+    if (!adjoint_options_ok(o)) return -6;
+    const int N = m.K + 2;
+    prepare_interp(m, o);
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        Gas gas(m, o->kinetics, o->pressure);
+        SourceQSS qss(gas, *o);
+        QssAdjoint adj(gas, qss);
+        std::vector<QssStepRecord> tape;
+        std::vector<double> y(N), pert(m.perturb);
+        const double* yb = nullptr;
+        double qb = 1.0;
+        auto run = [&](const double* y0) -> double {
+            double Qv;
+            if (frozen) {
+                std::copy(y0, y0 + N, y.begin());
+                Qv = adj.replay(tape, y.data());
+            } else {
+                qss.failed_rates = false;
+                qss.setState(y0);
+                qss.integrateToTime(dt);
+                Qv = qss.integrator.qint;
+                std::copy(qss.integrator.y.begin(), qss.integrator.y.end(), y.begin());
+            }
+            double J = qb * Qv;
+            if (yb)
+                for (int i = 0; i < N; i++) J += yb[i] * y[i];
+            return J;
+        };
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int j = 0; j < nPoints; j++) {
+            const double* yin = state_in + (size_t)j * N;
+            for (int i = 0; i < N; i++) qss.splitConst[i] = split_const ? split_const[(size_t)j * N + i] : 0.0;
+            qss.qLoss = qloss ? qloss[j] : 0.0;
+            qss.failed_rates = false;
+            tape.clear();
+            qss.integrator.tape = &tape;
+            qss.setState(yin);
+            qss.integrateToTime(dt);
+            qss.integrator.tape = nullptr;
+            qb = Qbar ? Qbar[j] : 1.0;
+            yb = ybar_final ? ybar_final + (size_t)j * N : nullptr;
+            std::vector<double> yp(yin, yin + N);
+            for (int i = 0; i < N; i++) {
+                double g = 0.0;
+                const double hi = h * std::abs(yin[i]);
+                if (i >= 1 && hi > 0.0) {
+                    yp[i] = yin[i] + hi;
+                    const double Qp = run(yp.data());
+                    yp[i] = yin[i] - hi;
+                    const double Qm = run(yp.data());
+                    yp[i] = yin[i];
+                    g = (Qp - Qm) / (2.0 * hi);
+                }
+                if (grad_y0) grad_y0[(size_t)j * N + i] = g;
+            }
+            if (grad_lnk) {
+                for (int r = 0; r < m.I; r++) {
+                    const double base = pert[r];
+                    pert[r] = base * std::exp(h);
+                    gas.perturb = pert.data();
+                    const double Qp = run(yin);
+                    pert[r] = base * std::exp(-h);
+                    const double Qm = run(yin);
+                    pert[r] = base;
+                    gas.perturb = nullptr;
+                    grad_lnk[(size_t)j * m.I + r] = (Qp - Qm) / (2.0 * h);
+                }
+            }
+        }
     }
     return 0;
 }

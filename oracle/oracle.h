@@ -83,6 +83,25 @@ int orc_diffusion_integrate(int nVars, int N, const double* B, int nB, const int
                             double multiplier, double dt_global, const double* h_per_var,
                             double t0, double t1, double* state, int nthreads);
 
+/* Discrete adjoint of the QSS stage (qss_itermax 1, no stability check; -6 otherwise).
+ * Per point: the stage as orc_source_integrate (state_out, Q = heat release integrated
+ * over accepted steps) and the gradients of J = Qbar*Q + ybar_final . y_final with the
+ * step sequence frozen: grad_y0 (nPoints x nVars) w.r.t. the initial state and grad_lnk
+ * (nPoints x nReactions) w.r.t. the logs of the rate multipliers.  Qbar NULL = ones,
+ * ybar_final NULL = zeros.  status as orc_source_integrate. */
+int orc_qss_adjoint(void* mech, const orc_options* o, int nPoints, const double* state_in,
+                    const double* split_const, const double* qloss, double dt, const double* Qbar,
+                    const double* ybar_final, double* state_out, double* Q, double* grad_y0,
+                    double* grad_lnk, int* status, int nthreads);
+
+/* Central finite differences of J = Qbar*Q + ybar_final . y_final for the same quantities.
+ * frozen = 1: the perturbed runs replay the unperturbed step sequence; 0: they rerun the
+ * integrator.  State steps are h*|y_i| (0 where y_i = 0, and for U), ln-multiplier steps h. */
+int orc_qss_fd_gradient(void* mech, const orc_options* o, int nPoints, const double* state_in,
+                        const double* split_const, const double* qloss, double dt, const double* Qbar,
+                        const double* ybar_final, double h, int frozen, double* grad_y0,
+                        double* grad_lnk, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

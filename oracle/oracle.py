@@ -62,6 +62,11 @@ def lib():
                                               C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                               C.c_double, C.c_double, _dp, C.c_double, C.c_double,
                                               _dp, C.c_int]
+        L.orc_qss_adjoint.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, _dp,
+                                      C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _ip, C.c_int]
+        L.orc_qss_fd_gradient.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, _dp,
+                                          C.c_double, _dp, _dp, C.c_double, C.c_int, _dp, _dp,
+                                          C.c_int]
         _lib = L
     return _lib
 
@@ -140,6 +145,48 @@ class Mech:
         lib().orc_source_integrate(self.h, C.byref(opts), n, _p(state), _p(sc), _p(ql), dt, _p(out),
                                    _p(status), _p(s1), _p(s2), _p(qint), nthreads)
         return out, status, s1, s2, qint
+
+    def adjoint(self, state, dt, opts=None, Qbar=None, ybar=None, split_const=None, qloss=None,
+                nthreads=0):
+        """Discrete adjoint of the QSS stage: returns (state_out, Q, grad_y0, grad_lnk, status)
+        for J = Qbar*Q + ybar . y_final (frozen step sequence)."""
+        opts = opts or options()
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        n, nv = state.shape
+        nr = lib().orc_mech_nreactions(self.h)
+        out = np.empty_like(state)
+        Q = np.zeros(n)
+        gy = np.zeros((n, nv))
+        gk = np.zeros((n, nr))
+        status = np.zeros(n, dtype=np.int32)
+        qb = None if Qbar is None else np.ascontiguousarray(np.broadcast_to(Qbar, (n,)), dtype=np.float64)
+        yb = None if ybar is None else np.ascontiguousarray(ybar, dtype=np.float64)
+        sc = None if split_const is None else np.ascontiguousarray(split_const, dtype=np.float64)
+        ql = None if qloss is None else np.ascontiguousarray(qloss, dtype=np.float64)
+        rc = lib().orc_qss_adjoint(self.h, C.byref(opts), n, _p(state), _p(sc), _p(ql), dt, _p(qb),
+                                   _p(yb), _p(out), _p(Q), _p(gy), _p(gk), _p(status), nthreads)
+        if rc:
+            raise RuntimeError(f"orc_qss_adjoint failed: {rc}")
+        return out, Q, gy, gk, status
+
+    def fd_gradient(self, state, dt, opts=None, h=1e-6, frozen=True, split_const=None, qloss=None,
+                    with_lnk=True, Qbar=None, ybar=None, nthreads=0):
+        """Central finite differences of J = Qbar*Q + ybar . y_final (orc_qss_fd_gradient)."""
+        opts = opts or options()
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        n, nv = state.shape
+        nr = lib().orc_mech_nreactions(self.h)
+        gy = np.zeros((n, nv))
+        gk = np.zeros((n, nr)) if with_lnk else None
+        sc = None if split_const is None else np.ascontiguousarray(split_const, dtype=np.float64)
+        ql = None if qloss is None else np.ascontiguousarray(qloss, dtype=np.float64)
+        qb = None if Qbar is None else np.ascontiguousarray(np.broadcast_to(Qbar, (n,)), dtype=np.float64)
+        yb = None if ybar is None else np.ascontiguousarray(ybar, dtype=np.float64)
+        rc = lib().orc_qss_fd_gradient(self.h, C.byref(opts), n, _p(state), _p(sc), _p(ql), dt, _p(qb),
+                                       _p(yb), h, 1 if frozen else 0, _p(gy), _p(gk), nthreads)
+        if rc:
+            raise RuntimeError(f"orc_qss_fd_gradient failed: {rc}")
+        return gy, gk
 
 
 def qss_test_problem(problem, y0, tf, epsmin=1e-2, epsmax=20.0, dtmin=1e-15, dtmax=1e-6,
