@@ -21,6 +21,10 @@ ARCH = os.environ.get("EMX_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    # contract a*b+c into FMA per source expression only, so every kernel that shares the
+    # integrator code computes it with the same roundings (the adjoint re-integrates the
+    # forward run and relies on reproducing it bit for bit)
+    "-ffp-contract=on",
     # packed FP32 instructions stay off: some of their forms have given wrong results on MI355X
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,

@@ -64,6 +64,11 @@ SIGNATURES = [
      [_vp, _sz, _sz, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _ip, _ip, _ip]),
     ("emx_source_integrate_device", C.c_int,
      [_vp, _sz, _sz, _vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp]),
+    ("emx_set_adjoint_capacity", C.c_int, [_vp, C.c_int]),
+    ("emx_source_adjoint_forward", C.c_int,
+     [_vp, _sz, _sz, _vp, _vp, _vp, C.c_double, _vp, _vp, _vp]),
+    ("emx_source_adjoint_backward", C.c_int,
+     [_vp, _sz, _sz, _vp, _vp, _vp, C.c_double, _vp, _vp, _vp, _vp, _vp]),
     ("emx_rates", C.c_int, [_vp, _sz, _dp, _dp, C.c_int, _dp, _dp, _dp, _ip]),
     ("emx_diffusion_integrate", C.c_int,
      [_vp, _sz, _sz, _dp, _sz, _ip, _dp, _dp, C.POINTER(Grid), _dp, C.c_double, C.c_double, _dp]),
@@ -198,6 +203,24 @@ class Context:
         _check(lib().emx_source_integrate_device(
             self.h, n, self.n_vars, _tptr(state_in), _tptr(split_const), _tptr(qloss), t_start, dt,
             _tptr(state_out), _tptr(status), _tptr(stat1), _tptr(stat2), _tptr(qint)))
+
+    # ---- discrete adjoint of the QSS stage (torch CUDA tensors, context stream)
+    def set_adjoint_capacity(self, max_checkpoints: int):
+        _check(lib().emx_set_adjoint_capacity(self.h, max_checkpoints))
+
+    def adjoint_forward(self, state_in, state_out, dt, heat_release=None, status=None,
+                        split_const=None, qloss=None):
+        n = state_in.shape[0]
+        _check(lib().emx_source_adjoint_forward(
+            self.h, n, self.n_vars, _tptr(state_in), _tptr(split_const), _tptr(qloss), dt,
+            _tptr(state_out), _tptr(status), _tptr(heat_release)))
+
+    def adjoint_backward(self, state_in, dt, grad_state, grad_lnk=None, Qbar=None, ybar=None,
+                         status=None, split_const=None, qloss=None):
+        n = state_in.shape[0]
+        _check(lib().emx_source_adjoint_backward(
+            self.h, n, self.n_vars, _tptr(state_in), _tptr(split_const), _tptr(qloss), dt,
+            _tptr(Qbar), _tptr(ybar), _tptr(grad_state), _tptr(grad_lnk), _tptr(status)))
 
     def rates(self, T, Y, nonorm=False):
         T = np.ascontiguousarray(np.atleast_1d(T), dtype=np.float64)

@@ -78,6 +78,11 @@ struct emx_context {
     int schedule = EMX_SCHED_COST;
     DevBuf cost, order;
     size_t cost_n = 0;  // number of points the recorded costs belong to (0: none)
+    // adjoint: checkpoints of the last adjoint forward stage, per-wave records
+    DevBuf ck_y, ck_s, nckpt, record;
+    int adj_cap = 64;       // checkpoints per point
+    size_t adj_n = 0;       // points of the last adjoint forward stage (0: none)
+    double adj_dt = 0.0;
 };
 
 namespace {
@@ -160,6 +165,12 @@ int upload_mech(emx_context* c) {
     pk.add(L.entry, &d.entry);
     pk.add(L.sp_chunk0, &d.sp_chunk0);
     pk.add(L.sp_nchunk, &d.sp_nchunk);
+    pk.add(L.grp_slot_off, &d.grp_slot_off);
+    pk.add(L.grp_slot, &d.grp_slot);
+    pk.add(L.sp_grp_off, &d.sp_grp_off);
+    pk.add(L.sp_grp, &d.sp_grp);
+    pk.add(L.sp_grp_eff, &d.sp_grp_eff);
+    pk.add(L.chunk_sp, &d.chunk_sp);
     pk.add(kf_c, &d.kf_c);
     pk.add(kf_s, &d.kf_s);
     pk.add(kc_c, &d.kc_c);
@@ -260,7 +271,7 @@ void emx_destroy(emx_context* c) {
     for (DevBuf* b : {&c->b_in, &c->b_out, &c->b_sc, &c->b_ql, &c->b_status, &c->b_s1, &c->b_s2,
                       &c->d_B, &c->d_Brow, &c->d_D, &c->d_sc, &c->d_x, &c->d_y, &c->d_h, &c->d_ns,
                       &c->d_work, &c->d_hmin, &c->t_a, &c->t_b, &c->t_c, &c->t_k, &c->jac, &c->plain_ws,
-                      &c->cost, &c->order})
+                      &c->cost, &c->order, &c->ck_y, &c->ck_s, &c->nckpt, &c->record})
         b->release();
     delete c;
 }
@@ -364,6 +375,94 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
                               c->waves_per_block, c->stream);
     if (rc) return fail(-10, std::string("qss launch: ") + hipGetErrorString((hipError_t)rc));
     c->cost_n = nPoints;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// discrete adjoint of the QSS stage
+// ---------------------------------------------------------------------------
+static int adjoint_check(emx_context* c, size_t nPoints, size_t nVars) {
+    if (int rc = check_ctx(c)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
+    if (c->opts.chemistry_integrator != EMX_QSS || c->opts.qss_iteration_count != 1 ||
+        c->opts.qss_stability_check)
+        return fail(-1, "the adjoint supports the QSS integrator with qss_iteration_count = 1 and "
+                        "no stability check");
+    return 0;
+}
+
+int emx_set_adjoint_capacity(emx_context* c, int max_checkpoints) {
+    if (!c) return fail(-1, "null context");
+    if (max_checkpoints < 1) return fail(-1, "max_checkpoints must be positive");
+    c->adj_cap = max_checkpoints;
+    c->adj_n = 0;
+    return 0;
+}
+
+int emx_source_adjoint_forward(emx_context* c, size_t nPoints, size_t nVars, const double* state_in,
+                               const double* split_const, const double* qloss, double dt,
+                               double* state_out, int32_t* status, double* heat_release) {
+    if (int rc = adjoint_check(c, nPoints, nVars)) return rc;
+    c->adj_n = 0;
+    if (nPoints == 0) return 0;
+    if (!state_in || !state_out) return fail(-1, "state pointers are NULL");
+    const size_t cap = (size_t)c->adj_cap;
+    HIP_TRY(c->ck_y.ensure(nPoints * cap * nVars * sizeof(double)));
+    HIP_TRY(c->ck_s.ensure(nPoints * cap * 2 * sizeof(double)));
+    HIP_TRY(c->nckpt.ensure(nPoints * sizeof(int)));
+    HIP_TRY(c->cost.ensure(nPoints * sizeof(int)));
+    HIP_TRY(c->order.ensure(nPoints * sizeof(int)));
+    const int* order = nullptr;
+    if (c->schedule == EMX_SCHED_COST && c->cost_n == nPoints) {
+        int rc = launch_cost_order(c->cost.as<int>(), (int)nPoints, c->order.as<int>(), c->stream);
+        if (rc) return fail(-10, std::string("order launch: ") + hipGetErrorString((hipError_t)rc));
+        order = c->order.as<int>();
+    }
+    const int wpb = std::min(c->waves_per_block, 8);
+    const size_t lds = adjoint_lds_bytes(c->dm, wpb, false);
+    int per_cu = std::max<int>(1, (int)((160 * 1024) / lds));
+    per_cu = std::min(per_cu, 32 / wpb);
+    const int blocks = (int)std::max<long>(1, std::min<long>((long)per_cu * c->num_cu, ((long)nPoints + wpb - 1) / wpb));
+    int rc = launch_qss_adjoint_forward(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
+                                        (int)nPoints, order, c->queue, state_out, status, heat_release,
+                                        c->cost.as<int>(), c->ck_y.as<double>(), c->ck_s.as<double>(),
+                                        c->nckpt.as<int>(), c->adj_cap, blocks, wpb, c->stream);
+    if (rc) return fail(-10, std::string("adjoint forward launch: ") + hipGetErrorString((hipError_t)rc));
+    c->cost_n = nPoints;
+    c->adj_n = nPoints;
+    c->adj_dt = dt;
+    return 0;
+}
+
+int emx_source_adjoint_backward(emx_context* c, size_t nPoints, size_t nVars, const double* state_in,
+                                const double* split_const, const double* qloss, double dt,
+                                const double* Qbar, const double* ybar_final, double* grad_state,
+                                double* grad_lnk, int32_t* status) {
+    if (int rc = adjoint_check(c, nPoints, nVars)) return rc;
+    if (nPoints == 0) return 0;
+    if (c->adj_n != nPoints || c->adj_dt != dt)
+        return fail(-1, "emx_source_adjoint_backward needs a preceding emx_source_adjoint_forward "
+                        "over the same points and dt");
+    if (!state_in || !grad_state) return fail(-1, "state pointers are NULL");
+    const int wpb = 8;
+    const size_t lds = adjoint_lds_bytes(c->dm, wpb, true);
+    if (lds > 160 * 1024) return fail(-1, "adjoint workgroup needs more than 160 KB of LDS");
+    const int blocks = (int)std::max<long>(1, std::min<long>((long)c->num_cu, ((long)nPoints + wpb - 1) / wpb));
+    HIP_TRY(c->record.ensure((size_t)blocks * wpb * adjoint_record_doubles_per_wave() * sizeof(double)));
+    // most expensive points first (costs of the forward stage)
+    const int* order = nullptr;
+    if (c->schedule == EMX_SCHED_COST && c->cost_n == nPoints) {
+        int rc = launch_cost_order(c->cost.as<int>(), (int)nPoints, c->order.as<int>(), c->stream);
+        if (rc) return fail(-10, std::string("order launch: ") + hipGetErrorString((hipError_t)rc));
+        order = c->order.as<int>();
+    }
+    int rc = launch_qss_adjoint_backward(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
+                                         (int)nPoints, order, c->queue, c->ck_y.as<double>(),
+                                         c->ck_s.as<double>(), c->nckpt.as<int>(), c->adj_cap, Qbar,
+                                         ybar_final, c->record.as<double>(), grad_state, grad_lnk,
+                                         status, blocks, wpb, c->stream);
+    if (rc) return fail(-10, std::string("adjoint backward launch: ") + hipGetErrorString((hipError_t)rc));
     return 0;
 }
 

@@ -82,6 +82,14 @@ struct DevMech {
     const uint32_t* entry;
     const int32_t* sp_chunk0; // [K]
     const int32_t* sp_nchunk; // [K]
+    // adjoint tables: group -> slots using it, species -> (group, efficiency - default),
+    // chunk -> species
+    const int32_t* grp_slot_off;  // [nGroups + 1]
+    const int32_t* grp_slot;
+    const int32_t* sp_grp_off;    // [K + 1]
+    const int32_t* sp_grp;
+    const double* sp_grp_eff;
+    const int32_t* chunk_sp;      // [nChunkRounds * 64]
     // interp tables [kInterpN][nRounds*64]
     const double* kf_c;
     const double* kf_s;

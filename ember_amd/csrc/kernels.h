@@ -55,6 +55,23 @@ int launch_bdf_stage(const DevMech& m, const DevOpts& o, const double* state_in,
                      int grid_blocks, int waves_per_block, double* jac_scratch, hipStream_t stream);
 size_t bdf_lds_bytes(const DevMech& m, int waves_per_block);
 size_t qss_lds_bytes(const DevMech& m, int waves_per_block);
+// discrete adjoint of the QSS stage
+size_t adjoint_record_doubles_per_wave();
+int adjoint_checkpoint_interval();
+size_t adjoint_lds_bytes(const DevMech& m, int waves_per_block, bool backward);
+int launch_qss_adjoint_forward(const DevMech& m, const DevOpts& o, const double* state_in,
+                               const double* split_const, const double* qloss,
+                               const double* perturb, double dt, int nPoints, const int* order,
+                               unsigned int* queue, double* state_out, int* status, double* Q,
+                               int* cost, double* ck_y, double* ck_s, int* nckpt, int cap,
+                               int grid_blocks, int waves_per_block, hipStream_t stream);
+int launch_qss_adjoint_backward(const DevMech& m, const DevOpts& o, const double* state_in,
+                                const double* split_const, const double* qloss,
+                                const double* perturb, double dt, int nPoints, const int* order,
+                                unsigned int* queue, const double* ck_y, const double* ck_s,
+                                const int* nckpt, int cap, const double* Qbar, const double* ybar,
+                                double* record, double* grad_state, double* grad_lnk, int* status,
+                                int grid_blocks, int waves_per_block, hipStream_t stream);
 // order[0..n) = point indices by descending cost (single workgroup)
 int launch_cost_order(const int* cost, int n, int* order, hipStream_t stream);
 

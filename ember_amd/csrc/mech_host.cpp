@@ -184,6 +184,31 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
     L.nChunkRounds = (nch + 63) / 64;
     if (L.nChunkRounds > kMaxChunkRounds) return "species-sum chunking does not fit";
     L.entry.resize((size_t)L.nChunkRounds * 64 * kChunkLen, zero_entry);
+    L.chunk_sp.assign((size_t)L.nChunkRounds * 64, 0);
+    for (int k = 0; k < m.K; k++)
+        for (int c = 0; c < L.sp_nchunk[k]; c++) L.chunk_sp[L.sp_chunk0[k] + c] = k;
+    // adjoint tables
+    L.grp_slot_off.assign(1, 0);
+    L.grp_slot.clear();
+    for (int g = 0; g < L.nGroups; g++) {
+        for (int s = 0; s < m.I; s++)
+            if (info_group(L.info[s]) == g) L.grp_slot.push_back(s);
+        L.grp_slot_off.push_back((int)L.grp_slot.size());
+    }
+    if (L.grp_slot.empty()) L.grp_slot.push_back(0);
+    L.sp_grp_off.assign(1, 0);
+    L.sp_grp.clear();
+    L.sp_grp_eff.clear();
+    for (int k = 0; k < m.K; k++) {
+        for (int g = 0; g < L.nGroups; g++)
+            for (int e = L.grp_off[g]; e < L.grp_off[g + 1]; e++)
+                if (L.grp_sp[e] == k) {
+                    L.sp_grp.push_back(g);
+                    L.sp_grp_eff.push_back(L.grp_eff[e]);
+                }
+        L.sp_grp_off.push_back((int)L.sp_grp.size());
+    }
+    if (L.sp_grp.empty()) { L.sp_grp.push_back(0); L.sp_grp_eff.push_back(0.0); }
     L.rW.resize(m.K);
     for (int k = 0; k < m.K; k++) L.rW[k] = 1.0 / m.W[k];
     return "";

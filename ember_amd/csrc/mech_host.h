@@ -44,6 +44,8 @@ struct DevLayout {
     std::vector<double> grp_eff, grp_def;
     std::vector<uint32_t> entry;     // nChunkRounds * 64 * kChunkLen
     std::vector<int32_t> sp_chunk0, sp_nchunk;
+    std::vector<int32_t> grp_slot_off, grp_slot, sp_grp_off, sp_grp, chunk_sp;
+    std::vector<double> sp_grp_eff;
     std::vector<double> rW;
 };
 

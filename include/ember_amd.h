@@ -129,6 +129,35 @@ int emx_source_integrate_device(emx_context* ctx, size_t nPoints, size_t nVars,
                                 double* state_out, int32_t* status, int32_t* stat1,
                                 int32_t* stat2, double* qint);
 
+/* Discrete adjoint of the QSS production stage (QSS integrator with
+ * qss_iteration_count = 1 and no stability check).  No counterpart in the reference: it
+ * differentiates the stage that SourceSystemQSS::integrateToTime runs per point
+ * (src/sourceSystem.cpp:434-443, src/qssintegrator.cpp:98-265).
+ *
+ * emx_source_adjoint_forward: the stage as emx_source_integrate_device (device pointers,
+ * asynchronous on the context stream), plus heat_release (nPoints, nullable) = the heat
+ * release rate integrated over the accepted internal steps [J/m^3], and checkpoints of the
+ * integrator state every 16 accepted steps kept in the context (status -5 for a point
+ * that needs more checkpoints than emx_set_adjoint_capacity allows).
+ *
+ * emx_source_adjoint_backward: for the points of the preceding adjoint forward call
+ * (same state_in, split_const, qloss, dt), the gradients of
+ *     J_j = Qbar_j * heat_release_j + ybar_final_j . state_out_j
+ * with the internal step sequence held fixed: grad_state (nPoints x nVars) w.r.t. the
+ * input state and grad_lnk (nPoints x nReactions, nullable) w.r.t. the natural logs of the
+ * reaction rate multipliers (emx_set_rate_multipliers), per point.  Qbar NULL = ones,
+ * ybar_final NULL = zeros.  status: 0 ok, < 0 failed point (its gradients are zero). */
+int emx_set_adjoint_capacity(emx_context* ctx, int max_checkpoints);
+int emx_source_adjoint_forward(emx_context* ctx, size_t nPoints, size_t nVars,
+                               const double* state_in, const double* split_const,
+                               const double* qloss, double dt, double* state_out,
+                               int32_t* status, double* heat_release);
+int emx_source_adjoint_backward(emx_context* ctx, size_t nPoints, size_t nVars,
+                                const double* state_in, const double* split_const,
+                                const double* qloss, double dt, const double* Qbar,
+                                const double* ybar_final, double* grad_state, double* grad_lnk,
+                                int32_t* status);
+
 /* Creation, destruction and net molar production rates [kmol/m^3/s] at n states
  * (CanteraGas::getCreationRates / getDestructionRates / getReactionRates,
  * src/chemistry0d.cpp:644-672).  T: n, Y: n x nSpecies, outputs n x nSpecies.
