@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libember_am
 
 EMX_QSS, EMX_BDF = 0, 1
 EMX_KIN_STANDARD, EMX_KIN_INTERP = 0, 1
-EMX_VARIANT_WAVE, EMX_VARIANT_PLAIN = 0, 1
+EMX_VARIANT_WAVE, EMX_VARIANT_PLAIN, EMX_VARIANT_WAVE_PER_POINT = 0, 1, 2
 EMX_SCHED_FIFO, EMX_SCHED_COST = 0, 1
 BC_FIXED_VALUE, BC_ZERO_GRADIENT, BC_CONTROL_VOLUME, BC_WALL_FLUX = 0, 1, 2, 3
 
@@ -73,7 +73,7 @@ SIGNATURES = [
     ("emx_diffusion_integrate", C.c_int,
      [_vp, _sz, _sz, _dp, _sz, _ip, _dp, _dp, C.POINTER(Grid), _dp, C.c_double, C.c_double, _dp]),
     ("emx_diffusion_integrate_device", C.c_int,
-     [_vp, _sz, _sz, _vp, _sz, _vp, _vp, _vp, C.POINTER(Grid), _vp, C.c_double, C.c_double, _vp]),
+     [_vp, _sz, _sz, _vp, _sz, _vp, _vp, _vp, C.POINTER(Grid), _dp, C.c_double, C.c_double, _vp]),
     ("emx_tridiagonal_integrate", C.c_int,
      [_vp, _sz, _sz, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, _dp]),
 ]
@@ -171,7 +171,8 @@ class Context:
         _check(lib().emx_set_launch(self.h, waves_per_block, blocks_per_cu))
 
     def set_kernel_variant(self, variant: int):
-        """0: wavefront-per-point kernel (default), 1: plain thread-per-point baseline."""
+        """0: wavefront-per-point kernel with work queue (default), 1: plain thread-per-point
+        baseline, 2: wavefront kernel with one wave per point and no queue."""
         _check(lib().emx_set_kernel_variant(self.h, variant))
 
     def set_schedule(self, mode: int):
@@ -260,7 +261,8 @@ class Context:
         g = Grid(C.cast(C.c_void_p(x.data_ptr()), _dp), alpha, left_bc, right_bc, y_inf, wall_const)
         _check(lib().emx_diffusion_integrate_device(
             self.h, nV, N, _tptr(B), B.shape[0], _tptr(B_row), _tptr(D), _tptr(split_const),
-            C.byref(g), _tptr(h_per_var), t0, t1, _tptr(state)))
+            C.byref(g), _p(None if h_per_var is None else np.ascontiguousarray(h_per_var, dtype=np.float64)),
+            t0, t1, _tptr(state)))
 
     def tridiagonal_integrate(self, a, b, c, k, y, h, t0, t1):
         """Constant systems, (nSys, N) arrays.  Returns y after integrateToTime(t1)."""

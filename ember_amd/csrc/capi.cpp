@@ -315,7 +315,8 @@ int emx_set_launch(emx_context* c, int waves_per_block, int blocks_per_cu) {
 
 int emx_set_kernel_variant(emx_context* c, int variant) {
     if (!c) return fail(-1, "null context");
-    if (variant != EMX_VARIANT_WAVE && variant != EMX_VARIANT_PLAIN) return fail(-1, "unknown variant");
+    if (variant != EMX_VARIANT_WAVE && variant != EMX_VARIANT_PLAIN && variant != EMX_VARIANT_WAVE_PER_POINT)
+        return fail(-1, "unknown variant");
     c->variant = variant;
     return 0;
 }
@@ -357,6 +358,8 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
         order = c->order.as<int>();
     }
     if (c->opts.chemistry_integrator == EMX_BDF) {
+        if (c->variant == EMX_VARIANT_WAVE_PER_POINT)
+            return fail(-1, "the per-point wavefront variant implements QSS only");
         if (bdf_lds_bytes(c->dm, c->waves_per_block) > 160 * 1024)
             return fail(-1, "BDF workgroup needs more than 160 KB of LDS; use fewer waves per block");
         const int blocks = grid_blocks(c, (int)nPoints, true);
@@ -369,9 +372,12 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
         c->cost_n = nPoints;
         return 0;
     }
+    const bool per_point = c->variant == EMX_VARIANT_WAVE_PER_POINT;
+    const int qblocks = per_point ? (int)((nPoints + c->waves_per_block - 1) / c->waves_per_block)
+                                  : grid_blocks(c, (int)nPoints, false);
     int rc = launch_qss_stage(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
-                              (int)nPoints, order, c->queue, state_out, status, stat1, stat2,
-                              c->cost.as<int>(), qint, grid_blocks(c, (int)nPoints, false),
+                              (int)nPoints, per_point ? nullptr : order, per_point ? nullptr : c->queue,
+                              state_out, status, stat1, stat2, c->cost.as<int>(), qint, qblocks,
                               c->waves_per_block, c->stream);
     if (rc) return fail(-10, std::string("qss launch: ") + hipGetErrorString((hipError_t)rc));
     c->cost_n = nPoints;
@@ -541,50 +547,49 @@ int emx_rates(emx_context* c, size_t n, const double* T, const double* Y, int no
 // ---------------------------------------------------------------------------
 // diffusion
 // ---------------------------------------------------------------------------
-static int diffusion_run(emx_context* c, const DiffArgs& a, const double* h_host_or_null,
-                         const double* h_dev_or_null, double t0, double t1, double* y_dev) {
+// Per variable: Integrator::initialize(t0, h) and integrateToTime(t1)
+// (src/integrator.cpp:37-47) with h from the caller or, when h_host is NULL, from
+// FlameSolver::setDiffusionSolverState's rule (src/flameSolver.cpp:709-711), which needs
+// one device-to-host read of the grid minimum.  Otherwise the call is asynchronous.
+static int diffusion_run(emx_context* c, const DiffArgs& a, const double* h_host, double t0,
+                         double t1, double* y_dev) {
     hipStream_t st = c->stream;
     const int nV = a.nV, N = a.N;
     HIP_TRY(c->d_work.ensure(diffusion_work_doubles(nV, N) * sizeof(double)));
     DiffWork w;
     diffusion_carve(c->d_work.as<double>(), nV, N, w);
-    std::vector<double> h(nV);
-    if (h_host_or_null) {
-        for (int v = 0; v < nV; v++) h[v] = h_host_or_null[v];
-    } else if (h_dev_or_null) {
-        HIP_TRY(hipMemcpyAsync(h.data(), h_dev_or_null, nV * 8, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
+    DiffSteps steps{};
+    if (h_host) {
+        for (int v = 0; v < nV; v++) steps.h[v] = h_host[v];
     } else {
         HIP_TRY(c->d_hmin.ensure(nV * sizeof(double)));
         int e = launch_diffusion_hmin(a, w, c->d_hmin.as<double>(), st);
         if (e) return fail(-10, "hmin launch failed");
-        HIP_TRY(hipMemcpyAsync(h.data(), c->d_hmin.p, nV * 8, hipMemcpyDeviceToHost, st));
+        std::vector<double> hm(nV);
+        HIP_TRY(hipMemcpyAsync(hm.data(), c->d_hmin.p, nV * 8, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        // FlameSolver::setDiffusionSolverState (src/flameSolver.cpp:709-711)
         for (int v = 0; v < nV; v++)
-            h[v] = std::min(c->opts.diffusion_timestep_multiplier * h[v], c->opts.global_timestep);
+            steps.h[v] = std::min(c->opts.diffusion_timestep_multiplier * hm[v], c->opts.global_timestep);
     }
     // Integrator::integrateToTime: equal steps (src/integrator.cpp:41-43)
-    std::vector<int> ns(nV);
     int maxs = 1;
     for (int v = 0; v < nV; v++) {
-        int n = static_cast<int>((t1 - t0) / h[v] + 1e-5);
+        if (!(steps.h[v] > 0.0)) return fail(-1, "diffusion step sizes must be positive");
+        int n = static_cast<int>((t1 - t0) / steps.h[v] + 1e-5);
         n = std::max(n, 1);
-        h[v] = (t1 - t0) / n;
-        ns[v] = n;
+        steps.h[v] = (t1 - t0) / n;
+        steps.ns[v] = n;
         maxs = std::max(maxs, n);
     }
     HIP_TRY(c->d_h.ensure(nV * sizeof(double)));
     HIP_TRY(c->d_ns.ensure(nV * sizeof(int)));
-    HIP_TRY(hipMemcpyAsync(c->d_h.p, h.data(), nV * 8, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(c->d_ns.p, ns.data(), nV * 4, hipMemcpyHostToDevice, st));
+    int e = launch_set_steps(steps, nV, c->d_h.as<double>(), c->d_ns.as<int>(), st);
+    if (e) return fail(-10, "step-size launch failed");
     const int nsolve = 8 + (maxs - 1);
     for (int q = 0; q < nsolve; q++) {
-        int e = launch_diffusion_solve(a, y_dev, c->d_h.as<double>(), c->d_ns.as<int>(), q, w, st);
+        e = launch_diffusion_solve(a, y_dev, c->d_h.as<double>(), c->d_ns.as<int>(), q, w, st);
         if (e) return fail(-10, std::string("diffusion launch: ") + hipGetErrorString((hipError_t)e));
     }
-    // keep host vectors alive until the copies above have completed
-    HIP_TRY(hipStreamSynchronize(st));
     return 0;
 }
 
@@ -607,7 +612,7 @@ int emx_diffusion_integrate_device(emx_context* c, size_t nVars, size_t nPoints,
     if (!B_row && nB != nVars) return fail(-1, "B_row is NULL but nB != nVars");
     DiffArgs a{B, B_row, D, split_const, g->x, (int)nPoints, (int)nVars, g->alpha, g->left_bc,
                g->right_bc, g->y_inf, g->wall_const};
-    return diffusion_run(c, a, nullptr, h_per_var, t0, t1, state);
+    return diffusion_run(c, a, h_per_var, t0, t1, state);
 }
 
 int emx_diffusion_integrate(emx_context* c, size_t nVars, size_t nPoints, const double* B, size_t nB,
@@ -643,7 +648,7 @@ int emx_diffusion_integrate(emx_context* c, size_t nVars, size_t nPoints, const 
     }
     DiffArgs a{c->d_B.as<double>(), brow, c->d_D.as<double>(), sc, c->d_x.as<double>(),
                (int)nPoints, (int)nVars, g->alpha, g->left_bc, g->right_bc, g->y_inf, g->wall_const};
-    int rc = diffusion_run(c, a, h_per_var, nullptr, t0, t1, c->d_y.as<double>());
+    int rc = diffusion_run(c, a, h_per_var, t0, t1, c->d_y.as<double>());
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(state, c->d_y.p, NV, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));

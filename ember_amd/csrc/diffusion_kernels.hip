@@ -428,6 +428,23 @@ void diffusion_carve(double* base, int nV, int N, DiffWork& w) {
     w.yprev = p; p += (size_t)N * nV;
 }
 
+namespace {
+// per-variable step sizes and step counts, delivered as kernel arguments so that the
+// values are fixed at launch (no host buffer has to outlive the call)
+__global__ void set_steps_kernel(DiffSteps s, int nV, double* h, int* ns) {
+    const int v = threadIdx.x;
+    if (v < nV) {
+        h[v] = s.h[v];
+        ns[v] = s.ns[v];
+    }
+}
+}  // namespace
+
+int launch_set_steps(const DiffSteps& s, int nV, double* h, int* ns, hipStream_t st) {
+    hipLaunchKernelGGL(set_steps_kernel, dim3(1), dim3(64), 0, st, s, nV, h, ns);
+    return (int)hipGetLastError();
+}
+
 int launch_diffusion_hmin(const DiffArgs& a, DiffWork& w, double* hmin, hipStream_t st) {
     DiffRows R{a.B, a.B_row, a.D, a.sc, a.x, nullptr, nullptr, nullptr, a.N, a.nV, a.alpha,
                a.left, a.right, a.yInf, a.wallConst, 0};

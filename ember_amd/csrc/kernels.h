@@ -35,6 +35,11 @@ struct DiffWork {
     double *own1, *nxt1, *X1, *own2, *nxt2, *X2, *cp, *dp, *cmin, *yprev;
 };
 
+struct DiffSteps {
+    double h[kMaxVars];
+    int ns[kMaxVars];
+};
+int launch_set_steps(const DiffSteps& s, int nV, double* h, int* ns, hipStream_t st);
 size_t diffusion_work_doubles(int nV, int N);
 void diffusion_carve(double* base, int nV, int N, DiffWork& w);
 int launch_diffusion_hmin(const DiffArgs& a, DiffWork& w, double* hmin, hipStream_t st);

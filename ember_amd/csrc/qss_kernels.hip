@@ -41,10 +41,18 @@ __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
     S.W = is_species_lane(lane, m) ? m.W[lane - 2] : 0.0;
     S.rW = is_species_lane(lane, m) ? m.rW[lane - 2] : 0.0;
     RateRegs<NR> R;
+    // queue == NULL: one point per wave, p = global wave index (non-persistent launch)
+    bool taken = false;
     while (true) {
         unsigned int t = 0;
-        if (lane == 0) t = atomicAdd(queue, 1u);
-        t = (unsigned int)bcast_i((int)t, 0);
+        if (queue) {
+            if (lane == 0) t = atomicAdd(queue, 1u);
+            t = (unsigned int)bcast_i((int)t, 0);
+        } else {
+            if (taken) break;
+            taken = true;
+            t = blockIdx.x * (blockDim.x / 64) + wave;
+        }
         if (t >= (unsigned int)nPoints) break;
         const int p = order ? order[t] : (int)t;
         const double* yin = state_in + (size_t)p * nv;
@@ -130,7 +138,7 @@ int launch_qss_stage(const DevMech& m, const DevOpts& o, const double* state_in,
                      int grid_blocks, int waves_per_block, hipStream_t stream) {
     size_t lds = ((size_t)mech_lds_doubles(m) + (size_t)waves_per_block * wave_lds_doubles(m.nRounds, m.nChunkRounds)) * sizeof(double);
     if (waves_per_block < 1 || waves_per_block > 12) return (int)hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, stream, queue);
+    if (queue) hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, stream, queue);
 #define EMX_QSS_LAUNCH(INTERP, OCC)                                                                    \
     hipLaunchKernelGGL((qss_stage_kernel<NR, INTERP, OCC>), dim3(grid_blocks), dim3(64 * waves_per_block), \
                        lds, stream, m, o, state_in, split_const, qloss, perturb, dt, nPoints, order,  \

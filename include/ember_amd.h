@@ -93,9 +93,10 @@ int emx_set_launch(emx_context* ctx, int waves_per_block, int blocks_per_cu);
 
 /* Chemistry kernel variant for the *_integrate entry points.  EMX_VARIANT_WAVE (default):
  * one wavefront per point, persistent work queue.  EMX_VARIANT_PLAIN: one thread per point
- * running the serial QssIntegrator loop (QSS only), kept as the plain-GPU comparison
- * baseline for benchmarks; it has no counterpart in the reference. */
-enum { EMX_VARIANT_WAVE = 0, EMX_VARIANT_PLAIN = 1 };
+ * running the serial QssIntegrator loop (QSS only).  EMX_VARIANT_WAVE_PER_POINT (QSS only):
+ * the wavefront kernel launched with one wave per point and no work queue.  The last two
+ * are comparison baselines for benchmarks; none has a counterpart in the reference. */
+enum { EMX_VARIANT_WAVE = 0, EMX_VARIANT_PLAIN = 1, EMX_VARIANT_WAVE_PER_POINT = 2 };
 int emx_set_kernel_variant(emx_context* ctx, int variant);
 
 /* Work-queue order of the chemistry kernels.  EMX_SCHED_COST (default): when the
@@ -182,7 +183,10 @@ int emx_diffusion_integrate(emx_context* ctx, size_t nVars, size_t nPoints, cons
                             const double* split_const, const emx_grid* grid,
                             const double* h_per_var, double t0, double t1, double* state);
 
-/* Same with device pointers (grid->x and B_row also device); asynchronous. */
+/* Same with device pointers (grid->x and B_row also device; h_per_var stays a host array,
+ * as the h of TridiagonalIntegrator::initialize, src/integrator.cpp:16-20, 111-116).  Asynchronous
+ * on the context stream when h_per_var is given; with NULL the call reads the grid minimum
+ * back to the host once (one stream synchronisation) to apply Ember's step rule. */
 int emx_diffusion_integrate_device(emx_context* ctx, size_t nVars, size_t nPoints,
                                    const double* B, size_t nB, const int32_t* B_row,
                                    const double* D, const double* split_const,
