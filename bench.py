@@ -40,7 +40,7 @@ def parse():
     p.add_argument("--bpc", type=int, default=0, help="workgroups per CU (0: as many as fit)")
     p.add_argument("--schedule", choices=["cost", "fifo"], default="cost")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
-    p.add_argument("--workload", choices=["stage", "uneven", "diffusion"], default="stage")
+    p.add_argument("--workload", choices=["stage", "uneven", "diffusion", "copy"], default="stage")
     return p.parse_args()
 
 
@@ -61,6 +61,8 @@ def main():
         return run_uneven(a, dev)
     if a.workload == "diffusion":
         return run_diffusion(a, dev)
+    if a.workload == "copy":
+        return run_copy(a, dev)
     N = a.points
     integ = capi.EMX_QSS if a.integrator == "qss" else capi.EMX_BDF
     kin = capi.EMX_KIN_STANDARD if a.kinetics == "standard" else capi.EMX_KIN_INTERP
@@ -364,6 +366,32 @@ def run_diffusion(a, dev):
                                "sample": f"the full stage, Thomas per species in parallel, {cpu_s:.2f} s"}
         res["parity"] = {"max_abs_err_over_max": float(err)}
     print(json.dumps(res))
+
+
+def run_copy(a, dev):
+    """Device-to-device copy of 512 MiB: the reference transfer for calibrating the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters (bytes read and written are known)."""
+    import torch
+    n = 512 * 1024 * 1024 // 8
+    x = torch.ones(n, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(a.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.steps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    gbs = 2 * n * 8 / (ms * 1e-3) / 1e9
+    print(json.dumps({"metric": "device copy GB/s (512 MiB read + 512 MiB written)", "value": gbs,
+                      "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+                      "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+                      "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": "torch copy_ of 512 MiB"}}))
 
 
 def adjoint_cost(ctx, N, dt, dev, stream, steps):

@@ -43,7 +43,8 @@ class Options(C.Structure):
 
 class Grid(C.Structure):
     _fields_ = [("x", _dp), ("alpha", C.c_int32), ("left_bc", C.c_int32),
-                ("right_bc", C.c_int32), ("y_inf", C.c_double), ("wall_const", C.c_double)]
+                ("right_bc", C.c_int32), ("y_inf", C.c_double), ("wall_const", C.c_double),
+                ("wall_var", C.c_int32)]
 
 
 # (name, restype, argtypes) for every entry point of include/ember_amd.h
@@ -237,7 +238,7 @@ class Context:
 
     # ---- diffusion
     def diffusion_integrate(self, state, B, D, x, t0, t1, B_row=None, split_const=None,
-                            alpha=0, left_bc=0, right_bc=0, y_inf=0.0, wall_const=0.0,
+                            alpha=0, left_bc=0, right_bc=0, y_inf=0.0, wall_const=0.0, wall_var=-1,
                             h_per_var=None):
         """state, D, split_const: (nPoints, nVars); B: (nB, nPoints).  Returns new state."""
         st = np.array(state, dtype=np.float64, order="C")
@@ -248,17 +249,18 @@ class Context:
         Br = None if B_row is None else np.ascontiguousarray(B_row, dtype=np.int32)
         sc = None if split_const is None else np.ascontiguousarray(split_const, dtype=np.float64)
         hv = None if h_per_var is None else np.ascontiguousarray(h_per_var, dtype=np.float64)
-        g = Grid(_p(x), alpha, left_bc, right_bc, y_inf, wall_const)
+        g = Grid(_p(x), alpha, left_bc, right_bc, y_inf, wall_const, wall_var)
         _check(lib().emx_diffusion_integrate(self.h, nV, N, _p(B), B.shape[0], _p(Br, _ip), _p(D),
                                              _p(sc), C.byref(g), _p(hv), t0, t1, _p(st)))
         return st
 
     def diffusion_integrate_device(self, state, B, D, x, t0, t1, B_row=None, split_const=None,
-                                   alpha=0, left_bc=0, right_bc=0, y_inf=0.0, wall_const=0.0,
+                                   alpha=0, left_bc=0, right_bc=0, y_inf=0.0, wall_const=0.0, wall_var=-1,
                                    h_per_var=None):
         """torch CUDA tensors; state is updated in place."""
         N, nV = state.shape
-        g = Grid(C.cast(C.c_void_p(x.data_ptr()), _dp), alpha, left_bc, right_bc, y_inf, wall_const)
+        g = Grid(C.cast(C.c_void_p(x.data_ptr()), _dp), alpha, left_bc, right_bc, y_inf, wall_const,
+                 wall_var)
         _check(lib().emx_diffusion_integrate_device(
             self.h, nV, N, _tptr(B), B.shape[0], _tptr(B_row), _tptr(D), _tptr(split_const),
             C.byref(g), _p(None if h_per_var is None else np.ascontiguousarray(h_per_var, dtype=np.float64)),

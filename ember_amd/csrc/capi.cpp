@@ -611,7 +611,7 @@ int emx_diffusion_integrate_device(emx_context* c, size_t nVars, size_t nPoints,
     if (int rc = check_diff(c, nVars, nPoints, g)) return rc;
     if (!B_row && nB != nVars) return fail(-1, "B_row is NULL but nB != nVars");
     DiffArgs a{B, B_row, D, split_const, g->x, (int)nPoints, (int)nVars, g->alpha, g->left_bc,
-               g->right_bc, g->y_inf, g->wall_const};
+               g->right_bc, g->y_inf, g->wall_const, g->wall_var};
     return diffusion_run(c, a, h_per_var, t0, t1, state);
 }
 
@@ -647,7 +647,7 @@ int emx_diffusion_integrate(emx_context* c, size_t nVars, size_t nPoints, const 
         sc = c->d_sc.as<double>();
     }
     DiffArgs a{c->d_B.as<double>(), brow, c->d_D.as<double>(), sc, c->d_x.as<double>(),
-               (int)nPoints, (int)nVars, g->alpha, g->left_bc, g->right_bc, g->y_inf, g->wall_const};
+               (int)nPoints, (int)nVars, g->alpha, g->left_bc, g->right_bc, g->y_inf, g->wall_const, g->wall_var};
     int rc = diffusion_run(c, a, h_per_var, t0, t1, c->d_y.as<double>());
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(state, c->d_y.p, NV, hipMemcpyDeviceToHost, st));

@@ -40,6 +40,7 @@ struct DiffRows {
     const double* h;      // per variable step
     int N, nV, alpha, left, right;
     double yInf, wallConst;
+    int wallVar;          // variable carrying yInf / wallConst (others: 0)
     int q;                // solve index: 0..7 BDF1 substeps, >= 8 BDF2 steps
 
     __device__ __forceinline__ double Bv(int v, int j) const {
@@ -75,7 +76,7 @@ struct DiffRows {
                 double hh0 = x[1] - x[0];
                 double c0 = Bv(v, 0) * (alpha + 1) / hh0;
                 double d = 0.5 * (Dv(v, 0) + Dv(v, 1));
-                b = -c0 * (d / hh0 + wallConst);
+                b = -c0 * (d / hh0 + (v == wallVar ? wallConst : 0.0));
                 c = d * c0 / hh0;
             }
         } else if (left == 1 && j == 1) {  // ZeroGradient (left)
@@ -95,7 +96,7 @@ struct DiffRows {
         k = sc ? sc[(size_t)j * nV + v] : 0.0;
         if (j == 0 && left == 3) {
             double hh0 = x[1] - x[0];
-            k += Bv(v, 0) * (alpha + 1) / hh0 * wallConst * yInf;
+            k += (v == wallVar) ? Bv(v, 0) * (alpha + 1) / hh0 * wallConst * yInf : 0.0;
         }
     }
     __device__ __forceinline__ void row(int v, int j, double& a, double& b, double& c,
@@ -447,7 +448,7 @@ int launch_set_steps(const DiffSteps& s, int nV, double* h, int* ns, hipStream_t
 
 int launch_diffusion_hmin(const DiffArgs& a, DiffWork& w, double* hmin, hipStream_t st) {
     DiffRows R{a.B, a.B_row, a.D, a.sc, a.x, nullptr, nullptr, nullptr, a.N, a.nV, a.alpha,
-               a.left, a.right, a.yInf, a.wallConst, 0};
+               a.left, a.right, a.yInf, a.wallConst, a.wallVar, 0};
     const int wpb = 4;
     const int C1 = (a.N + P1 - 1) / P1;
     hipLaunchKernelGGL(hmin_chunk_kernel, dim3((C1 + wpb - 1) / wpb), dim3(64 * wpb), 0, st, R, P1,
@@ -460,7 +461,7 @@ int launch_diffusion_hmin(const DiffArgs& a, DiffWork& w, double* hmin, hipStrea
 int launch_diffusion_solve(const DiffArgs& a, double* y, const double* h, const int* nsteps, int q,
                            DiffWork& w, hipStream_t st) {
     DiffRows R{a.B, a.B_row, a.D, a.sc, a.x, y, w.yprev, h, a.N, a.nV, a.alpha,
-               a.left, a.right, a.yInf, a.wallConst, q};
+               a.left, a.right, a.yInf, a.wallConst, a.wallVar, q};
     StateOut out{y, w.yprev, a.nV, q == 0 || q >= 8};
     return run_solve(R, out, a.N, a.nV, nsteps, q, w, st);
 }

@@ -29,6 +29,7 @@ struct DiffArgs {
     const double* x;
     int N, nV, alpha, left, right;
     double yInf, wallConst;
+    int wallVar;
 };
 
 struct DiffWork {

@@ -64,6 +64,9 @@ typedef struct emx_grid {
     int32_t left_bc, right_bc;
     double y_inf;         /* DiffusionSystem::yInf (WallFlux) */
     double wall_const;    /* DiffusionSystem::wallConst (WallFlux) */
+    int32_t wall_var;     /* the variable whose DiffusionSystem carries y_inf and wall_const
+                           * (Ember: kEnergy = 1, src/flameSolver.cpp:79-82); the others use
+                           * 0 and 0, DiffusionSystem's defaults.  -1: none. */
 } emx_grid;
 
 /* Fills `o` with Ember's defaults. */

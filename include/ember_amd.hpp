@@ -130,6 +130,7 @@ public:
         grid_.right_bc = rightBC;
         grid_.y_inf = 0.0;
         grid_.wall_const = 0.0;
+        grid_.wall_var = 1;  // kEnergy
     }
 
     // Diffusion coefficients as FlameSolver::prepareIntegrators sets them

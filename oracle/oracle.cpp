@@ -1270,9 +1270,9 @@ int orc_tridiag_integrate(int N, const double* a, const double* b, const double*
 
 int orc_diffusion_integrate(int nVars, int N, const double* B, int nB, const int* B_row,
                             const double* D, const double* split_const, const double* x, int alpha,
-                            int leftBC, int rightBC, double yInf, double wallConst, double multiplier,
-                            double dt_global, const double* h_per_var, double t0, double t1,
-                            double* state, int nthreads) {
+                            int leftBC, int rightBC, double yInf, double wallConst, int wallVar,
+                            double multiplier, double dt_global, const double* h_per_var, double t0,
+                            double t1, double* state, int nthreads) {
     (void)nB;
 #ifdef _OPENMP
     int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
@@ -1286,7 +1286,9 @@ int orc_diffusion_integrate(int nVars, int N, const double* B, int nB, const int
             sc[j] = split_const ? split_const[(size_t)j * nVars + v] : 0.0;
         }
         Tridiag s(N);
-        diffusion_coefficients(N, Bv, Dv.data(), sc.data(), x, alpha, leftBC, rightBC, yInf, wallConst, s);
+        // FlameSolver: only diffusionTerms[kEnergy] carries yInf / wallConst (src/flameSolver.cpp:79-82)
+        const double wc = (v == wallVar) ? wallConst : 0.0, yi = (v == wallVar) ? yInf : 0.0;
+        diffusion_coefficients(N, Bv, Dv.data(), sc.data(), x, alpha, leftBC, rightBC, yi, wc, s);
         double h;
         if (h_per_var) {
             h = h_per_var[v];

@@ -76,12 +76,14 @@ int orc_tridiag_integrate(int N, const double* a, const double* b, const double*
 /* Diffusion stage for nVars variables on one grid (FlameSolver::integrateDiffusionTerms).
  * B: nB x N rows (row-contiguous) with B_row[nVars]; D, split_const, state: nVars x N
  * column-major.  h_per_var NULL -> Ember's rule min(mult*min dlj^2/(B D), dtGlobal).
- * BC codes: 0 FixedValue, 1 ZeroGradient, 2 ControlVolume, 3 WallFlux. */
+ * BC codes: 0 FixedValue, 1 ZeroGradient, 2 ControlVolume, 3 WallFlux; yInf and wallConst
+ * apply to variable wallVar only (-1: none). */
 int orc_diffusion_integrate(int nVars, int N, const double* B, int nB, const int* B_row,
                             const double* D, const double* split_const, const double* x,
                             int alpha, int leftBC, int rightBC, double yInf, double wallConst,
-                            double multiplier, double dt_global, const double* h_per_var,
-                            double t0, double t1, double* state, int nthreads);
+                            int wallVar, double multiplier, double dt_global,
+                            const double* h_per_var, double t0, double t1, double* state,
+                            int nthreads);
 
 /* Discrete adjoint of the QSS stage (qss_itermax 1, no stability check; -6 otherwise).
  * Per point: the stage as orc_source_integrate (state_out, Q = heat release integrated

@@ -60,8 +60,8 @@ def lib():
                                             C.c_double, C.c_int, C.c_int, _dp, _dp, _dp]
         L.orc_diffusion_integrate.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _ip, _dp, _dp, _dp,
                                               C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
-                                              C.c_double, C.c_double, _dp, C.c_double, C.c_double,
-                                              _dp, C.c_int]
+                                              C.c_int, C.c_double, C.c_double, _dp, C.c_double,
+                                              C.c_double, _dp, C.c_int]
         L.orc_qss_adjoint.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, _dp,
                                       C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _ip, C.c_int]
         L.orc_qss_fd_gradient.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, _dp,
@@ -213,8 +213,8 @@ def tridiag_integrate(a, b, c, k, y0, t0, h, t1=0.0, mode=1, nsteps=0):
 
 
 def diffusion_integrate(B, B_row, D, split_const, x, state, t0, t1, alpha=0, leftBC=0, rightBC=0,
-                        yInf=0.0, wallConst=0.0, multiplier=10.0, dt_global=2e-5, h_per_var=None,
-                        nthreads=0):
+                        yInf=0.0, wallConst=0.0, wallVar=-1, multiplier=10.0, dt_global=2e-5,
+                        h_per_var=None, nthreads=0):
     """B: (nB, N); D, split_const, state: (N, nVars) (Ember column-major). Returns new state."""
     B = np.ascontiguousarray(B, dtype=np.float64)
     D = np.ascontiguousarray(D, dtype=np.float64)
@@ -225,6 +225,7 @@ def diffusion_integrate(B, B_row, D, split_const, x, state, t0, t1, alpha=0, lef
     sc = None if split_const is None else np.ascontiguousarray(split_const, dtype=np.float64)
     hv = None if h_per_var is None else np.ascontiguousarray(h_per_var, dtype=np.float64)
     lib().orc_diffusion_integrate(nVars, N, _p(B), B.shape[0], _p(Br), _p(D), _p(sc), _p(x), alpha,
-                                  leftBC, rightBC, yInf, wallConst, multiplier, dt_global, _p(hv),
+                                  leftBC, rightBC, yInf, wallConst, wallVar, multiplier, dt_global,
+                                  _p(hv),
                                   t0, t1, _p(st), nthreads)
     return st

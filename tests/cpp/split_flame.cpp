@@ -37,7 +37,7 @@ protected:
     void integrateDiffusionTerms() override {
         std::vector<int> brow(B_row.begin(), B_row.end());
         orc_diffusion_integrate((int)nVars, (int)nPoints, B.data(), (int)(B.size() / nPoints), brow.data(),
-                                D.data(), splitConstDiff.data(), x_.data(), 0, 0, 0, 0.0, 0.0, 10.0, dt,
+                                D.data(), splitConstDiff.data(), x_.data(), 0, 0, 0, 0.0, 0.0, 1, 10.0, dt,
                                 nullptr, tStageStart, tStageEnd, state.data(), 0);
     }
 

@@ -63,7 +63,7 @@ def test_diffusion_parity(ctx, oracle_mod, N, bc):
     B_row = rng.integers(0, 2, nV).astype(np.int32)
     left, right = bc
     alpha = 1 if left == 2 else 0
-    kw = dict(alpha=alpha, y_inf=0.3, wall_const=2.0)
+    kw = dict(alpha=alpha, y_inf=0.3, wall_const=2.0, wall_var=1)
     # Ember's step rule (several BDF2 steps on the denser grids); the largest grid
     # gets explicit steps so that the oracle finishes quickly
     t0, t1 = 1e-3, 1e-3 + 5e-6
@@ -71,7 +71,7 @@ def test_diffusion_parity(ctx, oracle_mod, N, bc):
     got = ctx.diffusion_integrate(case.y0, B, case.D, case.x, t0, t1, B_row=B_row, split_const=sc,
                                   left_bc=left, right_bc=right, h_per_var=hv, **kw)
     ref = oracle_mod.diffusion_integrate(B, B_row, case.D, sc, case.x, case.y0, t0, t1, alpha=alpha,
-                                         leftBC=left, rightBC=right, yInf=0.3, wallConst=2.0,
+                                         leftBC=left, rightBC=right, yInf=0.3, wallConst=2.0, wallVar=1,
                                          multiplier=10.0, dt_global=2e-5, h_per_var=hv)
     scale = np.abs(ref).max(axis=0, keepdims=True)
     np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12 * scale.max())
