@@ -27,7 +27,9 @@ namespace {
 
 constexpr int kCkptInterval = 16;  // accepted steps per checkpoint
 constexpr int kTapeAttempts = 64;  // attempts recorded per checkpoint interval
-constexpr int kTapeRows = 3 * kCkptInterval + 3 * kTapeAttempts;
+constexpr int kRowsPerEval = 5;     // y, q, d, creation rate, destruction rate
+constexpr int kTapeRows = kRowsPerEval * (kCkptInterval + kTapeAttempts);
+constexpr int kRecordDoubles = kTapeRows * 64;
 
 // Adjoint scratch per wave (doubles)
 struct AdjLds {
@@ -37,11 +39,13 @@ struct AdjLds {
     double* Mgb;    // [64] adjoint of third-body concentrations per group
     double* MB;     // [nRounds*64] per-slot third-body adjoint contributions
     double* att_dt; // [kTapeAttempts] attempt step sizes of the recorded interval
+    double* att_qd; // [kTapeAttempts] heat release rate of each attempt's corrector evaluation
+    double* stp_qd; // [kCkptInterval] heat release rate of each step's predictor evaluation
     int* att0;      // [kCkptInterval + 1] first attempt of each recorded step
 };
 
 __host__ __device__ inline int adj_lds_doubles(int nRounds) {
-    return 4 * 64 + nRounds * 64 + kTapeAttempts + (kCkptInterval + 2) / 2 + 1;
+    return 4 * 64 + nRounds * 64 + 2 * kTapeAttempts + kCkptInterval + (kCkptInterval + 2) / 2 + 1;
 }
 
 __device__ inline AdjLds carve_adj(double* base, int nRounds) {
@@ -52,7 +56,9 @@ __device__ inline AdjLds carve_adj(double* base, int nRounds) {
     a.Mgb = base + 192;
     a.MB = base + 256;
     a.att_dt = a.MB + nRounds * 64;
-    a.att0 = reinterpret_cast<int*>(a.att_dt + kTapeAttempts);
+    a.att_qd = a.att_dt + kTapeAttempts;
+    a.stp_qd = a.att_qd + kTapeAttempts;
+    a.att0 = reinterpret_cast<int*>(a.stp_qd + kCkptInterval);
     return a;
 }
 
@@ -101,15 +107,17 @@ struct ThermoBar {
 //             of Tn and of the thermo accumulate into Trb and thb.
 //  predictor: T = yy[T lane] and the thermo is computed from yy; thb holds the step's
 //             corrector contributions on entry and is consumed; Trb receives the T adjoint.
+//  wQ, wD, qdot: the evaluation's creation and destruction rates (this lane's species) and
+//  heat release rate, as recorded by the forward evaluation.
 //  qb, db: adjoints of the outputs q, d (this lane); qdotb: adjoint of qDot (uniform).
 //  yb accumulates the composition adjoint (species lanes); thetab[j] the ln-multiplier
 //  adjoint of slot j*64 + lane.
 template <int NR, bool INTERP>
 __device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, double Tn,
-                        double rho, double cp, double hk, double qb, double db, double qdotb,
-                        double qloss, ThermoBar& thb, double& Trb, double& yb, double (&thetab)[NR],
-                        RateRegs<NR>& R, const double* perturb, const WaveLds& L,
-                        const AdjLds& A) {
+                        double rho, double cp, double hk, double wQ, double wD, double qdot,
+                        double qb, double db, double qdotb, double qloss, ThermoBar& thb,
+                        double& Trb, double& yb, double (&thetab)[NR], RateRegs<NR>& R,
+                        const double* perturb, const WaveLds& L, const AdjLds& A) {
     const bool sp = is_species_lane(lane, m);
     const int S = m.nRounds * 64;
     const double T = corrector ? Tn : bcast(yy, 1);
@@ -127,10 +135,10 @@ __device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, d
         const double cp_mole = kGasConstant * (g.mmw * s);
         cp = cp_mole / g.mmw;
     }
-    rates_of_progress<NR>(m, lane, g, R, perturb, L);
-    double wQ, wD;
-    species_rates(m, lane, L, wQ, wD);
-    const double qdot = -wave_sum((wQ - wD) * hk);
+    // rates of progress are not needed again: the reverse sweep below works from the
+    // concentrations, the third-body sums and the cached rate constants, and the
+    // evaluation's creation/destruction rates and heat release rate come from the record
+    third_body_conc(m, lane, g, L);
     // ---- outputs -> creation/destruction rates and thermo
     const double qbT = bcast(qb, 1), dbT = bcast(db, 1);
     const double rc = rho * cp;
@@ -352,10 +360,10 @@ struct CkptHooks {
         nsteps++;
         return true;
     }
-    template <class Z>
-    __device__ void predicted(int, const Z&) {}
-    template <class Z>
-    __device__ bool corrected(int, const Z&, double) { return true; }
+    template <class Z, class Sys>
+    __device__ void predicted(int, const Z&, const Sys&) {}
+    template <class Z, class Sys>
+    __device__ bool corrected(int, const Z&, const Sys&, double) { return true; }
 };
 
 // Hooks: record up to kCkptInterval steps from a checkpoint (backward kernel)
@@ -372,23 +380,33 @@ struct RecordHooks {
         stop_y = z.y;
         return false;
     }
-    template <class Z>
-    __device__ void predicted(int lane, const Z& z) {
-        double* r = rows + (size_t)(3 * nsteps) * 64;
+    template <class Z, class Sys>
+    __device__ void predicted(int lane, const Z& z, const Sys& S) {
+        double* r = rows + (size_t)(kRowsPerEval * nsteps) * 64;
         r[lane] = z.y;
         r[64 + lane] = z.q;
         r[128 + lane] = z.d;
-        if (lane == 0) A.att0[nsteps] = natt;
+        r[192 + lane] = S.wQ;
+        r[256 + lane] = S.wD;
+        if (lane == 0) {
+            A.att0[nsteps] = natt;
+            A.stp_qd[nsteps] = S.qdot;
+        }
         nsteps++;
     }
-    template <class Z>
-    __device__ bool corrected(int lane, const Z& z, double dt) {
+    template <class Z, class Sys>
+    __device__ bool corrected(int lane, const Z& z, const Sys& S, double dt) {
         if (natt == kTapeAttempts) return false;
-        double* r = rows + (size_t)(3 * kCkptInterval + 3 * natt) * 64;
+        double* r = rows + (size_t)(kRowsPerEval * (kCkptInterval + natt)) * 64;
         r[lane] = z.y;
         r[64 + lane] = z.q;
         r[128 + lane] = z.d;
-        if (lane == 0) A.att_dt[natt] = dt;
+        r[192 + lane] = S.wQ;
+        r[256 + lane] = S.wD;
+        if (lane == 0) {
+            A.att_dt[natt] = dt;
+            A.att_qd[natt] = S.qdot;
+        }
         natt++;
         return true;
     }
@@ -479,7 +497,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_backward_kernel(
     init_wave_lds(L, lane, m.nRounds);
     AdjLds A = carve_adj(wbase + wave_lds_doubles(m.nRounds, m.nChunkRounds), m.nRounds);
     init_adj_lds(A, lane);
-    double* rows = record + (size_t)(blockIdx.x * (blockDim.x / 64) + wave) * kTapeRows * 64;
+    double* rows = record + (size_t)(blockIdx.x * (blockDim.x / 64) + wave) * kRecordDoubles;
     const int nv = m.nVars;
     const bool active = lane < nv;
     const bool sp = is_species_lane(lane, m);
@@ -539,7 +557,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_backward_kernel(
             wave_lds_sync();
             // ---- reverse sweep over the recorded steps
             for (int s = H.nsteps - 1; s >= 0; s--) {
-                const double* rs = rows + (size_t)(3 * s) * 64;
+                const double* rs = rows + (size_t)(kRowsPerEval * s) * 64;
                 const double yn = rs[lane], q0 = rs[64 + lane], d0 = rs[128 + lane];
                 const int t0 = A.att0[s];
                 const int na = A.att0[s + 1] - t0;
@@ -563,7 +581,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_backward_kernel(
                 ThermoBar thb{0.0, 0.0, 0.0};
                 double TrbStep = 0.0;
                 for (int a = na - 1; a >= 0; a--) {
-                    const double* ra = rows + (size_t)(3 * kCkptInterval + 3 * (t0 + a)) * 64;
+                    const double* ra = rows + (size_t)(kRowsPerEval * (kCkptInterval + t0 + a)) * 64;
                     const double yp = ra[lane], qc = ra[64 + lane], dc = ra[128 + lane];
                     const double dta = A.att_dt[t0 + a];
                     double qcb = qinb, dcb = dinb, ypb = 0.0, rtausb = 0.0;
@@ -589,7 +607,8 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_backward_kernel(
                             ypb += -0.5 * rbb * dta * dc / (yp * yp);
                         }
                     }
-                    if (!rhs_vjp<NR, INTERP>(m, lane, yp, true, Tn, rho, cp, hk, active ? qcb : 0.0,
+                    if (!rhs_vjp<NR, INTERP>(m, lane, yp, true, Tn, rho, cp, hk, ra[192 + lane],
+                                             ra[256 + lane], A.att_qd[t0 + a], active ? qcb : 0.0,
                                              active ? dcb : 0.0, 0.0, qloss_p, thb, TrbStep, ypb,
                                              thetab, R, perturb, L, A)) {
                         err = -2;
@@ -598,7 +617,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_backward_kernel(
                     // yp = max(ymin, y_n + dt s), s = (q_a - d_a) / G(rtau)
                     double qa = q0, da = d0;
                     if (a > 0) {
-                        const double* rp = rows + (size_t)(3 * kCkptInterval + 3 * (t0 + a - 1)) * 64;
+                        const double* rp = rows + (size_t)(kRowsPerEval * (kCkptInterval + t0 + a - 1)) * 64;
                         qa = rp[64 + lane];
                         da = rp[128 + lane];
                     }
@@ -620,7 +639,8 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_backward_kernel(
                 }
                 double TrbPred = 0.0;
                 const double dtA = A.att_dt[t0 + na - 1];
-                if (!rhs_vjp<NR, INTERP>(m, lane, yn, false, Tn, 0.0, 0.0, 0.0, active ? q0b : 0.0,
+                if (!rhs_vjp<NR, INTERP>(m, lane, yn, false, Tn, 0.0, 0.0, 0.0, rs[192 + lane],
+                                         rs[256 + lane], A.stp_qd[s], active ? q0b : 0.0,
                                          active ? d0b : 0.0, Qb * dtA, qloss_p, thb, TrbPred, ynb,
                                          thetab, R, perturb, L, A)) {
                     err = -2;
@@ -650,7 +670,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_backward_kernel(
 
 }  // namespace
 
-size_t adjoint_record_doubles_per_wave() { return (size_t)kTapeRows * 64; }
+size_t adjoint_record_doubles_per_wave() { return (size_t)kRecordDoubles; }
 int adjoint_checkpoint_interval() { return kCkptInterval; }
 
 size_t adjoint_lds_bytes(const DevMech& m, int waves_per_block, bool backward) {

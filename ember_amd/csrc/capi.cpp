@@ -586,10 +586,8 @@ static int diffusion_run(emx_context* c, const DiffArgs& a, const double* h_host
     int e = launch_set_steps(steps, nV, c->d_h.as<double>(), c->d_ns.as<int>(), st);
     if (e) return fail(-10, "step-size launch failed");
     const int nsolve = 8 + (maxs - 1);
-    for (int q = 0; q < nsolve; q++) {
-        e = launch_diffusion_solve(a, y_dev, c->d_h.as<double>(), c->d_ns.as<int>(), q, w, st);
-        if (e) return fail(-10, std::string("diffusion launch: ") + hipGetErrorString((hipError_t)e));
-    }
+    e = launch_diffusion_stage(a, y_dev, c->d_h.as<double>(), c->d_ns.as<int>(), nsolve, w, st);
+    if (e) return fail(-10, std::string("diffusion launch: ") + hipGetErrorString((hipError_t)e));
     return 0;
 }
 

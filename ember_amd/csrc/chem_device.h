@@ -273,14 +273,10 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
     return true;
 }
 
-// Rates of progress into LDS (composition-dependent part of updateROP).
-// perturb: per-slot rate multipliers (Cantera m_perturb).
-template <int NR>
-__device__ inline void rates_of_progress(const DevMech& m, int lane, const GasState& g,
-                                         const RateRegs<NR>& R, const double* perturb,
-                                         const WaveLds& L) {
+// ThirdBodyCalc::update per efficiency group: Mg = default * ctot + sum (eff - default) C
+__device__ inline void third_body_conc(const DevMech& m, int lane, const GasState& g,
+                                       const WaveLds& L) {
     wave_lds_sync();  // concentrations written by set_state
-    // ThirdBodyCalc::update per efficiency group
     if (lane < m.nGroups) {
         double sum = 0.0;
         for (int e = m.grp_off[lane]; e < m.grp_off[lane + 1]; e++)
@@ -288,6 +284,15 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
         L.Mg[lane] = m.grp_def[lane] * g.ctot + sum;
     }
     wave_lds_sync();
+}
+
+// Rates of progress into LDS (composition-dependent part of updateROP).
+// perturb: per-slot rate multipliers (Cantera m_perturb).
+template <int NR>
+__device__ inline void rates_of_progress(const DevMech& m, int lane, const GasState& g,
+                                         const RateRegs<NR>& R, const double* perturb,
+                                         const WaveLds& L) {
+    third_body_conc(m, lane, g, L);
 #pragma unroll
     for (int j = 0; j < NR; j++) {
         if (j < m.nRounds) {

@@ -283,7 +283,7 @@ struct SepOut {
 };
 
 __device__ __forceinline__ bool var_active(const int* nsteps, int v, int q) {
-    return q < 8 || (q - 7) < nsteps[v];
+    return !nsteps || q < 8 || (q - 7) < nsteps[v];
 }
 
 // ---------------------------------------------------------------------------
@@ -369,6 +369,389 @@ __global__ void hmin_reduce_kernel(const double* cmin, int C, int nV, double* ou
     double mn = __builtin_huge_val();
     for (int c = 0; c < C; c++) mn = fmin(mn, cmin[(size_t)c * nV + v]);
     out[v] = mn;
+}
+
+// ---------------------------------------------------------------------------
+// Fused stage path (C1 = ceil(N / P1) <= kPcrMax): per solve one PCR launch for the
+// separators and one chunk launch that back-substitutes this solve and eliminates the
+// chunk for the next solve with the matrix rows still in registers.
+// ---------------------------------------------------------------------------
+constexpr int kPcrMax = 4096;
+
+// TridiagonalIntegrator::step rows of solve q: shifted diagonal and right-hand side.
+// ih8 = 8 / h and ih = 1 / h (the reference divides by h / 8 and h; the quotients agree to
+// the last bit or one unit of it).
+__device__ __forceinline__ void bdf_row(int q, double ih8, double ih, double b, double k, double y,
+                                        double yp, double& bs, double& r) {
+    if (q < 8) {
+        bs = b - ih8;
+        r = -y * ih8 - k;
+    } else {
+        bs = b - 1.5 * ih;
+        r = -2 * y * ih + 0.5 * yp * ih - k;
+    }
+}
+
+// Matrix rows s .. s+m-1 of variable v.  Chunks away from both ends (all rows generic
+// interior rows of DiffusionSystem::get_A) share the grid metrics between neighbouring
+// rows; the end chunks take the per-row path with the boundary conditions.
+template <int P>
+__device__ __forceinline__ void chunk_matrix(const DiffRows& R, int v, int s, int m, double (&A)[P],
+                                             double (&Bm)[P], double (&Cm)[P], double (&K)[P]) {
+    const int N = R.N;
+    if (s >= 2 && s + m - 1 <= N - 3) {
+        const int nV = R.nV;
+        const double* Brow = R.B + (size_t)(R.B_row ? R.B_row[v] : v) * N;
+        double xm = R.x[s - 1], x0 = R.x[s];
+        double Dm = R.D[(size_t)(s - 1) * nV + v], D0 = R.D[(size_t)s * nV + v];
+        // c2 of row s-1
+        double c2m = (R.alpha ? 0.5 * (xm + x0) : 1.0) * (Dm + D0) / (x0 - xm);
+#pragma unroll
+        for (int t = 0; t < P; t++) {
+            if (t < m) {
+                const int j = s + t;
+                const double xp = R.x[j + 1];
+                const double Dp = R.D[(size_t)(j + 1) * nV + v];
+                const double dlj = 0.5 * (xp - xm);
+                const double rr = R.alpha ? x0 : 1.0;
+                const double c1 = 0.5 * Brow[j] / (dlj * rr);
+                const double c2 = (R.alpha ? 0.5 * (x0 + xp) : 1.0) * (D0 + Dp) / (xp - x0);
+                A[t] = c1 * c2m;
+                Bm[t] = -c1 * (c2m + c2);
+                Cm[t] = c1 * c2;
+                K[t] = R.sc ? R.sc[(size_t)j * nV + v] : 0.0;
+                c2m = c2;
+                xm = x0;
+                x0 = xp;
+                D0 = Dp;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < P; t++)
+            if (t < m) R.matrix(v, s + t, A[t], Bm[t], Cm[t], K[t]);
+    }
+}
+
+// One wave per chunk of P rows [s, e] (e = separator), lane = variable.
+//  q_solve >= 0: solve q_solve's interior rows with the separators X and write y
+//                (saving y to yprev first when q_solve starts a BDF step);
+//  q_next >= 0:  eliminate the chunk for solve q_next and write its summary (own, nxt).
+template <int P>
+__global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const int* nsteps, int q_solve,
+                                                        int q_next, const double* X, double* own,
+                                                        double* nxt) {
+    const int v = lane_id();
+    const int chunk = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int N = R.N, nV = R.nV;
+    const int C = (N + P - 1) / P;
+    if (chunk >= C || v >= nV) return;
+    const int s = chunk * P;
+    const int m = min(s + P, N) - s;  // rows in the chunk; the last is the separator
+    double A[P], Bm[P], Cm[P], K[P], Y[P], T1[P], T2[P];
+    chunk_matrix<P>(R, v, s, m, A, Bm, Cm, K);
+#pragma unroll
+    for (int t = 0; t < P; t++)
+        if (t < m) Y[t] = R.y[(size_t)(s + t) * nV + v];
+    const double hv = R.h[v];
+    const double ih8 = 8.0 / hv, ih = 1.0 / hv;
+    double* y = const_cast<double*>(R.y);
+    double* yprev = const_cast<double*>(R.yprev);
+    bool have_prev = false;  // T2 holds the y of the previous BDF step
+    if (q_solve >= 0 && var_active(nsteps, v, q_solve)) {
+        const double xL = chunk > 0 ? X[(size_t)(chunk - 1) * nV + v] : 0.0;
+        const double xR = X[(size_t)chunk * nV + v];
+        if (q_solve >= 8) {
+#pragma unroll
+            for (int t = 0; t < P; t++)
+                if (t < m) T2[t] = yprev[(size_t)(s + t) * nV + v];
+        }
+        // forward elimination over the interior: T1 = 1 / modified diagonal, T2 = rhs
+#pragma unroll
+        for (int t = 0; t < P; t++) {
+            if (t < m - 1) {
+                double bs, r;
+                bdf_row(q_solve, ih8, ih, Bm[t], K[t], Y[t], q_solve >= 8 ? T2[t] : 0.0, bs, r);
+                if (t == 0) r -= A[t] * xL;
+                if (t == m - 2) r -= Cm[t] * xR;
+                if (t == 0) {
+                    T1[t] = 1.0 / bs;
+                    T2[t] = r;
+                } else {
+                    const double f = A[t] * T1[t - 1];
+                    T1[t] = 1.0 / (bs - f * Cm[t - 1]);
+                    T2[t] = r - f * T2[t - 1];
+                }
+            }
+        }
+        // back substitution; T2 takes the old y (the next BDF2 step's y_(n-1))
+        const bool save_prev = q_solve == 0 || q_solve >= 8;
+        double xn = xR;
+#pragma unroll
+        for (int t = P - 1; t >= 0; t--) {
+            if (t < m) {
+                double xi = xR;
+                if (t < m - 1) xi = (t == m - 2) ? T2[t] * T1[t] : (T2[t] - Cm[t] * xn) * T1[t];
+                const size_t o = (size_t)(s + t) * nV + v;
+                if (save_prev) yprev[o] = Y[t];
+                T2[t] = Y[t];
+                Y[t] = xi;
+                y[o] = xi;
+                xn = xi;
+            }
+        }
+        have_prev = q_solve >= 8;
+    } else if (q_solve >= 0) {
+        return;  // variable finished its steps: nothing further for it in this stage
+    }
+    if (q_next < 0 || !var_active(nsteps, v, q_next)) return;
+    if (q_next >= 8 && !have_prev) {
+#pragma unroll
+        for (int t = 0; t < P; t++)
+            if (t < m) T2[t] = yprev[(size_t)(s + t) * nV + v];
+    }
+    // ---- chunk summary for solve q_next: downward and upward eliminations of the interior
+    double bt = 1.0, rt = 0.0, at = 0.0, cL = 0.0;
+#pragma unroll
+    for (int t = 0; t < P; t++) {
+        if (t < m - 1) {
+            double bs, r;
+            bdf_row(q_next, ih8, ih, Bm[t], K[t], Y[t], q_next >= 8 ? T2[t] : 0.0, bs, r);
+            if (t == 0) {
+                bt = bs;
+                rt = r;
+                at = A[t];
+            } else {
+                const double f = A[t] / bt;
+                bt = bs - f * cL;
+                rt = r - f * rt;
+                at = -f * at;
+            }
+            cL = Cm[t];
+        }
+    }
+    double bh = 1.0, rh = 0.0, ch = 0.0, aN = 0.0;
+#pragma unroll
+    for (int t = P - 1; t >= 0; t--) {
+        if (t < m - 1) {
+            double bs, r;
+            bdf_row(q_next, ih8, ih, Bm[t], K[t], Y[t], q_next >= 8 ? T2[t] : 0.0, bs, r);
+            if (t == m - 2) {
+                bh = bs;
+                rh = r;
+                ch = Cm[t];
+            } else {
+                const double g = Cm[t] / bh;
+                bh = bs - g * aN;
+                rh = r - g * rh;
+                ch = -g * ch;
+            }
+            aN = A[t];
+        }
+    }
+    double be = 0.0, re = 0.0, ae = 0.0;
+#pragma unroll
+    for (int t = 0; t < P; t++) {
+        if (t == m - 1) {
+            bdf_row(q_next, ih8, ih, Bm[t], K[t], Y[t], q_next >= 8 ? T2[t] : 0.0, be, re);
+            ae = A[t];
+        }
+    }
+    double* o = own + ((size_t)chunk * nV + v) * 3;
+    double* n = nxt + ((size_t)chunk * nV + v) * 3;
+    if (m > 1) {
+        const double ib = 1.0 / bt;
+        o[0] = -ae * at * ib;
+        o[1] = be - ae * cL * ib;
+        o[2] = re - ae * rt * ib;
+    } else {
+        o[0] = ae;
+        o[1] = be;
+        o[2] = re;
+    }
+    if (s > 0) {
+        double ap, bp, cp, kp;
+        R.matrix(v, s - 1, ap, bp, cp, kp);
+        if (m > 1) {
+            const double ib = 1.0 / bh;
+            n[0] = -cp * aN * ib;
+            n[1] = -cp * ch * ib;
+            n[2] = -cp * rh * ib;
+        } else {
+            n[0] = 0.0;
+            n[1] = cp;
+            n[2] = 0.0;
+        }
+    } else {
+        n[0] = n[1] = n[2] = 0.0;
+    }
+}
+
+// Separators of one variable per workgroup, from the reduced tridiagonal system of the
+// chunk summaries (C rows, C <= kPcrMax) held in LDS.  Each thread first eliminates its
+// own run of G = ceil(C / 1024) consecutive rows down to the run's last row (the same
+// downward/upward elimination as the chunks), the <= 1024 run separators are solved by
+// parallel cyclic reduction (each level couples row i with rows i -/+ stride and doubles
+// the stride), and each thread back-substitutes its run.
+__global__ void __launch_bounds__(1024) pcr_kernel(const double* own, const double* nxt, int C,
+                                                   int nV, const int* nsteps, int q, double* X) {
+    const int v = blockIdx.x;
+    if (!var_active(nsteps, v, q)) return;  // uniform over the workgroup
+    extern __shared__ double sm[];
+    double* a = sm;
+    double* b = sm + C;
+    double* c = sm + 2 * C;
+    double* r = sm + 3 * C;
+    double* sa = sm + 4 * C;  // separator system, <= 1024 rows
+    double* sb = sa + 1024;
+    double* sc = sb + 1024;
+    double* sr = sc + 1024;
+    ReducedRows RR{own, nxt, C, nV};
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < C; i += nt) RR.row(v, i, a[i], b[i], c[i], r[i]);
+    __syncthreads();
+    const int G = (C + nt - 1) / nt;        // rows per run
+    const int S = (C + G - 1) / G;          // runs (separator rows)
+    const int s0 = tid * G;
+    const int e0 = min(s0 + G, C) - 1;      // separator of this run
+    // run summary (as chunk_summary): separator row in terms of the neighbouring separators
+    double nb_ = 0.0, nc_ = 0.0, nr_ = 0.0;
+    if (tid < S) {
+        double bt = 1.0, rt = 0.0, at = 0.0, cL = 0.0;
+        for (int i = s0; i < e0; i++) {
+            if (i == s0) {
+                bt = b[i]; rt = r[i]; at = a[i];
+            } else {
+                const double f = a[i] / bt;
+                bt = b[i] - f * cL; rt = r[i] - f * rt; at = -f * at;
+            }
+            cL = c[i];
+        }
+        double bh = 1.0, rh = 0.0, ch = 0.0, aN = 0.0;
+        for (int i = e0 - 1; i >= s0; i--) {
+            if (i == e0 - 1) {
+                bh = b[i]; rh = r[i]; ch = c[i];
+            } else {
+                const double g = c[i] / bh;
+                bh = b[i] - g * aN; rh = r[i] - g * rh; ch = -g * ch;
+            }
+            aN = a[i];
+        }
+        double oa = a[e0], ob = b[e0], orr = r[e0];
+        if (e0 > s0) {
+            oa = -a[e0] * at / bt;
+            ob = b[e0] - a[e0] * cL / bt;
+            orr = r[e0] - a[e0] * rt / bt;
+        }
+        // own part of separator row tid
+        sa[tid] = oa;
+        sb[tid] = ob;
+        sr[tid] = orr;
+        // contribution of this run's interior to the previous separator's row (tid - 1)
+        if (tid > 0) {
+            const double cp = c[s0 - 1];
+            nb_ = 0.0;
+            nc_ = cp;
+            nr_ = 0.0;
+            if (e0 > s0) {
+                nb_ = -cp * aN / bh;
+                nc_ = -cp * ch / bh;
+                nr_ = -cp * rh / bh;
+            }
+        }
+    }
+    __syncthreads();  // every run has read its rows and its left neighbour's c
+    // pass the contribution to the previous separator through this run's first row slot
+    // (that row is reassembled for the back substitution)
+    if (tid < S && tid > 0) {
+        a[s0] = nb_;
+        c[s0] = nc_;
+        r[s0] = nr_;
+    }
+    __syncthreads();
+    if (tid < S) {
+        double cc = 0.0;
+        if (tid + 1 < S) {
+            const int sn = (tid + 1) * G;
+            sb[tid] += a[sn];
+            cc = c[sn];
+            sr[tid] += r[sn];
+        }
+        sc[tid] = cc;
+    }
+    __syncthreads();
+    // parallel cyclic reduction on the S separator rows, one row per thread
+    for (int st = 1; st < S; st <<= 1) {
+        double na = 0.0, nb = 0.0, nc = 0.0, nr = 0.0;
+        if (tid < S) {
+            const int lo = tid - st, hi = tid + st;
+            nb = sb[tid];
+            nr = sr[tid];
+            if (lo >= 0) {
+                const double al = -sa[tid] / sb[lo];
+                na = al * sa[lo];
+                nb += al * sc[lo];
+                nr += al * sr[lo];
+            }
+            if (hi < S) {
+                const double ga = -sc[tid] / sb[hi];
+                nc = ga * sc[hi];
+                nb += ga * sa[hi];
+                nr += ga * sr[hi];
+            }
+        }
+        __syncthreads();
+        if (tid < S) {
+            sa[tid] = na;
+            sb[tid] = nb;
+            sc[tid] = nc;
+            sr[tid] = nr;
+        }
+        __syncthreads();
+    }
+    // back-substitute each run's interior between its neighbouring separators
+    if (tid < S) {
+        const double xR = sr[tid] / sb[tid];
+        X[(size_t)e0 * nV + v] = xR;
+        double xL = 0.0;
+        if (tid > 0) {
+            const int ep = s0 - 1;  // previous run's separator
+            xL = sr[tid - 1] / sb[tid - 1];
+            (void)ep;
+        }
+        // rows s0 .. e0-1 (the first row's a/c/r were overwritten: reassemble it)
+        const int mi = e0 - s0;
+        if (mi > 0) {
+            double ra0, rb0, rc0, rr0;
+            RR.row(v, s0, ra0, rb0, rc0, rr0);
+            // forward elimination with 1/diag, then back substitution
+            double dinv[8], rhs[8];  // G <= kPcrMax / 1024 = 4
+            for (int t = 0; t < mi; t++) {
+                const int i = s0 + t;
+                const double ai = t == 0 ? ra0 : a[i], bi = t == 0 ? rb0 : b[i];
+                double ri = t == 0 ? rr0 : r[i];
+                if (t == 0) ri -= ai * xL;
+                if (t == mi - 1) ri -= (t == 0 ? rc0 : c[i]) * xR;
+                if (t == 0) {
+                    dinv[t] = 1.0 / bi;
+                    rhs[t] = ri;
+                } else {
+                    const double cprev = (t - 1 == 0) ? rc0 : c[i - 1];
+                    const double f = ai * dinv[t - 1];
+                    dinv[t] = 1.0 / (bi - f * cprev);
+                    rhs[t] = ri - f * rhs[t - 1];
+                }
+            }
+            double xn = xR;
+            for (int t = mi - 1; t >= 0; t--) {
+                const int i = s0 + t;
+                const double ci = (t == 0) ? rc0 : c[i];
+                const double xi = (t == mi - 1) ? rhs[t] * dinv[t] : (rhs[t] - ci * xn) * dinv[t];
+                X[(size_t)i * nV + v] = xi;
+                xn = xi;
+            }
+        }
+    }
 }
 
 constexpr int P1 = 16;   // level-1 chunk (points)
@@ -464,6 +847,32 @@ int launch_diffusion_solve(const DiffArgs& a, double* y, const double* h, const 
                a.left, a.right, a.yInf, a.wallConst, a.wallVar, q};
     StateOut out{y, w.yprev, a.nV, q == 0 || q >= 8};
     return run_solve(R, out, a.N, a.nV, nsteps, q, w, st);
+}
+
+int launch_diffusion_stage(const DiffArgs& a, double* y, const double* h, const int* nsteps,
+                           int nsolve, DiffWork& w, hipStream_t st) {
+    const int C1 = (a.N + P1 - 1) / P1;
+    if (C1 > kPcrMax || a.N < 2 * P1) {
+        for (int q = 0; q < nsolve; q++) {
+            int e = launch_diffusion_solve(a, y, h, nsteps, q, w, st);
+            if (e) return e;
+        }
+        return 0;
+    }
+    DiffRows R{a.B, a.B_row, a.D, a.sc, a.x, y, w.yprev, h, a.N, a.nV, a.alpha,
+               a.left, a.right, a.yInf, a.wallConst, a.wallVar, 0};
+    const int wpb = 4;
+    const dim3 grid((C1 + wpb - 1) / wpb), block(64 * wpb);
+    const size_t pcr_lds = ((size_t)4 * C1 + 4 * 1024) * sizeof(double);
+    hipLaunchKernelGGL((diff_step_kernel<P1>), grid, block, 0, st, R, nsteps, -1, 0, w.X1, w.own1,
+                       w.nxt1);
+    for (int q = 0; q < nsolve; q++) {
+        hipLaunchKernelGGL(pcr_kernel, dim3(a.nV), dim3(1024), pcr_lds, st, w.own1, w.nxt1, C1, a.nV,
+                           nsteps, q, w.X1);
+        hipLaunchKernelGGL((diff_step_kernel<P1>), grid, block, 0, st, R, nsteps, q,
+                           q + 1 < nsolve ? q + 1 : -1, w.X1, w.own1, w.nxt1);
+    }
+    return (int)hipGetLastError();
 }
 
 int launch_explicit_solve(const double* A, const double* B, const double* Cc, const double* K,

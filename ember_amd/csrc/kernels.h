@@ -46,6 +46,9 @@ void diffusion_carve(double* base, int nV, int N, DiffWork& w);
 int launch_diffusion_hmin(const DiffArgs& a, DiffWork& w, double* hmin, hipStream_t st);
 int launch_diffusion_solve(const DiffArgs& a, double* y, const double* h, const int* nsteps, int q,
                            DiffWork& w, hipStream_t st);
+// all nsolve solves of a stage (fused path when the grid allows it)
+int launch_diffusion_stage(const DiffArgs& a, double* y, const double* h, const int* nsteps,
+                           int nsolve, DiffWork& w, hipStream_t st);
 int launch_explicit_solve(const double* A, const double* B, const double* C, const double* K,
                           int nSys, int N, double* y, const double* h, const int* nsteps, int q,
                           DiffWork& w, hipStream_t st);

@@ -30,10 +30,10 @@ struct QssResume {
 struct NoHooks {
     template <class Z>
     __device__ __forceinline__ bool step_start(int, const Z&, double, double, bool) { return true; }
-    template <class Z>
-    __device__ __forceinline__ void predicted(int, const Z&) {}
-    template <class Z>
-    __device__ __forceinline__ bool corrected(int, const Z&, double) { return true; }
+    template <class Z, class Sys>
+    __device__ __forceinline__ void predicted(int, const Z&, const Sys&) {}
+    template <class Z, class Sys>
+    __device__ __forceinline__ bool corrected(int, const Z&, const Sys&, double) { return true; }
 };
 
 // Per-lane view of SourceSystemQSS: the variable this lane holds plus the
@@ -46,6 +46,7 @@ struct QssSystem {
     double rho, cp;    // from the last non-corrector call (SourceSystem::updateThermo)
     double hk;         // J/kmol, species lanes
     double qdot;
+    double wQ, wD;     // creation / destruction rate of this lane's species (last evaluation)
 };
 
 template <int NR, bool INTERP>
@@ -68,6 +69,8 @@ __device__ inline bool qss_odefun(const DevMech& m, int lane, double y, bool cor
     rates_of_progress<NR>(m, lane, g, R, perturb, L);
     double wQ, wD;
     species_rates(m, lane, L, wQ, wD);
+    S.wQ = wQ;
+    S.wD = wD;
     S.qdot = -wave_sum((wQ - wD) * S.hk);
     const double rho = S.rho, cp = S.cp;
     if (lane == 0) {
@@ -132,8 +135,8 @@ __device__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
         if (predict && !H.step_start(lane, Z, tn, dt, firstStep)) return 1;
         if (!qss_odefun<NR, INTERP>(m, lane, Z.y, !predict, S, R, perturb, L, Z.q, Z.d)) return -2;
         gcount += 1;
-        if (predict) H.predicted(lane, Z);
-        else if (!H.corrected(lane, Z, dt)) return -5;
+        if (predict) H.predicted(lane, Z, S);
+        else if (!H.corrected(lane, Z, S, dt)) return -5;
         bool start_attempt = false;
         if (predict) {
             // ---- integrateOneStep: start of a step
