@@ -375,173 +375,259 @@ __global__ void hmin_reduce_kernel(const double* cmin, int C, int nV, double* ou
 // Fused stage path (C1 = ceil(N / P1) <= kPcrMax): per solve one PCR launch for the
 // separators and one chunk launch that back-substitutes this solve and eliminates the
 // chunk for the next solve with the matrix rows still in registers.
+//
+// The reduced system of this path is variable-major: plane k of the chunk summaries
+// (own: alpha, beta, rho; nxt: dbeta, gamma, drho) holds element (v, c) at
+// (k * nV + v) * C + c, and the separators X at v * C + c, so the PCR workgroup of one
+// variable reads and writes contiguous rows.
 // ---------------------------------------------------------------------------
 constexpr int kPcrMax = 4096;
+// buffer resource word 3 of gfx9 (raw 32-bit records, no swizzle)
+constexpr int kBufferFlags = 0x00020000;
+// waves per workgroup of diff_step_kernel
+constexpr int kDiffWaves = 4;
 
-// TridiagonalIntegrator::step rows of solve q: shifted diagonal and right-hand side.
-// ih8 = 8 / h and ih = 1 / h (the reference divides by h / 8 and h; the quotients agree to
-// the last bit or one unit of it).
-__device__ __forceinline__ void bdf_row(int q, double ih8, double ih, double b, double k, double y,
-                                        double yp, double& bs, double& r) {
-    if (q < 8) {
-        bs = b - ih8;
-        r = -y * ih8 - k;
-    } else {
-        bs = b - 1.5 * ih;
-        r = -2 * y * ih + 0.5 * yp * ih - k;
+// Grid metrics of OneDimGrid (src/grid.cpp:43-69) as they enter DiffusionSystem::get_A
+// (src/diffusion_system.cpp:10-50): c1_j = B_j g1_j with g1_j = 0.5 / (dlj r_j), and
+// c2_j = (D_j + D_{j+1}) g2_j with g2_j = rphalf_j / hh_j.  They depend on the grid only,
+// so they are formed once per stage instead of once per variable and solve.
+__global__ void grid_metrics_kernel(const double* x, int N, int alpha, double* g1, double* g2) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    double a = 0.0, b = 0.0;
+    if (j >= 1 && j <= N - 2) {
+        const double dlj = 0.5 * (x[j + 1] - x[j - 1]);
+        a = 0.5 / (dlj * (alpha ? x[j] : 1.0));
     }
+    if (j <= N - 2) b = (alpha ? 0.5 * (x[j] + x[j + 1]) : 1.0) / (x[j + 1] - x[j]);
+    g1[j] = a;
+    g2[j] = b;
 }
 
-// Matrix rows s .. s+m-1 of variable v.  Chunks away from both ends (all rows generic
-// interior rows of DiffusionSystem::get_A) share the grid metrics between neighbouring
-// rows; the end chunks take the per-row path with the boundary conditions.
-template <int P>
-__device__ __forceinline__ void chunk_matrix(const DiffRows& R, int v, int s, int m, double (&A)[P],
-                                             double (&Bm)[P], double (&Cm)[P], double (&K)[P]) {
+// 1 / b by Newton-Raphson from the hardware estimate (v_rcp_f64, relative error below
+// 2^-22): each step r <- r + r (1 - b r) squares the relative error, so two steps reach
+// the rounding level.  For the pivots of diagonally dominant rows (no zero, overflow or
+// denormal range).
+__device__ __forceinline__ double rcp_nr(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    return fma(r, e, r);
+}
+
+// TridiagonalIntegrator::step rows of solve q (src/integrator.cpp:133-204): diagonal shifted
+// by sh and right-hand side cy y + cp yp - k, with BDF1 substeps of h/8 (q < 8:
+// sh = 8/h, cy = -8/h, cp = 0) and BDF2 steps of h (sh = 3/(2h), cy = -2/h, cp = 1/(2h)).
+// The reference divides by h/8 and h; the products with the reciprocals agree to the
+// last bit or one unit of it.
+struct BdfCoef {
+    double sh, cy, cp;
+    __device__ __forceinline__ BdfCoef(int q, double hv) {
+        const double ih = 1.0 / hv;
+        if (q < 8) {
+            sh = 8.0 * ih;
+            cy = -sh;
+            cp = 0.0;
+        } else {
+            sh = 1.5 * ih;
+            cy = -2.0 * ih;
+            cp = 0.5 * ih;
+        }
+    }
+    __device__ __forceinline__ double rhs(double y, double yp, double k) const {
+        return cy * y + cp * yp - k;
+    }
+};
+
+// Point-major array (nV values of one point contiguous) read and written through a buffer
+// resource: lane v supplies the byte offset 8 v, the row offset j * nV * 8 is a scalar, so
+// all rows of a chunk share one address register.  Out-of-range accesses read 0 and
+// write nothing; a null array has no records and reads as 0.
+struct PtBuf {
+    __amdgpu_buffer_rsrc_t r;
+    unsigned rowb;  // bytes per point row
+    __device__ __forceinline__ PtBuf(const double* p, int N, int nV)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), 0,
+                                              p ? (int)((size_t)N * nV * sizeof(double)) : 0,
+                                              kBufferFlags)),
+          rowb((unsigned)nV * sizeof(double)) {}
+    using u32x2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0u, 0u, 0));
+    __device__ __forceinline__ double ld(unsigned v8, int j) const {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v8, j * rowb, 0));
+    }
+    __device__ __forceinline__ void st(double x, unsigned v8, int j) const {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v8, j * rowb, 0);
+    }
+};
+
+// Matrix rows s .. s+m-1 of variable v, and the c coefficient of row s-1 (cp).  Chunks
+// away from both ends (all rows generic interior rows of DiffusionSystem::get_A) take the
+// grid metrics from g1/g2; the end chunks take the per-row path with the boundary
+// conditions.  FULL: m == P (every chunk but the last), so no row needs a guard and the
+// loads issue together.
+template <int P, bool FULL>
+__device__ __forceinline__ void chunk_matrix(const DiffRows& R, const double* __restrict__ g1,
+                                             const double* __restrict__ g2, int v, int s, int m,
+                                             double (&A)[P], double (&Bm)[P], double (&Cm)[P],
+                                             double* K, double& cp) {
     const int N = R.N;
+    const unsigned v8 = (unsigned)v * sizeof(double);
+    const PtBuf Kb(R.sc, N, R.nV);
+#pragma unroll
+    for (int t = 0; t < P; t++)
+        if (FULL || t < m) K[t * 64] = Kb.ld(v8, s + t);
     if (s >= 2 && s + m - 1 <= N - 3) {
-        const int nV = R.nV;
+        const PtBuf Db(R.D, N, R.nV);
         const double* Brow = R.B + (size_t)(R.B_row ? R.B_row[v] : v) * N;
-        double xm = R.x[s - 1], x0 = R.x[s];
-        double Dm = R.D[(size_t)(s - 1) * nV + v], D0 = R.D[(size_t)s * nV + v];
-        // c2 of row s-1
-        double c2m = (R.alpha ? 0.5 * (xm + x0) : 1.0) * (Dm + D0) / (x0 - xm);
+        double D0 = Db.ld(v8, s);
+        double c2m = (Db.ld(v8, s - 1) + D0) * g2[s - 1];
+        cp = Brow[s - 1] * g1[s - 1] * c2m;
 #pragma unroll
         for (int t = 0; t < P; t++) {
-            if (t < m) {
+            if (FULL || t < m) {
                 const int j = s + t;
-                const double xp = R.x[j + 1];
-                const double Dp = R.D[(size_t)(j + 1) * nV + v];
-                const double dlj = 0.5 * (xp - xm);
-                const double rr = R.alpha ? x0 : 1.0;
-                const double c1 = 0.5 * Brow[j] / (dlj * rr);
-                const double c2 = (R.alpha ? 0.5 * (x0 + xp) : 1.0) * (D0 + Dp) / (xp - x0);
+                const double Dp = Db.ld(v8, j + 1);
+                const double c1 = Brow[j] * g1[j];
+                const double c2 = (D0 + Dp) * g2[j];
                 A[t] = c1 * c2m;
                 Bm[t] = -c1 * (c2m + c2);
                 Cm[t] = c1 * c2;
-                K[t] = R.sc ? R.sc[(size_t)j * nV + v] : 0.0;
                 c2m = c2;
-                xm = x0;
-                x0 = xp;
                 D0 = Dp;
             }
         }
     } else {
+        double k;
 #pragma unroll
         for (int t = 0; t < P; t++)
-            if (t < m) R.matrix(v, s + t, A[t], Bm[t], Cm[t], K[t]);
+            if (FULL || t < m) R.matrix(v, s + t, A[t], Bm[t], Cm[t], k);
+        if (s == 0 && R.left == 3 && v == R.wallVar)  // WallFlux constant of get_k
+            K[0] += R.Bv(v, 0) * (R.alpha + 1) / (R.x[1] - R.x[0]) * R.wallConst * R.yInf;
+        cp = 0.0;
+        if (s > 0) {
+            double ap, bp;
+            R.matrix(v, s - 1, ap, bp, cp, k);
+        }
     }
 }
 
-// One wave per chunk of P rows [s, e] (e = separator), lane = variable.
-//  q_solve >= 0: solve q_solve's interior rows with the separators X and write y
-//                (saving y to yprev first when q_solve starts a BDF step);
-//  q_next >= 0:  eliminate the chunk for solve q_next and write its summary (own, nxt).
-template <int P>
-__global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const int* nsteps, int q_solve,
-                                                        int q_next, const double* X, double* own,
-                                                        double* nxt) {
-    const int v = lane_id();
-    const int chunk = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+// Downward pivots of the chunk interior rows 0 .. m-2 for the shift sh: T1[t] = 1 / bt_t
+// with bt_0 = b_0 - sh, bt_t = b_t - sh - a_t c_(t-1) / bt_(t-1).  The Thomas forward
+// sweep of a solve and the downward sweep of a chunk summary share them.
+template <int P, bool FULL>
+__device__ __forceinline__ void down_pivots(int m, double sh, const double (&A)[P],
+                                            const double (&Bm)[P], const double (&Cm)[P],
+                                            double (&T1)[P]) {
+#pragma unroll
+    for (int t = 0; t < P - 1; t++) {
+        if (FULL || t < m - 1) {
+            const double bs = Bm[t] - sh;
+            if (t == 0)
+                T1[t] = rcp_nr(bs);
+            else
+                T1[t] = rcp_nr(bs - A[t] * T1[t - 1] * Cm[t - 1]);
+        }
+    }
+}
+
+template <int P, bool FULL>
+__device__ __forceinline__ void chunk_step(const DiffRows& R, const double* __restrict__ g1,
+                                           const double* __restrict__ g2, const int* nsteps,
+                                           int q_solve, int q_next, const double* X, double* own,
+                                           double* nxt, double* K, int v, int chunk, int C) {
     const int N = R.N, nV = R.nV;
-    const int C = (N + P - 1) / P;
-    if (chunk >= C || v >= nV) return;
     const int s = chunk * P;
-    const int m = min(s + P, N) - s;  // rows in the chunk; the last is the separator
-    double A[P], Bm[P], Cm[P], K[P], Y[P], T1[P], T2[P];
-    chunk_matrix<P>(R, v, s, m, A, Bm, Cm, K);
+    const int m = FULL ? P : min(s + P, N) - s;  // rows in the chunk; the last is the separator
+    const unsigned v8 = (unsigned)v * sizeof(double);
+    const PtBuf Yb(R.y, N, nV), Pb(R.yprev, N, nV);
+    double A[P], Bm[P], Cm[P], Y[P], T1[P], T2[P];
+    double cp;
+    chunk_matrix<P, FULL>(R, g1, g2, v, s, m, A, Bm, Cm, K, cp);
 #pragma unroll
     for (int t = 0; t < P; t++)
-        if (t < m) Y[t] = R.y[(size_t)(s + t) * nV + v];
+        if (FULL || t < m) Y[t] = Yb.ld(v8, s + t);
     const double hv = R.h[v];
-    const double ih8 = 8.0 / hv, ih = 1.0 / hv;
-    double* y = const_cast<double*>(R.y);
-    double* yprev = const_cast<double*>(R.yprev);
-    bool have_prev = false;  // T2 holds the y of the previous BDF step
+    int pivots = -1;         // shift class (0: BDF1, 1: BDF2) of the pivots held in T1
     if (q_solve >= 0 && var_active(nsteps, v, q_solve)) {
-        const double xL = chunk > 0 ? X[(size_t)(chunk - 1) * nV + v] : 0.0;
-        const double xR = X[(size_t)chunk * nV + v];
+        const double* Xv = X + (size_t)v * C;
+        const double xL = chunk > 0 ? Xv[chunk - 1] : 0.0;
+        const double xR = Xv[chunk];
         if (q_solve >= 8) {
 #pragma unroll
             for (int t = 0; t < P; t++)
-                if (t < m) T2[t] = yprev[(size_t)(s + t) * nV + v];
+                if (FULL || t < m) T2[t] = Pb.ld(v8, s + t);
         }
-        // forward elimination over the interior: T1 = 1 / modified diagonal, T2 = rhs
+        const BdfCoef bc(q_solve, hv);
+        down_pivots<P, FULL>(m, bc.sh, A, Bm, Cm, T1);
+        pivots = q_solve >= 8;
+        // forward sweep of the right-hand side (T2 = modified rhs)
 #pragma unroll
-        for (int t = 0; t < P; t++) {
-            if (t < m - 1) {
-                double bs, r;
-                bdf_row(q_solve, ih8, ih, Bm[t], K[t], Y[t], q_solve >= 8 ? T2[t] : 0.0, bs, r);
+        for (int t = 0; t < P - 1; t++) {
+            if (FULL || t < m - 1) {
+                double r = bc.rhs(Y[t], q_solve >= 8 ? T2[t] : 0.0, K[t * 64]);
                 if (t == 0) r -= A[t] * xL;
                 if (t == m - 2) r -= Cm[t] * xR;
-                if (t == 0) {
-                    T1[t] = 1.0 / bs;
+                if (t == 0)
                     T2[t] = r;
-                } else {
-                    const double f = A[t] * T1[t - 1];
-                    T1[t] = 1.0 / (bs - f * Cm[t - 1]);
-                    T2[t] = r - f * T2[t - 1];
-                }
+                else
+                    T2[t] = r - A[t] * T1[t - 1] * T2[t - 1];
             }
         }
-        // back substitution; T2 takes the old y (the next BDF2 step's y_(n-1))
+        // back substitution (yprev takes the old y: the next BDF2 step's y_(n-1))
         const bool save_prev = q_solve == 0 || q_solve >= 8;
         double xn = xR;
 #pragma unroll
         for (int t = P - 1; t >= 0; t--) {
-            if (t < m) {
+            if (FULL || t < m) {
                 double xi = xR;
                 if (t < m - 1) xi = (t == m - 2) ? T2[t] * T1[t] : (T2[t] - Cm[t] * xn) * T1[t];
-                const size_t o = (size_t)(s + t) * nV + v;
-                if (save_prev) yprev[o] = Y[t];
-                T2[t] = Y[t];
+                if (save_prev) Pb.st(Y[t], v8, s + t);
                 Y[t] = xi;
-                y[o] = xi;
+                Yb.st(xi, v8, s + t);
                 xn = xi;
             }
         }
-        have_prev = q_solve >= 8;
     } else if (q_solve >= 0) {
         return;  // variable finished its steps: nothing further for it in this stage
     }
     if (q_next < 0 || !var_active(nsteps, v, q_next)) return;
-    if (q_next >= 8 && !have_prev) {
+    if (q_next >= 8) {  // y_(n-1) of the BDF2 step, as stored above or by an earlier solve
 #pragma unroll
         for (int t = 0; t < P; t++)
-            if (t < m) T2[t] = yprev[(size_t)(s + t) * nV + v];
+            if (FULL || t < m) T2[t] = Pb.ld(v8, s + t);
     }
-    // ---- chunk summary for solve q_next: downward and upward eliminations of the interior
-    double bt = 1.0, rt = 0.0, at = 0.0, cL = 0.0;
+    const BdfCoef bc(q_next, hv);
+    if (pivots != (q_next >= 8 ? 1 : 0)) down_pivots<P, FULL>(m, bc.sh, A, Bm, Cm, T1);
+    // ---- chunk summary for solve q_next: downward sweep of the fill-in column and the
+    // right-hand side with the pivots T1, upward elimination of the interior
+    double rt = 0.0, at = 0.0;
 #pragma unroll
-    for (int t = 0; t < P; t++) {
-        if (t < m - 1) {
-            double bs, r;
-            bdf_row(q_next, ih8, ih, Bm[t], K[t], Y[t], q_next >= 8 ? T2[t] : 0.0, bs, r);
+    for (int t = 0; t < P - 1; t++) {
+        if (FULL || t < m - 1) {
+            const double r = bc.rhs(Y[t], q_next >= 8 ? T2[t] : 0.0, K[t * 64]);
             if (t == 0) {
-                bt = bs;
                 rt = r;
                 at = A[t];
             } else {
-                const double f = A[t] / bt;
-                bt = bs - f * cL;
+                const double f = A[t] * T1[t - 1];
                 rt = r - f * rt;
                 at = -f * at;
             }
-            cL = Cm[t];
         }
     }
     double bh = 1.0, rh = 0.0, ch = 0.0, aN = 0.0;
 #pragma unroll
-    for (int t = P - 1; t >= 0; t--) {
-        if (t < m - 1) {
-            double bs, r;
-            bdf_row(q_next, ih8, ih, Bm[t], K[t], Y[t], q_next >= 8 ? T2[t] : 0.0, bs, r);
+    for (int t = P - 2; t >= 0; t--) {
+        if (FULL || t < m - 1) {
+            const double bs = Bm[t] - bc.sh;
+            const double r = bc.rhs(Y[t], q_next >= 8 ? T2[t] : 0.0, K[t * 64]);
             if (t == m - 2) {
                 bh = bs;
                 rh = r;
                 ch = Cm[t];
             } else {
-                const double g = Cm[t] / bh;
+                const double g = Cm[t] * rcp_nr(bh);
                 bh = bs - g * aN;
                 rh = r - g * rh;
                 ch = -g * ch;
@@ -553,38 +639,64 @@ __global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const int
 #pragma unroll
     for (int t = 0; t < P; t++) {
         if (t == m - 1) {
-            bdf_row(q_next, ih8, ih, Bm[t], K[t], Y[t], q_next >= 8 ? T2[t] : 0.0, be, re);
+            be = Bm[t] - bc.sh;
+            re = bc.rhs(Y[t], q_next >= 8 ? T2[t] : 0.0, K[t * 64]);
             ae = A[t];
         }
     }
-    double* o = own + ((size_t)chunk * nV + v) * 3;
-    double* n = nxt + ((size_t)chunk * nV + v) * 3;
+    double o0 = ae, o1 = be, o2 = re;
     if (m > 1) {
-        const double ib = 1.0 / bt;
-        o[0] = -ae * at * ib;
-        o[1] = be - ae * cL * ib;
-        o[2] = re - ae * rt * ib;
-    } else {
-        o[0] = ae;
-        o[1] = be;
-        o[2] = re;
+        double ib = 0.0, cL = 0.0;
+#pragma unroll
+        for (int t = 0; t < P - 1; t++)
+            if (t == m - 2) {
+                ib = T1[t];
+                cL = Cm[t];
+            }
+        o0 = -ae * at * ib;
+        o1 = be - ae * cL * ib;
+        o2 = re - ae * rt * ib;
     }
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0;
     if (s > 0) {
-        double ap, bp, cp, kp;
-        R.matrix(v, s - 1, ap, bp, cp, kp);
+        n1 = cp;
         if (m > 1) {
-            const double ib = 1.0 / bh;
-            n[0] = -cp * aN * ib;
-            n[1] = -cp * ch * ib;
-            n[2] = -cp * rh * ib;
-        } else {
-            n[0] = 0.0;
-            n[1] = cp;
-            n[2] = 0.0;
+            const double ib = rcp_nr(bh);
+            n0 = -cp * aN * ib;
+            n1 = -cp * ch * ib;
+            n2 = -cp * rh * ib;
         }
-    } else {
-        n[0] = n[1] = n[2] = 0.0;
     }
+    const size_t pl = (size_t)nV * C, e = (size_t)v * C + chunk;
+    own[e] = o0;
+    own[pl + e] = o1;
+    own[2 * pl + e] = o2;
+    nxt[e] = n0;
+    nxt[pl + e] = n1;
+    nxt[2 * pl + e] = n2;
+}
+
+// One wave per chunk of P rows [s, e] (e = separator), lane = variable.
+//  q_solve >= 0: solve q_solve's interior rows with the separators X and write y
+//                (saving y to yprev first when q_solve starts a BDF step);
+//  q_next >= 0:  eliminate the chunk for solve q_next and write its summary (own, nxt).
+template <int P>
+__global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const double* __restrict__ g1,
+                                                        const double* __restrict__ g2,
+                                                        const int* nsteps, int q_solve, int q_next,
+                                                        const double* X, double* own, double* nxt) {
+    const int v = lane_id();
+    const int chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+    const int N = R.N;
+    const int C = (N + P - 1) / P;
+    if (chunk >= C || v >= R.nV) return;
+    // splitConst rows of the chunk, lane-interleaved in LDS (K[t * 64] = row t of this lane)
+    __shared__ double kl[kDiffWaves * P * 64];
+    double* K = kl + (threadIdx.x / 64) * P * 64 + v;
+    if ((chunk + 1) * P <= N)
+        chunk_step<P, true>(R, g1, g2, nsteps, q_solve, q_next, X, own, nxt, K, v, chunk, C);
+    else
+        chunk_step<P, false>(R, g1, g2, nsteps, q_solve, q_next, X, own, nxt, K, v, chunk, C);
 }
 
 // Separators of one variable per workgroup, from the reduced tridiagonal system of the
@@ -606,17 +718,39 @@ __global__ void __launch_bounds__(1024) pcr_kernel(const double* own, const doub
     double* sb = sa + 1024;
     double* sc = sb + 1024;
     double* sr = sc + 1024;
-    ReducedRows RR{own, nxt, C, nV};
+    const size_t pl = (size_t)nV * C;
+    const double* ov = own + (size_t)v * C;
+    const double* nv = nxt + (size_t)v * C;
+    double* Xv = X + (size_t)v * C;
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int i = tid; i < C; i += nt) RR.row(v, i, a[i], b[i], c[i], r[i]);
+    // reduced row i: own part of chunk i plus the interior contribution of chunk i + 1
+    for (int i = tid; i < C; i += nt) {
+        double bi = ov[pl + i], ri = ov[2 * pl + i], ci = 0.0;
+        if (i + 1 < C) {
+            bi += nv[i + 1];
+            ci = nv[pl + i + 1];
+            ri += nv[2 * pl + i + 1];
+        }
+        a[i] = ov[i];
+        b[i] = bi;
+        c[i] = ci;
+        r[i] = ri;
+    }
     __syncthreads();
     const int G = (C + nt - 1) / nt;        // rows per run
     const int S = (C + G - 1) / G;          // runs (separator rows)
     const int s0 = tid * G;
     const int e0 = min(s0 + G, C) - 1;      // separator of this run
-    // run summary (as chunk_summary): separator row in terms of the neighbouring separators
+    // row s0 as loaded (its a/c/r slots carry the run's contribution to the previous
+    // separator below and are reassembled for the back substitution)
+    double ra0 = 0.0, rb0 = 0.0, rc0 = 0.0, rr0 = 0.0;
+    // run summary (as the chunk summary): separator row in terms of the neighbouring separators
     double nb_ = 0.0, nc_ = 0.0, nr_ = 0.0;
     if (tid < S) {
+        ra0 = a[s0];
+        rb0 = b[s0];
+        rc0 = c[s0];
+        rr0 = r[s0];
         double bt = 1.0, rt = 0.0, at = 0.0, cL = 0.0;
         for (int i = s0; i < e0; i++) {
             if (i == s0) {
@@ -662,7 +796,6 @@ __global__ void __launch_bounds__(1024) pcr_kernel(const double* own, const doub
     }
     __syncthreads();  // every run has read its rows and its left neighbour's c
     // pass the contribution to the previous separator through this run's first row slot
-    // (that row is reassembled for the back substitution)
     if (tid < S && tid > 0) {
         a[s0] = nb_;
         c[s0] = nc_;
@@ -712,18 +845,11 @@ __global__ void __launch_bounds__(1024) pcr_kernel(const double* own, const doub
     // back-substitute each run's interior between its neighbouring separators
     if (tid < S) {
         const double xR = sr[tid] / sb[tid];
-        X[(size_t)e0 * nV + v] = xR;
-        double xL = 0.0;
-        if (tid > 0) {
-            const int ep = s0 - 1;  // previous run's separator
-            xL = sr[tid - 1] / sb[tid - 1];
-            (void)ep;
-        }
-        // rows s0 .. e0-1 (the first row's a/c/r were overwritten: reassemble it)
+        Xv[e0] = xR;
+        const double xL = tid > 0 ? sr[tid - 1] / sb[tid - 1] : 0.0;
+        // rows s0 .. e0-1 (row s0 from the values loaded before its slots were reused)
         const int mi = e0 - s0;
         if (mi > 0) {
-            double ra0, rb0, rc0, rr0;
-            RR.row(v, s0, ra0, rb0, rc0, rr0);
             // forward elimination with 1/diag, then back substitution
             double dinv[8], rhs[8];  // G <= kPcrMax / 1024 = 4
             for (int t = 0; t < mi; t++) {
@@ -747,7 +873,7 @@ __global__ void __launch_bounds__(1024) pcr_kernel(const double* own, const doub
                 const int i = s0 + t;
                 const double ci = (t == 0) ? rc0 : c[i];
                 const double xi = (t == mi - 1) ? rhs[t] * dinv[t] : (rhs[t] - ci * xn) * dinv[t];
-                X[(size_t)i * nV + v] = xi;
+                Xv[i] = xi;
                 xn = xi;
             }
         }
@@ -791,8 +917,9 @@ int run_solve(const Rows& R, StateOut out, int n, int nV, const int* nsteps, int
 size_t diffusion_work_doubles(int nV, int N) {
     const int C1 = (N + P1 - 1) / P1;
     const int C2 = (C1 + P2 - 1) / P2;
-    // own1, nxt1 (3 each), X1, own2, nxt2, X2, cp, dp, hmin chunk mins, yprev
-    return (size_t)nV * (6 * C1 + C1 + 6 * C2 + C2 + 2 * (size_t)std::max(C1, C2) + C1 + N);
+    // own1, nxt1 (3 each), X1, own2, nxt2, X2, cp, dp, hmin chunk mins, yprev; grid metrics
+    return (size_t)nV * (6 * C1 + C1 + 6 * C2 + C2 + 2 * (size_t)std::max(C1, C2) + C1 + N) +
+           2 * (size_t)N;
 }
 
 void diffusion_carve(double* base, int nV, int N, DiffWork& w) {
@@ -810,6 +937,8 @@ void diffusion_carve(double* base, int nV, int N, DiffWork& w) {
     w.dp = p; p += Cm * nV;
     w.cmin = p; p += (size_t)C1 * nV;
     w.yprev = p; p += (size_t)N * nV;
+    w.g1 = p; p += N;
+    w.g2 = p; p += N;
 }
 
 namespace {
@@ -852,7 +981,9 @@ int launch_diffusion_solve(const DiffArgs& a, double* y, const double* h, const 
 int launch_diffusion_stage(const DiffArgs& a, double* y, const double* h, const int* nsteps,
                            int nsolve, DiffWork& w, hipStream_t st) {
     const int C1 = (a.N + P1 - 1) / P1;
-    if (C1 > kPcrMax || a.N < 2 * P1) {
+    // the fused path addresses point-major arrays with 32-bit buffer offsets
+    const bool fits32 = (size_t)a.N * a.nV * sizeof(double) < ((size_t)1 << 31);
+    if (C1 > kPcrMax || a.N < 2 * P1 || !fits32) {
         for (int q = 0; q < nsolve; q++) {
             int e = launch_diffusion_solve(a, y, h, nsteps, q, w, st);
             if (e) return e;
@@ -861,15 +992,17 @@ int launch_diffusion_stage(const DiffArgs& a, double* y, const double* h, const 
     }
     DiffRows R{a.B, a.B_row, a.D, a.sc, a.x, y, w.yprev, h, a.N, a.nV, a.alpha,
                a.left, a.right, a.yInf, a.wallConst, a.wallVar, 0};
-    const int wpb = 4;
+    const int wpb = kDiffWaves;
     const dim3 grid((C1 + wpb - 1) / wpb), block(64 * wpb);
     const size_t pcr_lds = ((size_t)4 * C1 + 4 * 1024) * sizeof(double);
-    hipLaunchKernelGGL((diff_step_kernel<P1>), grid, block, 0, st, R, nsteps, -1, 0, w.X1, w.own1,
-                       w.nxt1);
+    hipLaunchKernelGGL(grid_metrics_kernel, dim3((a.N + 255) / 256), dim3(256), 0, st, a.x, a.N,
+                       a.alpha, w.g1, w.g2);
+    hipLaunchKernelGGL((diff_step_kernel<P1>), grid, block, 0, st, R, w.g1, w.g2, nsteps, -1, 0, w.X1,
+                       w.own1, w.nxt1);
     for (int q = 0; q < nsolve; q++) {
         hipLaunchKernelGGL(pcr_kernel, dim3(a.nV), dim3(1024), pcr_lds, st, w.own1, w.nxt1, C1, a.nV,
                            nsteps, q, w.X1);
-        hipLaunchKernelGGL((diff_step_kernel<P1>), grid, block, 0, st, R, nsteps, q,
+        hipLaunchKernelGGL((diff_step_kernel<P1>), grid, block, 0, st, R, w.g1, w.g2, nsteps, q,
                            q + 1 < nsolve ? q + 1 : -1, w.X1, w.own1, w.nxt1);
     }
     return (int)hipGetLastError();
