@@ -699,185 +699,220 @@ __global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const dou
         chunk_step<P, false>(R, g1, g2, nsteps, q_solve, q_next, X, own, nxt, K, v, chunk, C);
 }
 
-// Separators of one variable per workgroup, from the reduced tridiagonal system of the
-// chunk summaries (C rows, C <= kPcrMax) held in LDS.  Each thread first eliminates its
-// own run of G = ceil(C / 1024) consecutive rows down to the run's last row (the same
-// downward/upward elimination as the chunks), the <= 1024 run separators are solved by
-// parallel cyclic reduction (each level couples row i with rows i -/+ stride and doubles
-// the stride), and each thread back-substitutes its run.
-__global__ void __launch_bounds__(1024) pcr_kernel(const double* own, const double* nxt, int C,
-                                                   int nV, const int* nsteps, int q, double* X) {
+// Separators of one variable per workgroup of kPcrThreads threads, from the reduced
+// tridiagonal system of the chunk summaries (C rows, C <= kPcrMax).  The rows are read
+// coalesced into LDS; each thread owns a run of G = ceil(C / kPcrThreads) <= 16
+// consecutive rows and eliminates its interior towards both ends (the same
+// downward/upward elimination as the chunks); the <= kPcrThreads run separators are
+// solved by parallel cyclic reduction (each level couples row i with rows i -/+ stride and
+// doubles the stride; two buffers, one barrier per level), and each thread back-substitutes
+// its run.  Few long runs keep the O(S log S) reduction small: the workgroup is the only
+// one of its variable, so its VALU work is the critical path.
+constexpr int kPcrThreads = 256;
+constexpr int kRunMax = kPcrMax / kPcrThreads;
+
+// LDS index of reduced row i: one pad double per 16 rows, so that the rows of consecutive
+// runs (16 apart) fall into different banks
+__host__ __device__ __forceinline__ int pcr_slot(int i) { return i + (i >> 4); }
+
+size_t pcr_lds_doubles(int C) { return 4 * (size_t)pcr_slot(C) + 2 * 4 * kPcrThreads; }
+
+__global__ void __launch_bounds__(kPcrThreads) pcr_kernel(const double* own, const double* nxt, int C,
+                                                          int nV, const int* nsteps, int q,
+                                                          double* X) {
     const int v = blockIdx.x;
     if (!var_active(nsteps, v, q)) return;  // uniform over the workgroup
-    extern __shared__ double sm[];
-    double* a = sm;
-    double* b = sm + C;
-    double* c = sm + 2 * C;
-    double* r = sm + 3 * C;
-    double* sa = sm + 4 * C;  // separator system, <= 1024 rows
-    double* sb = sa + 1024;
-    double* sc = sb + 1024;
-    double* sr = sc + 1024;
+    extern __shared__ double lds[];
+    const int CS = pcr_slot(C);
+    double* a = lds;
+    double* b = a + CS;
+    double* c = b + CS;
+    double* r = c + CS;
+    double* sm = r + CS;  // separator system: buffer p, array k at sm + (4 p + k) * kPcrThreads
+    auto sep = [&](int p, int k) { return sm + (4 * p + k) * kPcrThreads; };
     const size_t pl = (size_t)nV * C;
     const double* ov = own + (size_t)v * C;
     const double* nv = nxt + (size_t)v * C;
     double* Xv = X + (size_t)v * C;
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = threadIdx.x;
     // reduced row i: own part of chunk i plus the interior contribution of chunk i + 1
-    for (int i = tid; i < C; i += nt) {
-        double bi = ov[pl + i], ri = ov[2 * pl + i], ci = 0.0;
-        if (i + 1 < C) {
-            bi += nv[i + 1];
-            ci = nv[pl + i + 1];
-            ri += nv[2 * pl + i + 1];
+    // (batches of kLoadBatch rows per thread, all loads of a batch in flight together)
+    constexpr int kLoadBatch = 16;
+    for (int kb = 0; kb * kPcrThreads < C; kb += kLoadBatch) {
+        double la[kLoadBatch], lb[kLoadBatch], lc[kLoadBatch], lr[kLoadBatch];
+#pragma unroll
+        for (int k = 0; k < kLoadBatch; k++) {
+            const int i = min(tid + (kb + k) * kPcrThreads, C - 1), i1 = min(i + 1, C - 1);
+            const double last = i + 1 < C ? 1.0 : 0.0;
+            la[k] = ov[i];
+            lb[k] = ov[pl + i] + last * nv[i1];
+            lc[k] = last * nv[pl + i1];
+            lr[k] = ov[2 * pl + i] + last * nv[2 * pl + i1];
         }
-        a[i] = ov[i];
-        b[i] = bi;
-        c[i] = ci;
-        r[i] = ri;
+#pragma unroll
+        for (int k = 0; k < kLoadBatch; k++) {
+            const int i = tid + (kb + k) * kPcrThreads;
+            if (i < C) {
+                const int o = pcr_slot(i);
+                a[o] = la[k];
+                b[o] = lb[k];
+                c[o] = lc[k];
+                r[o] = lr[k];
+            }
+        }
     }
-    __syncthreads();
-    const int G = (C + nt - 1) / nt;        // rows per run
-    const int S = (C + G - 1) / G;          // runs (separator rows)
+    const int G = (C + kPcrThreads - 1) / kPcrThreads;  // rows per run
+    const int S = (C + G - 1) / G;                      // runs (separator rows)
     const int s0 = tid * G;
-    const int e0 = min(s0 + G, C) - 1;      // separator of this run
-    // row s0 as loaded (its a/c/r slots carry the run's contribution to the previous
-    // separator below and are reassembled for the back substitution)
-    double ra0 = 0.0, rb0 = 0.0, rc0 = 0.0, rr0 = 0.0;
-    // run summary (as the chunk summary): separator row in terms of the neighbouring separators
-    double nb_ = 0.0, nc_ = 0.0, nr_ = 0.0;
-    if (tid < S) {
-        ra0 = a[s0];
-        rb0 = b[s0];
-        rc0 = c[s0];
-        rr0 = r[s0];
-        double bt = 1.0, rt = 0.0, at = 0.0, cL = 0.0;
-        for (int i = s0; i < e0; i++) {
-            if (i == s0) {
-                bt = b[i]; rt = r[i]; at = a[i];
-            } else {
-                const double f = a[i] / bt;
-                bt = b[i] - f * cL; rt = r[i] - f * rt; at = -f * at;
-            }
-            cL = c[i];
-        }
-        double bh = 1.0, rh = 0.0, ch = 0.0, aN = 0.0;
-        for (int i = e0 - 1; i >= s0; i--) {
-            if (i == e0 - 1) {
-                bh = b[i]; rh = r[i]; ch = c[i];
-            } else {
-                const double g = c[i] / bh;
-                bh = b[i] - g * aN; rh = r[i] - g * rh; ch = -g * ch;
-            }
-            aN = a[i];
-        }
-        double oa = a[e0], ob = b[e0], orr = r[e0];
-        if (e0 > s0) {
-            oa = -a[e0] * at / bt;
-            ob = b[e0] - a[e0] * cL / bt;
-            orr = r[e0] - a[e0] * rt / bt;
-        }
-        // own part of separator row tid
-        sa[tid] = oa;
-        sb[tid] = ob;
-        sr[tid] = orr;
-        // contribution of this run's interior to the previous separator's row (tid - 1)
-        if (tid > 0) {
-            const double cp = c[s0 - 1];
-            nb_ = 0.0;
-            nc_ = cp;
-            nr_ = 0.0;
-            if (e0 > s0) {
-                nb_ = -cp * aN / bh;
-                nc_ = -cp * ch / bh;
-                nr_ = -cp * rh / bh;
-            }
-        }
+    const int e0 = min(s0 + G, C) - 1;  // separator of this run
+    const int mi = e0 - s0;             // interior rows of the run
+    __syncthreads();
+    // the run's rows in registers (rows past the run are clamped copies, never used)
+    double ra[kRunMax], rb[kRunMax], rc[kRunMax], rr[kRunMax];
+#pragma unroll
+    for (int k = 0; k < kRunMax; k++) {
+        const int o = pcr_slot(min(s0 + k, C - 1));
+        ra[k] = a[o];
+        rb[k] = b[o];
+        rc[k] = c[o];
+        rr[k] = r[o];
     }
-    __syncthreads();  // every run has read its rows and its left neighbour's c
-    // pass the contribution to the previous separator through this run's first row slot
-    if (tid < S && tid > 0) {
-        a[s0] = nb_;
-        c[s0] = nc_;
-        r[s0] = nr_;
+    const double cp = tid > 0 && tid < S ? c[pcr_slot(s0 - 1)] : 0.0;
+    if (tid < S) {
+        double aE = 0.0, bE = 0.0, rE = 0.0;
+#pragma unroll
+        for (int k = 0; k < kRunMax; k++)
+            if (k == mi) {
+                aE = ra[k];
+                bE = rb[k];
+                rE = rr[k];
+            }
+        // downward over the interior: the separator row in terms of the previous separator
+        double bt = 1.0, rt = 0.0, at = 0.0, cL = 0.0;
+#pragma unroll
+        for (int k = 0; k < kRunMax; k++) {
+            if (k < mi) {
+                if (k == 0) {
+                    bt = rb[k]; rt = rr[k]; at = ra[k];
+                } else {
+                    const double f = ra[k] * rcp_nr(bt);
+                    bt = rb[k] - f * cL; rt = rr[k] - f * rt; at = -f * at;
+                }
+                cL = rc[k];
+            }
+        }
+        double oa = aE, ob = bE, orr = rE;
+        if (mi > 0) {
+            const double ib = rcp_nr(bt);
+            oa = -aE * at * ib;
+            ob = bE - aE * cL * ib;
+            orr = rE - aE * rt * ib;
+        }
+        // upward over the interior: the run's contribution to the previous separator's row
+        double nb_ = 0.0, nc_ = cp, nr_ = 0.0;
+        if (tid > 0 && mi > 0) {
+            double bh = 1.0, rh = 0.0, ch = 0.0, aN = 0.0;
+#pragma unroll
+            for (int k = kRunMax - 1; k >= 0; k--) {
+                if (k < mi) {
+                    if (k == mi - 1) {
+                        bh = rb[k]; rh = rr[k]; ch = rc[k];
+                    } else {
+                        const double g = rc[k] * rcp_nr(bh);
+                        bh = rb[k] - g * aN; rh = rr[k] - g * rh; ch = -g * ch;
+                    }
+                    aN = ra[k];
+                }
+            }
+            const double ib = rcp_nr(bh);
+            nb_ = -cp * aN * ib;
+            nc_ = -cp * ch * ib;
+            nr_ = -cp * rh * ib;
+        }
+        sep(0, 0)[tid] = oa;
+        sep(0, 1)[tid] = ob;
+        sep(0, 3)[tid] = orr;
+        sep(1, 1)[tid] = nb_;  // exchange slots (buffer 1 is free until the first level)
+        sep(1, 2)[tid] = nc_;
+        sep(1, 3)[tid] = nr_;
     }
     __syncthreads();
     if (tid < S) {
         double cc = 0.0;
         if (tid + 1 < S) {
-            const int sn = (tid + 1) * G;
-            sb[tid] += a[sn];
-            cc = c[sn];
-            sr[tid] += r[sn];
+            sep(0, 1)[tid] += sep(1, 1)[tid + 1];
+            cc = sep(1, 2)[tid + 1];
+            sep(0, 3)[tid] += sep(1, 3)[tid + 1];
         }
-        sc[tid] = cc;
+        sep(0, 2)[tid] = cc;
     }
     __syncthreads();
     // parallel cyclic reduction on the S separator rows, one row per thread
+    int p = 0;
     for (int st = 1; st < S; st <<= 1) {
-        double na = 0.0, nb = 0.0, nc = 0.0, nr = 0.0;
         if (tid < S) {
+            const double* sa = sep(p, 0);
+            const double* sb = sep(p, 1);
+            const double* sc = sep(p, 2);
+            const double* sr = sep(p, 3);
             const int lo = tid - st, hi = tid + st;
-            nb = sb[tid];
-            nr = sr[tid];
+            double na = 0.0, nc = 0.0, nb = sb[tid], nr = sr[tid];
             if (lo >= 0) {
-                const double al = -sa[tid] / sb[lo];
+                const double al = -sa[tid] * rcp_nr(sb[lo]);
                 na = al * sa[lo];
                 nb += al * sc[lo];
                 nr += al * sr[lo];
             }
             if (hi < S) {
-                const double ga = -sc[tid] / sb[hi];
+                const double ga = -sc[tid] * rcp_nr(sb[hi]);
                 nc = ga * sc[hi];
                 nb += ga * sa[hi];
                 nr += ga * sr[hi];
             }
+            sep(1 - p, 0)[tid] = na;
+            sep(1 - p, 1)[tid] = nb;
+            sep(1 - p, 2)[tid] = nc;
+            sep(1 - p, 3)[tid] = nr;
         }
-        __syncthreads();
-        if (tid < S) {
-            sa[tid] = na;
-            sb[tid] = nb;
-            sc[tid] = nc;
-            sr[tid] = nr;
-        }
+        p = 1 - p;
         __syncthreads();
     }
     // back-substitute each run's interior between its neighbouring separators
     if (tid < S) {
-        const double xR = sr[tid] / sb[tid];
-        Xv[e0] = xR;
-        const double xL = tid > 0 ? sr[tid - 1] / sb[tid - 1] : 0.0;
-        // rows s0 .. e0-1 (row s0 from the values loaded before its slots were reused)
-        const int mi = e0 - s0;
-        if (mi > 0) {
-            // forward elimination with 1/diag, then back substitution
-            double dinv[8], rhs[8];  // G <= kPcrMax / 1024 = 4
-            for (int t = 0; t < mi; t++) {
-                const int i = s0 + t;
-                const double ai = t == 0 ? ra0 : a[i], bi = t == 0 ? rb0 : b[i];
-                double ri = t == 0 ? rr0 : r[i];
-                if (t == 0) ri -= ai * xL;
-                if (t == mi - 1) ri -= (t == 0 ? rc0 : c[i]) * xR;
-                if (t == 0) {
-                    dinv[t] = 1.0 / bi;
-                    rhs[t] = ri;
+        const double xR = sep(p, 3)[tid] / sep(p, 1)[tid];
+        const double xL = tid > 0 ? sep(p, 3)[tid - 1] / sep(p, 1)[tid - 1] : 0.0;
+        // forward elimination with 1/diag (in ra) and swept right-hand sides (in rr), then
+        // back substitution
+#pragma unroll
+        for (int k = 0; k < kRunMax; k++) {
+            if (k < mi) {
+                double ri = rr[k];
+                if (k == 0) ri -= ra[k] * xL;
+                if (k == mi - 1) ri -= rc[k] * xR;
+                if (k == 0) {
+                    ra[k] = rcp_nr(rb[k]);
                 } else {
-                    const double cprev = (t - 1 == 0) ? rc0 : c[i - 1];
-                    const double f = ai * dinv[t - 1];
-                    dinv[t] = 1.0 / (bi - f * cprev);
-                    rhs[t] = ri - f * rhs[t - 1];
+                    const double f = ra[k] * ra[k - 1];
+                    ra[k] = rcp_nr(rb[k] - f * rc[k - 1]);
+                    ri -= f * rr[k - 1];
                 }
+                rr[k] = ri;
             }
-            double xn = xR;
-            for (int t = mi - 1; t >= 0; t--) {
-                const int i = s0 + t;
-                const double ci = (t == 0) ? rc0 : c[i];
-                const double xi = (t == mi - 1) ? rhs[t] * dinv[t] : (rhs[t] - ci * xn) * dinv[t];
-                Xv[i] = xi;
+        }
+        double xn = xR;
+#pragma unroll
+        for (int k = kRunMax - 1; k >= 0; k--) {
+            if (k < mi) {
+                const double xi = (k == mi - 1) ? rr[k] * ra[k] : (rr[k] - rc[k] * xn) * ra[k];
+                r[pcr_slot(s0 + k)] = xi;
                 xn = xi;
             }
         }
+        r[pcr_slot(e0)] = xR;
     }
+    __syncthreads();
+    // coalesced store of the separators (each run wrote its rows' values into the r slots)
+    for (int i = tid; i < C; i += kPcrThreads) Xv[i] = r[pcr_slot(i)];
 }
 
 constexpr int P1 = 16;   // level-1 chunk (points)
@@ -994,13 +1029,13 @@ int launch_diffusion_stage(const DiffArgs& a, double* y, const double* h, const 
                a.left, a.right, a.yInf, a.wallConst, a.wallVar, 0};
     const int wpb = kDiffWaves;
     const dim3 grid((C1 + wpb - 1) / wpb), block(64 * wpb);
-    const size_t pcr_lds = ((size_t)4 * C1 + 4 * 1024) * sizeof(double);
+    const size_t pcr_lds = pcr_lds_doubles(C1) * sizeof(double);
     hipLaunchKernelGGL(grid_metrics_kernel, dim3((a.N + 255) / 256), dim3(256), 0, st, a.x, a.N,
                        a.alpha, w.g1, w.g2);
     hipLaunchKernelGGL((diff_step_kernel<P1>), grid, block, 0, st, R, w.g1, w.g2, nsteps, -1, 0, w.X1,
                        w.own1, w.nxt1);
     for (int q = 0; q < nsolve; q++) {
-        hipLaunchKernelGGL(pcr_kernel, dim3(a.nV), dim3(1024), pcr_lds, st, w.own1, w.nxt1, C1, a.nV,
+        hipLaunchKernelGGL(pcr_kernel, dim3(a.nV), dim3(kPcrThreads), pcr_lds, st, w.own1, w.nxt1, C1, a.nV,
                            nsteps, q, w.X1);
         hipLaunchKernelGGL((diff_step_kernel<P1>), grid, block, 0, st, R, w.g1, w.g2, nsteps, q,
                            q + 1 < nsolve ? q + 1 : -1, w.X1, w.own1, w.nxt1);
