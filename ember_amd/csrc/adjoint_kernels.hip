@@ -187,13 +187,13 @@ __device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, d
                     work = R.work[j];
                     pr0 = Mgv * klow / (khigh + kSmallNumber);
                     if (kind == kTroe) {
-                        const double lpr = log10(fmax(pr0, kSmallNumber));
+                        const double lpr = log(fmax(pr0, kSmallNumber)) * kInvLn10;
                         const double cc = -0.4 - 0.67 * work;
                         nn = 0.75 - 1.27 * work;
                         u = lpr + cc;
                         f1 = u / (nn - 0.14 * u);
                         const double lf = work / (1.0 + f1 * f1);
-                        F = pow(10.0, lf);
+                        F = exp(kLn10 * lf);
                     }
                     kf = pr0 * (F / (1.0 + pr0)) * khigh;
                 }
@@ -223,7 +223,7 @@ __device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, d
                     const double Fb = prfb * pr0 / (1.0 + pr0);
                     double pr0b = prfb * F / ((1.0 + pr0) * (1.0 + pr0));
                     if (kind == kTroe) {
-                        const double ln10 = 2.302585092994045684;
+                        const double ln10 = kLn10;
                         const double lfb = Fb * F * ln10;
                         const double gg = 1.0 + f1 * f1;
                         double workb = lfb / gg;

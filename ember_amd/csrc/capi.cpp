@@ -136,6 +136,7 @@ int upload_mech(emx_context* c) {
     d.interp = c->opts.kinetics_model == EMX_KIN_INTERP ? 1 : 0;
     d.P = c->P;
     d.Pref = m.Pref;
+    d.lnRPref = std::log(kGasConstant / m.Pref);
     std::vector<double> kf_c, kf_s, kc_c, kc_s;
     if (d.interp) build_interp_tables(m, L, c->P, kf_c, kf_s, kc_c, kc_s);
     Packer pk;

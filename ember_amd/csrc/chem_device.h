@@ -25,7 +25,8 @@ namespace emx {
 // per-wave LDS scratch (doubles)
 struct WaveLds {
     double* C;      // [64] concentrations
-    double* mu;     // [64] standard chemical potentials
+    double* ge;     // [64] exp(mu_k / RT - ln c0): standard-state factors of 1/Kc
+    double* gi;     // [64] 1 / ge
     double* Mg;     // [64] third-body concentrations per efficiency group
     double* ropf;   // [nRounds*64 + 1]: the last entry is the zero slot
     double* ropr;   // [nRounds*64 + 1]
@@ -34,28 +35,30 @@ struct WaveLds {
 };
 
 __host__ __device__ inline int wave_lds_doubles(int nRounds, int nChunkRounds) {
-    return 3 * 64 + 2 * (nRounds * 64 + 1) + 2 * nChunkRounds * 64;
+    return 4 * 64 + 2 * (nRounds * 64 + 1) + 2 * nChunkRounds * 64;
 }
 
 __device__ inline WaveLds carve_lds(double* base, int nRounds, int nChunkRounds) {
     WaveLds w;
     w.C = base;
-    w.mu = base + 64;
-    w.Mg = base + 128;
-    w.ropf = base + 192;
+    w.ge = base + 64;
+    w.gi = base + 128;
+    w.Mg = base + 192;
+    w.ropf = base + 256;
     w.ropr = w.ropf + nRounds * 64 + 1;
     w.partq = w.ropr + nRounds * 64 + 1;
     w.partd = w.partq + nChunkRounds * 64;
     return w;
 }
 
-// Constant cells of the per-wave scratch: C[kPadSpecies] = 1 and mu[kPadSpecies] = 0
-// (stoichiometry padding), ropf/ropr zero slot (species-sum padding).  Species lanes
-// never write index kPadSpecies since K <= 62.
+// Constant cells of the per-wave scratch: C[kPadSpecies] = ge = gi = 1 (stoichiometry
+// padding), ropf/ropr zero slot (species-sum padding).  Species lanes never write index
+// kPadSpecies since K <= 62.
 __device__ inline void init_wave_lds(const WaveLds& L, int lane, int nRounds) {
     if (lane == 0) {
         L.C[kPadSpecies] = 1.0;
-        L.mu[kPadSpecies] = 0.0;
+        L.ge[kPadSpecies] = 1.0;
+        L.gi[kPadSpecies] = 1.0;
         L.ropf[nRounds * 64] = 0.0;
         L.ropr[nRounds * 64] = 0.0;
     }
@@ -179,14 +182,20 @@ __device__ __forceinline__ double arrhenius(double A, double b, double EaR, doub
     return A * exp(b * logT - EaR * recipT);
 }
 
-// Products and sums over the (padded) three stoichiometry indices: the padding cells
-// (C = 1, mu = 0) leave the one- and two-species results unchanged bit for bit.
+// Products over the (padded) three stoichiometry indices: the padding cells (C = 1)
+// leave the one- and two-species results unchanged bit for bit.
 __device__ __forceinline__ double conc_product(uint32_t st, const double* C) {
     return C[st_k(st, 0)] * C[st_k(st, 1)] * C[st_k(st, 2)];
 }
 
-__device__ __forceinline__ double mu_sum(uint32_t st, const double* mu) {
-    return mu[st_k(st, 0)] + mu[st_k(st, 1)] + mu[st_k(st, 2)];
+// 1/Kc = exp(sum_k nu_k mu_k / RT - dn ln c0) (reactants negative, dn = sum of the product
+// minus the reactant coefficients), as the product of the per-species factors
+// ge_k = exp(mu_k / RT - ln c0) of the products and 1/ge_k of the reactants: one exp per
+// species instead of one per reaction.  Pairing product and reactant factors keeps the
+// partial products in range; the padding cells are 1.
+__device__ __forceinline__ double inv_kc(uint32_t reac, uint32_t prod, const WaveLds& L) {
+    return (L.ge[st_k(prod, 0)] * L.gi[st_k(reac, 0)]) * (L.ge[st_k(prod, 1)] * L.gi[st_k(reac, 1)]) *
+           (L.ge[st_k(prod, 2)] * L.gi[st_k(reac, 2)]);
 }
 
 // Thermo for this lane's species plus the T-dependent rate data, cached on T.
@@ -198,7 +207,6 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
     const double T = g.T;
     const double logT = log(T);
     const double recipT = 1.0 / T;
-    const double RT = T * kGasConstant;
     const bool sp = is_species_lane(lane, m);
     double sR = 0.0;
     hRT = 0.0;
@@ -220,12 +228,15 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
             }
         }
     } else {
-        // IdealGasPhase::getStandardChemPotentials; pressure() = R * molarDensity * T
-        const double pressure = kGasConstant * (g.dens / g.mmw) * T;
-        const double tmp = log(pressure / m.Pref) * RT;
-        if (sp) L.mu[lane - 2] = RT * (hRT - sR) + tmp;
-        const double logStandConc = log(pressure / RT);
-        const double rrt = 1.0 / RT;
+        // IdealGasPhase::getStandardChemPotentials: mu_k = RT (h_k/RT - s_k/R) + RT ln(P/Pref);
+        // with the standard concentration c0 = P / RT the pressure cancels from
+        // mu_k / RT - ln c0 = h_k/RT - s_k/R + ln T + ln(R / Pref)
+        if (sp) {
+            const double x = (hRT - sR) + (logT + m.lnRPref);
+            const double e = exp(x);
+            L.ge[lane - 2] = e;
+            L.gi[lane - 2] = 1.0 / e;
+        }
         wave_lds_sync();
 #pragma unroll
         for (int j = 0; j < NR; j++) {
@@ -235,12 +246,7 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
                 int kind = info_kind(w);
                 double rkc = 0.0, kf = 0.0;
                 if (kind != kNone) {
-                    if (info_rev(w)) {
-                        double dg = 0.0;
-                        dg += mu_sum(m.prod[s], L.mu);
-                        dg -= mu_sum(m.reac[s], L.mu);
-                        rkc = fmin(exp(dg * rrt - m.dn[s] * logStandConc), kBigNumber);
-                    }
+                    if (info_rev(w)) rkc = fmin(inv_kc(m.reac[s], m.prod[s], L), kBigNumber);
                     if (kind <= kThreeBody) kf = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT);
                 }
                 R.rkc[j] = rkc;
@@ -261,7 +267,7 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
                 if (kind == kTroe) {
                     double Fcent = (1.0 - m.troeA[s]) * exp(-T * m.rT3[s]) + m.troeA[s] * exp(-T * m.rT1[s]);
                     if (m.T2[s] != 0.0) Fcent += exp(-m.T2[s] / T);
-                    work = log10(fmax(Fcent, kSmallNumber));
+                    work = log(fmax(Fcent, kSmallNumber)) * kInvLn10;
                 }
             }
             R.klow[j] = klow;
@@ -308,12 +314,12 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
                     double F = 1.0;
                     if (kind == kTroe) {
                         double work = R.work[j];
-                        double lpr = log10(fmax(pr, kSmallNumber));
+                        double lpr = log(fmax(pr, kSmallNumber)) * kInvLn10;
                         double cc = -0.4 - 0.67 * work;
                         double nn = 0.75 - 1.27 * work;
                         double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
                         double lf = work / (1.0 + f1 * f1);
-                        F = pow(10.0, lf);
+                        F = exp(kLn10 * lf);
                     }
                     pr *= F / (1.0 + pr);
                     kf = pr * khigh;

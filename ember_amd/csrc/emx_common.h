@@ -19,6 +19,8 @@ constexpr double kInterpTmax = 3500.0;
 constexpr double kGasConstant = 6.02214076e26 * 1.380649e-23;  // J/kmol/K
 constexpr double kSmallNumber = 1e-300;
 constexpr double kBigNumber = 1e300;
+constexpr double kLn10 = 2.302585092994045684;  // ln 10: Troe's 10^x evaluated as exp(x ln 10)
+constexpr double kInvLn10 = 0.43429448190325182765;  // 1 / ln 10: log10(x) = ln(x) / ln 10
 
 // reaction kinds in device order (falloff first, then three-body, then elementary)
 enum RxnKind : int { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3, kNone = 4 };
@@ -52,6 +54,7 @@ struct DevMech {
     int nGrpEntries;       // third-body efficiency entries
     int interp;            // 1: InterpKinetics tables
     double P, Pref;
+    double lnRPref;  // ln(R / Pref): standard-state term of the 1/Kc factors
     // species [K]
     const double* W;
     const double* rW;
