@@ -88,7 +88,7 @@ __device__ inline void predictor_thermo(const DevMech& m, int lane, double y, do
     const double rW = sp ? m.rW[lane - 2] : 0.0;
     GasState g = set_state(m, lane, y, rW, T, false, L);
     double cpR = 0.0, hRT = 0.0, sR = 0.0;
-    if (sp) nasa(m, lane - 2, T, log(T), cpR, hRT, sR);
+    if (sp) nasa(m, lane - 2, T, emx_log(T), cpR, hRT, sR);
     hk = sp ? g.T * kGasConstant * hRT : 0.0;
     rho = g.dens;
     const double s = wave_sum(g.ym * cpR);
@@ -153,7 +153,7 @@ __device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, d
     }
     wave_lds_sync();
     // ---- reaction slots: rates of progress -> base rate, rate constants, concentrations
-    const double logT = log(T), recipT = 1.0 / T;
+    const double logT = emx_log(T), recipT = 1.0 / T;
     size_t nI = 0;
     if (INTERP) nI = (size_t)floor((T - kInterpTmin) / (kInterpTmax - kInterpTmin) * (kInterpN - 1));
     double Trl = 0.0;  // this lane's T adjoint contributions
@@ -187,13 +187,13 @@ __device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, d
                     work = R.work[j];
                     pr0 = Mgv * klow / (khigh + kSmallNumber);
                     if (kind == kTroe) {
-                        const double lpr = log(fmax(pr0, kSmallNumber)) * kInvLn10;
+                        const double lpr = emx_log(fmax(pr0, kSmallNumber)) * kInvLn10;
                         const double cc = -0.4 - 0.67 * work;
                         nn = 0.75 - 1.27 * work;
                         u = lpr + cc;
                         f1 = u / (nn - 0.14 * u);
                         const double lf = work / (1.0 + f1 * f1);
-                        F = exp(kLn10 * lf);
+                        F = emx_exp(kLn10 * lf);
                     }
                     kf = pr0 * (F / (1.0 + pr0)) * khigh;
                 }
@@ -234,13 +234,13 @@ __device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, d
                         workb += -0.67 * ub - 1.27 * nnb;
                         if (pr0 > kSmallNumber) pr0b += ub / (pr0 * ln10);
                         const double a3 = m.troeA[s], rT3 = m.rT3[s], rT1 = m.rT1[s], T2 = m.T2[s];
-                        const double e3 = exp(-T * rT3), e1 = exp(-T * rT1);
+                        const double e3 = emx_exp(-T * rT3), e1 = emx_exp(-T * rT1);
                         double Fcent = (1.0 - a3) * e3 + a3 * e1;
                         double dF = 0.0;
                         if (rT3 < kBigNumber) dF += -(1.0 - a3) * rT3 * e3;
                         if (rT1 < kBigNumber) dF += -a3 * rT1 * e1;
                         if (T2 != 0.0) {
-                            const double e2 = exp(-T2 / T);
+                            const double e2 = emx_exp(-T2 / T);
                             Fcent += e2;
                             dF += T2 * recipT * recipT * e2;
                         }

@@ -18,6 +18,7 @@
 #pragma once
 
 #include "emx_common.h"
+#include "emx_math.h"
 #include "wave.h"
 
 namespace emx {
@@ -25,7 +26,7 @@ namespace emx {
 // per-wave LDS scratch (doubles)
 struct WaveLds {
     double* C;      // [64] concentrations
-    double* ge;     // [64] exp(mu_k / RT - ln c0): standard-state factors of 1/Kc
+    double* ge;     // [64] emx_exp(mu_k / RT - ln c0): standard-state factors of 1/Kc
     double* gi;     // [64] 1 / ge
     double* Mg;     // [64] third-body concentrations per efficiency group
     double* ropf;   // [nRounds*64 + 1]: the last entry is the zero slot
@@ -179,7 +180,7 @@ __device__ inline void nasa(const DevMech& m, int k, double T, double logT, doub
 
 __device__ __forceinline__ double arrhenius(double A, double b, double EaR, double logT,
                                             double recipT) {
-    return A * exp(b * logT - EaR * recipT);
+    return A * emx_exp(b * logT - EaR * recipT);
 }
 
 // Products over the (padded) three stoichiometry indices: the padding cells (C = 1)
@@ -205,7 +206,7 @@ template <int NR, bool INTERP>
 __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, RateRegs<NR>& R,
                                 const WaveLds& L, double& hRT, double& cpR) {
     const double T = g.T;
-    const double logT = log(T);
+    const double logT = emx_log(T);
     const double recipT = 1.0 / T;
     const bool sp = is_species_lane(lane, m);
     double sR = 0.0;
@@ -233,7 +234,7 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
         // mu_k / RT - ln c0 = h_k/RT - s_k/R + ln T + ln(R / Pref)
         if (sp) {
             const double x = (hRT - sR) + (logT + m.lnRPref);
-            const double e = exp(x);
+            const double e = emx_exp(x);
             L.ge[lane - 2] = e;
             L.gi[lane - 2] = 1.0 / e;
         }
@@ -265,9 +266,9 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
                 klow = arrhenius(m.A0[s], m.b0[s], m.EaR0[s], logT, recipT);
                 khigh = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT);
                 if (kind == kTroe) {
-                    double Fcent = (1.0 - m.troeA[s]) * exp(-T * m.rT3[s]) + m.troeA[s] * exp(-T * m.rT1[s]);
-                    if (m.T2[s] != 0.0) Fcent += exp(-m.T2[s] / T);
-                    work = log(fmax(Fcent, kSmallNumber)) * kInvLn10;
+                    double Fcent = (1.0 - m.troeA[s]) * emx_exp(-T * m.rT3[s]) + m.troeA[s] * emx_exp(-T * m.rT1[s]);
+                    if (m.T2[s] != 0.0) Fcent += emx_exp(-m.T2[s] / T);
+                    work = emx_log(fmax(Fcent, kSmallNumber)) * kInvLn10;
                 }
             }
             R.klow[j] = klow;
@@ -314,12 +315,12 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
                     double F = 1.0;
                     if (kind == kTroe) {
                         double work = R.work[j];
-                        double lpr = log(fmax(pr, kSmallNumber)) * kInvLn10;
+                        double lpr = emx_log(fmax(pr, kSmallNumber)) * kInvLn10;
                         double cc = -0.4 - 0.67 * work;
                         double nn = 0.75 - 1.27 * work;
                         double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
                         double lf = work / (1.0 + f1 * f1);
-                        F = exp(kLn10 * lf);
+                        F = emx_exp(kLn10 * lf);
                     }
                     pr *= F / (1.0 + pr);
                     kf = pr * khigh;
