@@ -356,12 +356,9 @@ __device__ inline void species_rates(const DevMech& m, int lane, const WaveLds& 
             double q = 0.0, d = 0.0;
 #pragma unroll
             for (int e = 0; e < kChunkLen; e++) {
-                const int s = en_slot(en[e]);
                 const double nu = (double)en_nu(en[e]);
-                const double f = L.ropf[s], r = L.ropr[s];
-                const bool pr = en_prod(en[e]);
-                q += nu * (pr ? f : r);
-                d += nu * (pr ? r : f);
+                q += nu * L.ropf[en_q(en[e])];  // ropf, zero slot and ropr are one array
+                d += nu * L.ropf[en_d(en[e])];
             }
             L.partq[c] = q;
             L.partd[c] = d;

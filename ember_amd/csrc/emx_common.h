@@ -36,11 +36,19 @@ __host__ __device__ inline int info_hasT2(int32_t w) { return (w >> 4) & 1; }
 __host__ __device__ inline int st_n(uint32_t s) { return s & 0xff; }
 __host__ __device__ inline int st_k(uint32_t s, int j) { return (s >> (8 * (j + 1))) & 0xff; }
 
-// species-sum entry: slot (bits 0..15), multiplicity (16..23), side (24: 1 = product);
-// padding entries have multiplicity 0 and point at the zero slot nRounds * 64
-__host__ __device__ inline int en_slot(uint32_t e) { return e & 0xffff; }
-__host__ __device__ inline int en_nu(uint32_t e) { return (e >> 16) & 0xff; }
-__host__ __device__ inline int en_prod(uint32_t e) { return (e >> 24) & 1; }
+// species-sum entry over the rates of progress held as one array rop = [ropf (nRounds*64
+// slots), zero slot, ropr (nRounds*64 slots)]: index added to creation (bits 0..11), index
+// added to destruction (12..23), multiplicity (24..31).  A product entry of slot s reads
+// (s, S1 + s), a reactant entry (S1 + s, s), S1 = nRounds*64 + 1; padding entries have
+// multiplicity 0 and point at the zero slot nRounds*64 twice.
+__host__ __device__ inline uint32_t en_pack(uint32_t q, uint32_t d, uint32_t nu) {
+    return q | (d << 12) | (nu << 24);
+}
+__host__ __device__ inline int en_q(uint32_t e) { return e & 0xfff; }
+__host__ __device__ inline int en_d(uint32_t e) { return (e >> 12) & 0xfff; }
+__host__ __device__ inline int en_nu(uint32_t e) { return e >> 24; }
+__host__ __device__ inline int en_slot(uint32_t e) { return en_q(e) < en_d(e) ? en_q(e) : en_d(e); }
+__host__ __device__ inline int en_prod(uint32_t e) { return en_q(e) < en_d(e); }
 
 // Device-side mechanism description: pointers into one device allocation.
 struct DevMech {

@@ -148,22 +148,31 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
     }
     if (L.grp_sp.empty()) { L.grp_sp.push_back(0); L.grp_eff.push_back(0.0); }
     if (L.grp_def.empty()) L.grp_def.push_back(0.0);
-    // species-sum entries: products then reactants, slot order
+    // species-sum entries: products then reactants, slot order.  Indices address the
+    // rates of progress as one array (forward rates, zero slot, reverse rates), so that
+    // a product entry adds ropf to creation and ropr to destruction, a reactant entry the
+    // other way round
+    const uint32_t S1 = (uint32_t)(L.nRounds * 64 + 1);
+    auto make_entry = [&](uint32_t slot, uint32_t nu, bool product) {
+        const uint32_t f = slot, r = S1 + slot;
+        return en_pack(product ? f : r, product ? r : f, nu);
+    };
     std::vector<std::vector<uint32_t>> ent(m.K);
     for (int s = 0; s < m.I; s++) {
         const HostRxn& r = m.rxn[order[s]];
         std::map<int, int> cnt;
         for (int j = 0; j < r.np; j++) cnt[r.p[j]]++;
-        for (auto& c : cnt) ent[c.first].push_back((uint32_t)s | ((uint32_t)c.second << 16) | (1u << 24));
+        for (auto& c : cnt) ent[c.first].push_back(make_entry((uint32_t)s, (uint32_t)c.second, true));
     }
     for (int s = 0; s < m.I; s++) {
         const HostRxn& r = m.rxn[order[s]];
         std::map<int, int> cnt;
         for (int j = 0; j < r.nr; j++) cnt[r.r[j]]++;
-        for (auto& c : cnt) ent[c.first].push_back((uint32_t)s | ((uint32_t)c.second << 16));
+        for (auto& c : cnt) ent[c.first].push_back(make_entry((uint32_t)s, (uint32_t)c.second, false));
     }
     // fixed-length chunks of kChunkLen entries; a species' chunks are consecutive
-    const uint32_t zero_entry = (uint32_t)(L.nRounds * 64);  // zero slot, multiplicity 0
+    const uint32_t zs = (uint32_t)(L.nRounds * 64);
+    const uint32_t zero_entry = en_pack(zs, zs, 0);  // zero slot, multiplicity 0
     L.sp_chunk0.assign(m.K, 0);
     L.sp_nchunk.assign(m.K, 0);
     L.entry.clear();
