@@ -124,6 +124,14 @@ def main():
     print(f"[bench] timed {a.steps} stages: {ms:.3f} ms/stage", file=sys.stderr, flush=True)
     value = N / (ms * 1e-3)
     out = d_out.cpu().numpy()
+    # the host-buffer entry point on the same states: copy in, stage, copy out (PCIe included)
+    ctx.source_integrate(states, dt)
+    c0 = time.perf_counter()
+    n_e2e = 5
+    for _ in range(n_e2e):
+        ctx.source_integrate(states, dt)
+    ms_e2e = (time.perf_counter() - c0) / n_e2e * 1e3
+    print(f"[bench] host-buffer stage (H2D + stage + D2H): {ms_e2e:.3f} ms", file=sys.stderr, flush=True)
     st = status.cpu().numpy()
     g = s1.cpu().numpy().astype(np.float64)
     r = s2.cpu().numpy().astype(np.float64)
@@ -156,6 +164,7 @@ def main():
                    "pressure_Pa": mechanism.ONE_ATM, "seed": 2},
         "failed_points": int((st != 0).sum()),
         "schedule": a.schedule, "index_order_ms_per_step": ms_fifo,
+        "host_buffer_ms_per_step": ms_e2e,
     }
     if ms_plain:
         res["plain_gpu"] = {"value": N / (ms_plain * 1e-3), "ms_per_step": ms_plain,
@@ -176,6 +185,11 @@ def main():
         cpu = {"value": N / cpu_s, "unit": "point-integrations/s", "cores": threads,
                "kind": "port", "sample": f"all {N} benchmark states, one stage, OpenMP dynamic,1 "
                                           f"(TBB grain-1 analogue), {cpu_s:.2f} s"}
+        # one core, on the first 128 states
+        c0 = time.perf_counter()
+        om.integrate(states[:128], dt, oopts, nthreads=1)
+        cpu1_s = time.perf_counter() - c0
+        cpu["one_core"] = {"value": 128 / cpu1_s, "sample": f"first 128 benchmark states, {cpu1_s:.2f} s"}
         res["cpu_baseline"] = cpu
         # parity of this run
         dT = np.abs(out[:, 1] - ref[:, 1]) / np.abs(ref[:, 1])
