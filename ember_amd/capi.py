@@ -47,6 +47,11 @@ class Grid(C.Structure):
                 ("wall_var", C.c_int32)]
 
 
+class SplitArrays(C.Structure):
+    _fields_ = [("state", _vp), ("ddt_conv", _vp), ("ddt_diff", _vp), ("ddt_prod", _vp),
+                ("ddt_cross", _vp), ("delta_diff", _vp), ("delta_prod", _vp), ("work", _vp)]
+
+
 # (name, restype, argtypes) for every entry point of include/ember_amd.h
 SIGNATURES = [
     ("emx_default_options", None, [C.POINTER(Options)]),
@@ -77,6 +82,11 @@ SIGNATURES = [
      [_vp, _sz, _sz, _vp, _sz, _vp, _vp, _vp, C.POINTER(Grid), _dp, C.c_double, C.c_double, _vp]),
     ("emx_tridiagonal_integrate", C.c_int,
      [_vp, _sz, _sz, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, _dp]),
+    ("emx_split_step_device", C.c_int,
+     [_vp, _sz, _sz, C.POINTER(SplitArrays), _vp, _sz, _vp, _vp, C.POINTER(Grid), _vp, _dp,
+      C.c_double, C.c_double, C.c_int, _vp]),
+    ("emx_correct_mass_fractions_device", C.c_int, [_vp, _sz, _sz, _vp]),
+    ("emx_heat_release_device", C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp]),
 ]
 
 _lib = None
@@ -266,6 +276,27 @@ class Context:
             C.byref(g), _p(None if h_per_var is None else np.ascontiguousarray(h_per_var, dtype=np.float64)),
             t0, t1, _tptr(state)))
 
+    def split_step_device(self, arrays: "SplitState", B, D, x, t, dt, B_row=None, qloss=None,
+                          alpha=0, left_bc=0, right_bc=0, y_inf=0.0, wall_const=0.0, wall_var=1,
+                          h_per_var=None, balanced=True, status=None):
+        """One device-resident split step (SplitSolver::step) on arrays.state, in place."""
+        N, nV = arrays.state.shape
+        g = Grid(C.cast(C.c_void_p(x.data_ptr()), _dp), alpha, left_bc, right_bc, y_inf, wall_const,
+                 wall_var)
+        sa = SplitArrays(*(_tptr(t_) for t_ in arrays.fields()))
+        hv = None if h_per_var is None else np.ascontiguousarray(h_per_var, dtype=np.float64)
+        _check(lib().emx_split_step_device(
+            self.h, nV, N, C.byref(sa), _tptr(B), B.shape[0], _tptr(B_row), _tptr(D), C.byref(g),
+            _tptr(qloss), _p(hv), t, dt, 1 if balanced else 0, _tptr(status)))
+
+    def correct_mass_fractions_device(self, state):
+        N, nV = state.shape
+        _check(lib().emx_correct_mass_fractions_device(self.h, nV, N, _tptr(state)))
+
+    def heat_release_device(self, state, qdot, status=None):
+        N, nV = state.shape
+        _check(lib().emx_heat_release_device(self.h, nV, N, _tptr(state), _tptr(qdot), _tptr(status)))
+
     def tridiagonal_integrate(self, a, b, c, k, y, h, t0, t1):
         """Constant systems, (nSys, N) arrays.  Returns y after integrateToTime(t1)."""
         a, b, c, k = (np.ascontiguousarray(np.atleast_2d(v), dtype=np.float64) for v in (a, b, c, k))
@@ -275,3 +306,21 @@ class Context:
         _check(lib().emx_tridiagonal_integrate(self.h, nS, N, _p(a), _p(b), _p(c), _p(k), _p(h), t0,
                                                t1, _p(yy)))
         return yy
+
+
+class SplitState:
+    """Device matrices of SplitSolver's bookkeeping for emx_split_step_device
+    (torch CUDA tensors, (nPoints, nVars) = Ember's point-contiguous layout)."""
+
+    def __init__(self, state, ddt_cross=None):
+        import torch
+        self.state = state
+        z = lambda: torch.zeros_like(state)
+        self.ddt_conv, self.ddt_diff, self.ddt_prod = z(), z(), z()
+        self.ddt_cross = ddt_cross
+        self.delta_diff, self.delta_prod = z(), z()
+        self.work = torch.zeros((3,) + tuple(state.shape), dtype=state.dtype, device=state.device)
+
+    def fields(self):
+        return (self.state, self.ddt_conv, self.ddt_diff, self.ddt_prod, self.ddt_cross,
+                self.delta_diff, self.delta_prod, self.work)

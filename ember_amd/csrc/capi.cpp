@@ -654,6 +654,108 @@ int emx_diffusion_integrate(emx_context* c, size_t nVars, size_t nPoints, const 
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// device-resident split step
+// ---------------------------------------------------------------------------
+// Ember's diffusion step rule (src/flameSolver.cpp:709-711) for every variable, from one
+// read of the per-variable grid minimum.
+static int diffusion_steps(emx_context* c, const DiffArgs& a, std::vector<double>& h) {
+    hipStream_t st = c->stream;
+    const int nV = a.nV, N = a.N;
+    HIP_TRY(c->d_work.ensure(diffusion_work_doubles(nV, N) * sizeof(double)));
+    DiffWork w;
+    diffusion_carve(c->d_work.as<double>(), nV, N, w);
+    HIP_TRY(c->d_hmin.ensure(nV * sizeof(double)));
+    if (launch_diffusion_hmin(a, w, c->d_hmin.as<double>(), st)) return fail(-10, "hmin launch failed");
+    std::vector<double> hm(nV);
+    HIP_TRY(hipMemcpyAsync(hm.data(), c->d_hmin.p, nV * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    h.resize(nV);
+    for (int v = 0; v < nV; v++)
+        h[v] = std::min(c->opts.diffusion_timestep_multiplier * hm[v], c->opts.global_timestep);
+    return 0;
+}
+
+int emx_split_step_device(emx_context* c, size_t nVars, size_t nPoints, const emx_split_arrays* s,
+                          const double* B, size_t nB, const int32_t* B_row, const double* D,
+                          const emx_grid* g, const double* qloss, const double* h_per_var, double t,
+                          double dt, int balanced, int32_t* status) {
+    if (int rc = check_diff(c, nVars, nPoints, g)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (!s || !s->state || !s->ddt_conv || !s->ddt_diff || !s->ddt_prod || !s->delta_diff ||
+        !s->delta_prod || !s->work)
+        return fail(-1, "split arrays: only ddt_cross may be NULL");
+    if (!B_row && nB != nVars) return fail(-1, "B_row is NULL but nB != nVars");
+    if (!(dt > 0.0)) return fail(-1, "dt must be positive");
+    hipStream_t st = c->stream;
+    const size_t n = nVars * nPoints;
+    double* state = s->state;
+    double* start = s->work;
+    double* sc_diff = s->work + n;
+    double* sc_prod = s->work + 2 * n;
+    DiffArgs a{B, B_row, D, sc_diff, g->x, (int)nPoints, (int)nVars, g->alpha, g->left_bc,
+               g->right_bc, g->y_inf, g->wall_const, g->wall_var};
+    std::vector<double> h;
+    if (h_per_var) {
+        h.assign(h_per_var, h_per_var + nVars);
+    } else if (int rc = diffusion_steps(c, a, h)) {
+        return rc;
+    }
+    HIP_TRY(c->b_status.ensure(nPoints * sizeof(int32_t)));
+    HIP_TRY(c->b_s1.ensure(nPoints * sizeof(int32_t)));
+    HIP_TRY(c->b_s2.ensure(nPoints * sizeof(int32_t)));
+    auto launched = [&](int e, const char* what) {
+        return e ? fail(-10, std::string(what) + ": " + hipGetErrorString((hipError_t)e)) : 0;
+    };
+    // SplitSolver::step: split constants from the previous derivatives
+    if (int rc = launched(launch_split_constants(n, balanced, s->ddt_conv, s->ddt_diff, s->ddt_prod,
+                                                 s->ddt_cross, sc_diff, sc_prod, st), "split constants"))
+        return rc;
+    // integrateSplitTerms (src/splitSolver.cpp:60-75); the convection stages change nothing
+    if (int rc = launched(launch_fill(n, 0.0, s->delta_diff, s->delta_prod, st), "fill")) return rc;
+    auto diffusion_stage = [&](double t0, double t1) -> int {
+        if (int rc = launched(launch_copy(n, state, start, st), "copy")) return rc;
+        if (int rc = diffusion_run(c, a, h.data(), t0, t1, state)) return rc;
+        return launched(launch_accumulate(n, state, start, s->delta_diff, st), "accumulate");
+    };
+    if (int rc = diffusion_stage(t, t + 0.25 * dt)) return rc;
+    if (int rc = diffusion_stage(t + 0.25 * dt, t + 0.5 * dt)) return rc;
+    if (int rc = launched(launch_copy(n, state, start, st), "copy")) return rc;
+    // the stage length as SplitSolver forms it: tStageEnd - tStageStart
+    const double dt_prod = (t + dt) - t;
+    if (int rc = emx_source_integrate_device(c, nPoints, nVars, start, sc_prod, qloss, t, dt_prod, state,
+                                             status ? status : c->b_status.as<int32_t>(),
+                                             c->b_s1.as<int32_t>(), c->b_s2.as<int32_t>(), nullptr))
+        return rc;
+    if (int rc = launched(launch_accumulate(n, state, start, s->delta_prod, st), "accumulate")) return rc;
+    if (int rc = diffusion_stage(t + 0.5 * dt, t + 0.75 * dt)) return rc;
+    if (int rc = diffusion_stage(t + 0.75 * dt, t + dt)) return rc;
+    // calculateTimeDerivatives
+    return launched(launch_time_derivatives(n, dt, s->delta_diff, s->delta_prod, sc_diff, sc_prod,
+                                            s->ddt_diff, s->ddt_prod, st), "time derivatives");
+}
+
+int emx_correct_mass_fractions_device(emx_context* c, size_t nVars, size_t nPoints, double* state) {
+    if (int rc = check_ctx(c)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
+    if (nPoints == 0) return 0;
+    if (!state) return fail(-1, "state is NULL");
+    int e = launch_correct_mass_fractions((int)nVars, (int)nPoints, state, c->stream);
+    return e ? fail(-10, "mass fraction launch failed") : 0;
+}
+
+int emx_heat_release_device(emx_context* c, size_t nVars, size_t nPoints, const double* state,
+                            double* qdot, int32_t* status) {
+    if (int rc = check_ctx(c)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
+    if (nPoints == 0) return 0;
+    if (!state || !qdot) return fail(-1, "state or qdot is NULL");
+    int e = launch_heat_release(c->dm, state, c->perturb, (int)nPoints, qdot, status, c->stream);
+    return e ? fail(-10, std::string("heat release launch: ") + hipGetErrorString((hipError_t)e)) : 0;
+}
+
 int emx_tridiagonal_integrate(emx_context* c, size_t nSys, size_t N, const double* a, const double* b,
                               const double* cc, const double* k, const double* h_in, double t0,
                               double t1, double* y) {

@@ -17,6 +17,21 @@ int launch_rates(const DevMech& m, const double* T, const double* Y, int nonorm,
                  const double* perturb, int n, double* wQ, double* wD, double* wnet, int* status,
                  hipStream_t stream);
 
+// device-resident split step (split_kernels.hip)
+int launch_split_constants(size_t n, int balanced, double* ddt_conv, const double* ddt_diff,
+                           const double* ddt_prod, const double* ddt_cross, double* sc_diff,
+                           double* sc_prod, hipStream_t st);
+int launch_fill(size_t n, double v, double* a, double* b, hipStream_t st);
+int launch_copy(size_t n, const double* src, double* dst, hipStream_t st);
+int launch_accumulate(size_t n, const double* state, const double* start, double* delta,
+                      hipStream_t st);
+int launch_time_derivatives(size_t n, double dt, const double* delta_diff, const double* delta_prod,
+                            const double* sc_diff, const double* sc_prod, double* ddt_diff,
+                            double* ddt_prod, hipStream_t st);
+int launch_correct_mass_fractions(int nVars, int nPoints, double* state, hipStream_t st);
+int launch_heat_release(const DevMech& m, const double* state, const double* perturb, int n,
+                        double* qdot, int* status, hipStream_t stream);
+
 }  // namespace emx
 
 namespace emx {

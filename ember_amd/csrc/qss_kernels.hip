@@ -125,6 +125,45 @@ __global__ void __launch_bounds__(256) rates_kernel(DevMech mg, const double* __
     if (lane == 0 && status) status[p] = ok ? 0 : -2;
 }
 
+// FlameSolver::calculateQdot (src/flameSolver.cpp:926-939) per point: setStateMass,
+// getEnthalpies, getReactionRates, qdot = -sum_k wdot_k h_k [W/m^3].  One wavefront per
+// point over Ember's point-contiguous state (U, T, Y).
+template <int NR, bool INTERP>
+__global__ void __launch_bounds__(256) heat_release_kernel(DevMech mg, const double* __restrict__ state,
+                                                           const double* __restrict__ perturb_g, int n,
+                                                           double* qdot, int* status) {
+    extern __shared__ double lds[];
+    const double* perturb;
+    const DevMech m = stage_mech(mg, perturb_g, lds, perturb);
+    __syncthreads();
+    const int lane = lane_id();
+    const int wave = threadIdx.x / 64;
+    const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
+    WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
+    init_wave_lds(L, lane, m.nRounds);
+    const int p = blockIdx.x * (blockDim.x / 64) + wave;
+    if (p >= n) return;
+    const int nv = m.nVars;
+    const bool sp = is_species_lane(lane, m);
+    const double y = lane < nv ? state[(size_t)p * nv + lane] : 0.0;
+    const double T = bcast(y, 1);
+    const double rW = sp ? m.rW[lane - 2] : 0.0;
+    RateRegs<NR> R;
+    R.T = -1.0;
+    GasState g = set_state(m, lane, y, rW, T, false, L);
+    double hRT, cpR;
+    const bool ok = update_T<NR, INTERP>(m, lane, g, R, L, hRT, cpR);
+    rates_of_progress<NR>(m, lane, g, R, perturb, L);
+    double wQ, wD;
+    species_rates(m, lane, L, wQ, wD);
+    const double hk = sp ? hRT * kGasConstant * T : 0.0;  // J/kmol
+    const double q = -wave_sum(sp ? (wQ - wD) * hk : 0.0);
+    if (lane == 0) {
+        qdot[p] = q;
+        if (status) status[p] = ok ? 0 : -2;
+    }
+}
+
 }  // namespace
 
 size_t qss_lds_bytes(const DevMech& m, int waves_per_block) {
@@ -168,6 +207,22 @@ int launch_rates(const DevMech& m, const double* T, const double* Y, int nonorm,
         else
             hipLaunchKernelGGL((rates_kernel<NR, false>), dim3(blocks), dim3(64 * wpb), lds, stream, m, T,
                                Y, nonorm, perturb, n, wQ, wD, wnet, status);
+    });
+    return (int)hipGetLastError();
+}
+
+int launch_heat_release(const DevMech& m, const double* state, const double* perturb, int n,
+                        double* qdot, int* status, hipStream_t stream) {
+    const int wpb = 4;
+    const size_t lds = qss_lds_bytes(m, wpb);
+    const int blocks = (n + wpb - 1) / wpb;
+    EMX_DISPATCH_NR(m.nRounds, {
+        if (m.interp)
+            hipLaunchKernelGGL((heat_release_kernel<NR, true>), dim3(blocks), dim3(64 * wpb), lds,
+                               stream, m, state, perturb, n, qdot, status);
+        else
+            hipLaunchKernelGGL((heat_release_kernel<NR, false>), dim3(blocks), dim3(64 * wpb), lds,
+                               stream, m, state, perturb, n, qdot, status);
     });
     return (int)hipGetLastError();
 }

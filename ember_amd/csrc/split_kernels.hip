@@ -1,0 +1,138 @@
+// Device-resident split step: the elementwise bookkeeping of SplitSolver
+// (src/splitSolver.cpp:27-121) and FlameSolver::correctMassFractions
+// (src/flameSolver.cpp:942-949), so that a whole Strang step runs on the GPU with the state
+// resident between the diffusion and production stages.
+//
+// All arrays are Ember's nVars x nPoints (point-contiguous) matrices, processed as flat
+// vectors of n = nVars * nPoints doubles.  The arithmetic is written operation by
+// operation (products and sums rounded separately, no contraction) so that it matches the
+// host-side SplitSolver bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "chem_device.h"
+#include "emx_common.h"
+#include "kernels.h"
+#include "wave.h"
+
+namespace emx {
+
+namespace {
+
+constexpr int kSplitThreads = 256;
+
+__device__ __forceinline__ size_t gid() {
+    return (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+
+// SplitSolver::step (src/splitSolver.cpp:38-44): balanced split constants from the
+// previous step's time derivatives; the convection stage is empty here (its delta stays
+// zero), so this step's ddtConv = 0 / dt - splitConstConv = -splitConstConv is written
+// directly.  Non-balanced: splitConstDiff = ddtCross, the other constants stay zero.
+__global__ void split_constants_kernel(size_t n, int balanced, double* ddt_conv,
+                                       const double* ddt_diff, const double* ddt_prod,
+                                       const double* ddt_cross, double* sc_diff, double* sc_prod) {
+    const size_t i = gid();
+    if (i >= n) return;
+    const double cross = ddt_cross ? ddt_cross[i] : 0.0;
+    if (balanced) {
+        const double conv = ddt_conv[i], diff = ddt_diff[i], prod = ddt_prod[i];
+        // 0.25 * (prod + conv + cross - 3 * diff), each operation rounded
+        sc_diff[i] = __dmul_rn(0.25, __dsub_rn(__dadd_rn(__dadd_rn(prod, conv), cross), __dmul_rn(3.0, diff)));
+        const double scc = __dmul_rn(0.25, __dsub_rn(__dadd_rn(__dadd_rn(prod, diff), cross), __dmul_rn(3.0, conv)));
+        sc_prod[i] = __dmul_rn(0.5, __dsub_rn(__dadd_rn(__dadd_rn(conv, diff), cross), prod));
+        ddt_conv[i] = 0.0 - scc;
+    } else {
+        sc_diff[i] = cross;
+        sc_prod[i] = 0.0;
+        ddt_conv[i] = 0.0;
+    }
+}
+
+__global__ void fill_kernel(size_t n, double v, double* a, double* b) {
+    const size_t i = gid();
+    if (i >= n) return;
+    a[i] = v;
+    if (b) b[i] = v;
+}
+
+__global__ void copy_kernel(size_t n, const double* src, double* dst) {
+    const size_t i = gid();
+    if (i < n) dst[i] = src[i];
+}
+
+// SplitSolver stage bookkeeping (src/splitSolver.cpp:77-121): delta += state - startState
+__global__ void accumulate_kernel(size_t n, const double* state, const double* start, double* delta) {
+    const size_t i = gid();
+    if (i < n) delta[i] = __dadd_rn(delta[i], __dsub_rn(state[i], start[i]));
+}
+
+// SplitSolver::calculateTimeDerivatives (src/splitSolver.cpp:27-32) for the diffusion and
+// production terms
+__global__ void time_derivatives_kernel(size_t n, double dt, const double* delta_diff,
+                                        const double* delta_prod, const double* sc_diff,
+                                        const double* sc_prod, double* ddt_diff, double* ddt_prod) {
+    const size_t i = gid();
+    if (i >= n) return;
+    ddt_diff[i] = __dsub_rn(__ddiv_rn(delta_diff[i], dt), sc_diff[i]);
+    ddt_prod[i] = __dsub_rn(__ddiv_rn(delta_prod[i], dt), sc_prod[i]);
+}
+
+// FlameSolver::correctMassFractions: setStateMass then getMassFractions, i.e. Cantera's
+// Phase::setMassFractions (negative mass fractions clipped to zero, then scaled by the
+// reciprocal of their sum).  One wavefront per point, lane = state variable.
+__global__ void __launch_bounds__(256) correct_mass_fractions_kernel(int nVars, int nPoints,
+                                                                     double* state) {
+    const int lane = lane_id();
+    const int p = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (p >= nPoints) return;
+    const bool sp = lane >= 2 && lane < nVars;
+    const size_t o = (size_t)p * nVars + lane;
+    const double y = sp ? fmax(state[o], 0.0) : 0.0;
+    const double norm = wave_sum(y);
+    if (sp) state[o] = y * (1.0 / norm);
+}
+
+inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + kSplitThreads - 1) / kSplitThreads)); }
+
+}  // namespace
+
+int launch_split_constants(size_t n, int balanced, double* ddt_conv, const double* ddt_diff,
+                           const double* ddt_prod, const double* ddt_cross, double* sc_diff,
+                           double* sc_prod, hipStream_t st) {
+    hipLaunchKernelGGL(split_constants_kernel, grid_for(n), dim3(kSplitThreads), 0, st, n, balanced,
+                       ddt_conv, ddt_diff, ddt_prod, ddt_cross, sc_diff, sc_prod);
+    return (int)hipGetLastError();
+}
+
+int launch_fill(size_t n, double v, double* a, double* b, hipStream_t st) {
+    hipLaunchKernelGGL(fill_kernel, grid_for(n), dim3(kSplitThreads), 0, st, n, v, a, b);
+    return (int)hipGetLastError();
+}
+
+int launch_copy(size_t n, const double* src, double* dst, hipStream_t st) {
+    hipLaunchKernelGGL(copy_kernel, grid_for(n), dim3(kSplitThreads), 0, st, n, src, dst);
+    return (int)hipGetLastError();
+}
+
+int launch_accumulate(size_t n, const double* state, const double* start, double* delta,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(accumulate_kernel, grid_for(n), dim3(kSplitThreads), 0, st, n, state, start,
+                       delta);
+    return (int)hipGetLastError();
+}
+
+int launch_time_derivatives(size_t n, double dt, const double* delta_diff, const double* delta_prod,
+                            const double* sc_diff, const double* sc_prod, double* ddt_diff,
+                            double* ddt_prod, hipStream_t st) {
+    hipLaunchKernelGGL(time_derivatives_kernel, grid_for(n), dim3(kSplitThreads), 0, st, n, dt,
+                       delta_diff, delta_prod, sc_diff, sc_prod, ddt_diff, ddt_prod);
+    return (int)hipGetLastError();
+}
+
+int launch_correct_mass_fractions(int nVars, int nPoints, double* state, hipStream_t st) {
+    hipLaunchKernelGGL(correct_mass_fractions_kernel, dim3((nPoints + 3) / 4), dim3(256), 0, st,
+                       nVars, nPoints, state);
+    return (int)hipGetLastError();
+}
+
+}  // namespace emx

@@ -196,6 +196,55 @@ int emx_diffusion_integrate_device(emx_context* ctx, size_t nVars, size_t nPoint
                                    const emx_grid* grid, const double* h_per_var, double t0,
                                    double t1, double* state);
 
+/* Device-resident split step: SplitSolver::step (src/splitSolver.cpp:34-58), i.e. the
+ * split constants (:38-44), integrateSplitTerms' stage sequence D1/4 C1/2 D1/4 P D1/4 C1/2
+ * D1/4 with its delta bookkeeping (:60-121) and calculateTimeDerivatives (:27-32), with
+ * FlameSolver's production and diffusion stages (flameSolver.cpp:702-734, 754-854) run
+ * on the GPU and the state resident in device memory throughout.  The convection stages
+ * are empty (convection is outside this library).  All matrices are nVars x nPoints,
+ * point-contiguous, device pointers owned by the caller:
+ *   state             in/out
+ *   ddt_conv/diff/prod  in: the previous step's time derivatives (zeros initially);
+ *                     out: this step's
+ *   ddt_cross         nullable (zeros): the cross-diffusion derivative
+ *   delta_diff/prod   out: the step's diffusion and production increments
+ *   work              scratch of 3 x nVars x nPoints doubles
+ * B, B_row, D, grid as emx_diffusion_integrate_device (device pointers, fixed over the
+ * step); qloss (nPoints, nullable) as emx_source_integrate_device; status (nPoints,
+ * device) receives the production stage's per-point status.  balanced: 1 = Ember's
+ * balanced splitting, 0 = Strang with splitConstDiff = ddt_cross.  The diffusion step
+ * sizes follow Ember's rule (one read of the grid minimum per step; the four diffusion
+ * stages share it since B and D are fixed) unless h_per_var (host, nVars) is given, in
+ * which case the call is fully asynchronous on the context stream. */
+typedef struct emx_split_arrays {
+    double* state;
+    double* ddt_conv;
+    double* ddt_diff;
+    double* ddt_prod;
+    const double* ddt_cross;
+    double* delta_diff;
+    double* delta_prod;
+    double* work;
+} emx_split_arrays;
+
+int emx_split_step_device(emx_context* ctx, size_t nVars, size_t nPoints,
+                          const emx_split_arrays* arrays, const double* B, size_t nB,
+                          const int32_t* B_row, const double* D, const emx_grid* grid,
+                          const double* qloss, const double* h_per_var, double t, double dt,
+                          int balanced, int32_t* status);
+
+/* FlameSolver::correctMassFractions (src/flameSolver.cpp:942-949): per point, negative
+ * mass fractions clipped to zero and the rest scaled to sum to one (Cantera
+ * Phase::setMassFractions).  state: nVars x nPoints device matrix, in place, async. */
+int emx_correct_mass_fractions_device(emx_context* ctx, size_t nVars, size_t nPoints,
+                                      double* state);
+
+/* FlameSolver::calculateQdot (src/flameSolver.cpp:926-939): qdot[j] = -sum_k wdot_k h_k
+ * [W/m^3] at each point's (T, Y) of the device state matrix; qdot (nPoints) device.
+ * status (nPoints, device, nullable): 0, or -2 if T is outside the interpolation table. */
+int emx_heat_release_device(emx_context* ctx, size_t nVars, size_t nPoints, const double* state,
+                            double* qdot, int32_t* status);
+
 /* TridiagonalIntegrator::integrateToTime for nSys constant systems y' = A y + k
  * (test/tridiagonalIntegrator.cpp ConstantTridiagonalODE).  a, b, c, k, y: nSys x N,
  * row-major (system s contiguous).  h: initial step per system.  Host pointers. */

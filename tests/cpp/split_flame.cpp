@@ -1,11 +1,16 @@
 // Split-step sequence on a flame-like profile (H2/O2 mechanism, ~150 points):
-// ember_amd::FlameStages (GPU production + diffusion through the C ABI) against the
-// same SplitSolver driven by the CPU oracle, T and Y compared after every step.
+// ember_amd::FlameStages (GPU production + diffusion through the host-pointer C ABI)
+// against the same SplitSolver driven by the CPU oracle, T and Y compared after every
+// step; and the device-resident split step (emx_split_step_device, state kept in device
+// memory) on the same inputs.
 //
 // usage: split_flame <mech_table> <input.bin> <output.bin> <nsteps> <dt>
 // input.bin: int64 nVars, nPoints, nB; doubles x[N], state[N*nV], B[nB*N], D[N*nV];
 //            int32 B_row[nV]
-// output.bin: doubles gpu[nsteps][N*nV], oracle[nsteps][N*nV]; int32 failures[2]
+// output.bin: doubles gpu[nsteps][N*nV], oracle[nsteps][N*nV], device[nsteps][N*nV];
+//             int32 failures[3]
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -110,10 +115,53 @@ int main(int argc, char** argv) {
             fflush(stdout);
         }
     }
+    // device-resident split step on the same inputs
+    std::vector<double> traj_d;
+    int dfail = 0;
+    {
+        const size_t n = N * nV;
+        double *d_state, *d_ddt, *d_delta, *d_work, *d_B, *d_D, *d_x;
+        int32_t *d_Brow, *d_status;
+        bool ok = hipMalloc(&d_state, n * 8) == hipSuccess && hipMalloc(&d_ddt, 3 * n * 8) == hipSuccess &&
+                  hipMalloc(&d_delta, 2 * n * 8) == hipSuccess && hipMalloc(&d_work, 3 * n * 8) == hipSuccess &&
+                  hipMalloc(&d_B, nB * N * 8) == hipSuccess && hipMalloc(&d_D, n * 8) == hipSuccess &&
+                  hipMalloc(&d_x, N * 8) == hipSuccess && hipMalloc(&d_Brow, nV * 4) == hipSuccess &&
+                  hipMalloc(&d_status, N * 4) == hipSuccess;
+        ok = ok && hipMemcpy(d_state, st.data(), n * 8, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemset(d_ddt, 0, 3 * n * 8) == hipSuccess &&
+             hipMemcpy(d_B, B.data(), nB * N * 8, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(d_D, D.data(), n * 8, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(d_x, x.data(), N * 8, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(d_Brow, B_row.data(), nV * 4, hipMemcpyHostToDevice) == hipSuccess;
+        if (!ok) return 6;
+        emx_split_arrays sa{d_state, d_ddt, d_ddt + n, d_ddt + 2 * n, nullptr, d_delta, d_delta + n, d_work};
+        emx_grid g{d_x, 0, EMX_BC_FIXED_VALUE, EMX_BC_FIXED_VALUE, 0.0, 0.0, 1};
+        std::vector<double> host(n);
+        std::vector<int32_t> hst(N);
+        t = 0.0;
+        for (int s = 0; s < nsteps; s++) {
+            ember_amd::check(emx_split_step_device(ctx.get(), nV, N, &sa, d_B, nB, d_Brow, d_D, &g,
+                                                   nullptr, nullptr, t, dt, 1, d_status));
+            if (hipMemcpy(host.data(), d_state, n * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(hst.data(), d_status, N * 4, hipMemcpyDeviceToHost) != hipSuccess)
+                return 7;
+            for (int32_t v : hst) dfail += (v != 0);
+            traj_d.insert(traj_d.end(), host.begin(), host.end());
+            t += dt;
+            if (s % 10 == 0) {
+                printf("[split_flame] device-resident step %d done\n", s);
+                fflush(stdout);
+            }
+        }
+        for (void* p : {(void*)d_state, (void*)d_ddt, (void*)d_delta, (void*)d_work, (void*)d_B,
+                        (void*)d_D, (void*)d_x, (void*)d_Brow, (void*)d_status})
+            (void)hipFree(p);
+    }
     fwrite(traj_g.data(), sizeof(double), traj_g.size(), o);
     fwrite(traj_c.data(), sizeof(double), traj_c.size(), o);
-    int32_t fails[2] = {gfail, cpu.failures};
-    fwrite(fails, sizeof(int32_t), 2, o);
+    fwrite(traj_d.data(), sizeof(double), traj_d.size(), o);
+    int32_t fails[3] = {gfail, cpu.failures, dfail};
+    fwrite(fails, sizeof(int32_t), 3, o);
     fclose(o);
     orc_mech_free(mech);
     printf("[split_flame] ok\n");

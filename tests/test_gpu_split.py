@@ -1,7 +1,9 @@
 """Split-step sequence (diffusion 1/4 x 4 + production, balanced splitting) on a
 150-point H2/air flame-like profile for 100 steps of dt = 1e-5 s: the C++ SplitSolver
 mirror over the GPU library against the same driver over the CPU oracle, T and Y
-compared after every step (rtol 1e-6).  Convection is out of scope (no-op stage)."""
+compared after every step (rtol 1e-6); and the device-resident split step
+(emx_split_step_device) against the host-driven one, bit for bit.  Convection is out of
+scope (no-op stage)."""
 import os
 import subprocess
 
@@ -32,10 +34,13 @@ def test_split_flame_100_steps(gpu, tmp_path):
     r = subprocess.run([EXE, mechanism.table_path("h2o2"), str(inp), str(out), str(nsteps), str(dt)],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    data = np.fromfile(out, dtype=np.float64, count=2 * nsteps * N * nV).reshape(2, nsteps, N, nV)
-    fails = np.fromfile(out, dtype=np.int32, offset=2 * nsteps * N * nV * 8)
-    assert fails.tolist() == [0, 0]
-    g, c = data
+    data = np.fromfile(out, dtype=np.float64, count=3 * nsteps * N * nV).reshape(3, nsteps, N, nV)
+    fails = np.fromfile(out, dtype=np.int32, offset=3 * nsteps * N * nV * 8)
+    assert fails.tolist() == [0, 0, 0]
+    g, c, d = data
+    # the device-resident step runs the same kernels with the same arithmetic
+    print(f"[split] device-resident vs host-driven max |diff| {np.abs(d - g).max():.3e}")
+    np.testing.assert_array_equal(d, g)
     dT = np.abs(g[:, :, 1] - c[:, :, 1]) / c[:, :, 1]
     dY = np.abs(g[:, :, 2:] - c[:, :, 2:]) - 1e-6 * np.abs(c[:, :, 2:])
     print(f"[split] max rel dT {dT.max():.2e}, max Y excess {dY.max():.2e}")
@@ -43,3 +48,43 @@ def test_split_flame_100_steps(gpu, tmp_path):
     assert dY.max() <= 1e-12
     # the flame evolved: chemistry and diffusion both acted
     assert np.abs(g[-1, :, 1] - case.state[:, 1]).max() > 1.0
+
+
+def test_correct_mass_fractions_device(gpu):
+    """FlameSolver::correctMassFractions: clip negative Y to zero, scale to unit sum."""
+    import torch
+    from ember_amd import capi
+    print("[split] correct mass fractions", flush=True)
+    ctx = capi.Context("gri30")
+    st = W.tanh_progress_states(200, seed=7, name="gri30")
+    rng = np.random.default_rng(7)
+    st[:, 2:] += rng.normal(0.0, 1e-4, st[:, 2:].shape)  # some negative, sums off one
+    d = torch.from_numpy(st.copy()).to(gpu)
+    ctx.correct_mass_fractions_device(d)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    y = np.maximum(st[:, 2:], 0.0)
+    ref = y * (1.0 / y.sum(axis=1, keepdims=True))
+    np.testing.assert_allclose(got[:, 2:], ref, rtol=1e-15, atol=0)
+    np.testing.assert_array_equal(got[:, :2], st[:, :2])
+    assert (got[:, 2:] >= 0).all()
+
+
+@pytest.mark.parametrize("name", ["h2o2", "gri30"])
+def test_heat_release_device(gpu, oracle_mod, name):
+    """FlameSolver::calculateQdot against the oracle's rates and enthalpies."""
+    import torch
+    from ember_amd import capi
+    print(f"[split] heat release {name}", flush=True)
+    ctx = capi.Context(name)
+    st = W.tanh_progress_states(96, seed=3, name=name)
+    d = torch.from_numpy(st).to(gpu)
+    q = torch.zeros(st.shape[0], dtype=torch.float64, device=gpu)
+    status = torch.zeros(st.shape[0], dtype=torch.int32, device=gpu)
+    ctx.heat_release_device(d, q, status)
+    torch.cuda.synchronize()
+    m = oracle_mod.Mech(mechanism.table_path(name))
+    r = m.rates(st[:, 1], st[:, 2:])
+    ref = -(r["wdot"] * r["hk"]).sum(axis=1)
+    assert (status.cpu().numpy() == 0).all()
+    np.testing.assert_allclose(q.cpu().numpy(), ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
