@@ -193,7 +193,7 @@ __device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, d
                         u = lpr + cc;
                         f1 = u / (nn - 0.14 * u);
                         const double lf = work / (1.0 + f1 * f1);
-                        F = emx_exp(kLn10 * lf);
+                        F = emx_exp(kLn10 * lf, m.exp2t);
                     }
                     kf = pr0 * (F / (1.0 + pr0)) * khigh;
                 }
@@ -234,13 +234,13 @@ __device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, d
                         workb += -0.67 * ub - 1.27 * nnb;
                         if (pr0 > kSmallNumber) pr0b += ub / (pr0 * ln10);
                         const double a3 = m.troeA[s], rT3 = m.rT3[s], rT1 = m.rT1[s], T2 = m.T2[s];
-                        const double e3 = emx_exp(-T * rT3), e1 = emx_exp(-T * rT1);
+                        const double e3 = emx_exp(-T * rT3, m.exp2t), e1 = emx_exp(-T * rT1, m.exp2t);
                         double Fcent = (1.0 - a3) * e3 + a3 * e1;
                         double dF = 0.0;
                         if (rT3 < kBigNumber) dF += -(1.0 - a3) * rT3 * e3;
                         if (rT1 < kBigNumber) dF += -a3 * rT1 * e1;
                         if (T2 != 0.0) {
-                            const double e2 = emx_exp(-T2 / T);
+                            const double e2 = emx_exp(-T2 / T, m.exp2t);
                             Fcent += e2;
                             dF += T2 * recipT * recipT * e2;
                         }

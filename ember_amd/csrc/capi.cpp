@@ -9,6 +9,7 @@
 
 #include "../../include/ember_amd.h"
 #include "emx_common.h"
+#include "emx_math.h"
 #include "kernels.h"
 #include "mech_host.h"
 
@@ -144,6 +145,7 @@ int upload_mech(emx_context* c) {
     pk.add(L.rW, &d.rW);
     pk.add(m.Tmid, &d.Tmid);
     pk.add(m.nasa, &d.nasa);
+    pk.add(std::vector<double>(kExp2Tab, kExp2Tab + 16), &d.exp2t);
     pk.add(L.A, &d.A);
     pk.add(L.b, &d.b);
     pk.add(L.EaR, &d.EaR);

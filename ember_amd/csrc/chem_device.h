@@ -74,7 +74,7 @@ __device__ inline void init_wave_lds(const WaveLds& L, int lane, int nRounds) {
 // ---------------------------------------------------------------------------
 __host__ __device__ inline int mech_lds_doubles(const DevMech& m) {
     const int S = m.nRounds * 64;
-    int n = S + 3 * m.K + 14 * m.K + m.nGrpEntries + (m.nGroups > 0 ? m.nGroups : 1);
+    int n = S + 3 * m.K + 14 * m.K + 16 + m.nGrpEntries + (m.nGroups > 0 ? m.nGroups : 1);
     int ints = 3 * S + m.nEntries + 2 * m.K + (m.nGroups + 1) + m.nGrpEntries;
     return n + (ints + 1) / 2;
 }
@@ -98,6 +98,7 @@ __device__ inline DevMech stage_mech(const DevMech& m, const double* perturb, do
     v.rW = cpd(m.rW, m.K);
     v.Tmid = cpd(m.Tmid, m.K);
     v.nasa = cpd(m.nasa, 14 * m.K);
+    v.exp2t = cpd(m.exp2t, 16);
     v.grp_eff = cpd(m.grp_eff, m.nGrpEntries);
     v.grp_def = cpd(m.grp_def, m.nGroups > 0 ? m.nGroups : 1);
     int32_t* q = reinterpret_cast<int32_t*>(p);
@@ -179,8 +180,8 @@ __device__ inline void nasa(const DevMech& m, int k, double T, double logT, doub
 }
 
 __device__ __forceinline__ double arrhenius(double A, double b, double EaR, double logT,
-                                            double recipT) {
-    return A * emx_exp(b * logT - EaR * recipT);
+                                            double recipT, const double* e2t) {
+    return A * emx_exp(b * logT - EaR * recipT, e2t);
 }
 
 // Products over the (padded) three stoichiometry indices: the padding cells (C = 1)
@@ -234,7 +235,7 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
         // mu_k / RT - ln c0 = h_k/RT - s_k/R + ln T + ln(R / Pref)
         if (sp) {
             const double x = (hRT - sR) + (logT + m.lnRPref);
-            const double e = emx_exp(x);
+            const double e = emx_exp(x, m.exp2t);
             L.ge[lane - 2] = e;
             L.gi[lane - 2] = 1.0 / e;
         }
@@ -248,7 +249,7 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
                 double rkc = 0.0, kf = 0.0;
                 if (kind != kNone) {
                     if (info_rev(w)) rkc = fmin(inv_kc(m.reac[s], m.prod[s], L), kBigNumber);
-                    if (kind <= kThreeBody) kf = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT);
+                    if (kind <= kThreeBody) kf = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT, m.exp2t);
                 }
                 R.rkc[j] = rkc;
                 R.kf[j] = kf;
@@ -263,11 +264,11 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
             int kind = info_kind(m.info[s]);
             double klow = 0.0, khigh = 0.0, work = 0.0;
             if (kind == kLindemann || kind == kTroe) {
-                klow = arrhenius(m.A0[s], m.b0[s], m.EaR0[s], logT, recipT);
-                khigh = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT);
+                klow = arrhenius(m.A0[s], m.b0[s], m.EaR0[s], logT, recipT, m.exp2t);
+                khigh = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT, m.exp2t);
                 if (kind == kTroe) {
-                    double Fcent = (1.0 - m.troeA[s]) * emx_exp(-T * m.rT3[s]) + m.troeA[s] * emx_exp(-T * m.rT1[s]);
-                    if (m.T2[s] != 0.0) Fcent += emx_exp(-m.T2[s] / T);
+                    double Fcent = (1.0 - m.troeA[s]) * emx_exp(-T * m.rT3[s], m.exp2t) + m.troeA[s] * emx_exp(-T * m.rT1[s], m.exp2t);
+                    if (m.T2[s] != 0.0) Fcent += emx_exp(-m.T2[s] / T, m.exp2t);
                     work = emx_log(fmax(Fcent, kSmallNumber)) * kInvLn10;
                 }
             }
@@ -320,7 +321,7 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
                         double nn = 0.75 - 1.27 * work;
                         double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
                         double lf = work / (1.0 + f1 * f1);
-                        F = emx_exp(kLn10 * lf);
+                        F = emx_exp(kLn10 * lf, m.exp2t);
                     }
                     pr *= F / (1.0 + pr);
                     kf = pr * khigh;

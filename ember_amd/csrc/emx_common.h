@@ -68,6 +68,7 @@ struct DevMech {
     const double* rW;
     const double* Tmid;
     const double* nasa;    // [K][14]: low a0..a6, high a0..a6
+    const double* exp2t;   // [16] 2^(j/16) for emx_exp
     // reactions in device order [nRounds * 64]
     const double* A;
     const double* b;
