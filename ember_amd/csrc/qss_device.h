@@ -150,7 +150,7 @@ __device__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
                     double scr1 = scr2 * Z.d;
                     c = fmax(scr1, -fabs(absq - Z.d) * scr2);
                 }
-                double scrtch = fmax(wave_max(c), 1.0e-25);
+                double scrtch = fmax(wave_max0(c), 1.0e-25);
                 const double sqreps = 0.5;
                 dt = fmin(sqreps / scrtch, tf);
                 dt = fmin(dt, o.qss_dtmax);
@@ -191,7 +191,7 @@ __device__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
                     c = .5 * (scr1 + fmin(fabs(Z.q - Z.d) / (Z.q + Z.d + 1e-30), scr1));
                 }
             }
-            eps = fmax(eps, wave_max(c));
+            eps = fmax(eps, wave_max0(c));
             if (stab_on) {
                 Z.ym2 = Z.ym1;
                 Z.ym1 = Z.y;
@@ -204,7 +204,7 @@ __device__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
                 double sc = kNegInf;
                 if (active && fabs(Z.y) > o.qss_abstol)
                     sc = fabs(Z.y - Z.ym1) / (fabs(Z.ym1 - Z.ym2) + 1e-20 * Z.y);
-                stab = fmax(0.01, wave_max(sc));
+                stab = fmax(0.01, wave_max0(sc));
             }
             const bool ok = eps <= o.qss_epsmax && stab <= 1.0;
             if (ok) {

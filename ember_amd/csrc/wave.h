@@ -21,6 +21,17 @@ __device__ __forceinline__ double dpp_move(double v, double identity) {
     return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+// The same moves for reductions whose identity is +0.0: lanes without a source read 0
+// (bound_ctrl), and the destination's previous contents are not needed, so no copy of an
+// identity value precedes each move.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move0(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
 // value of `v` held by lane `src` (src uniform across the wave): v_readlane
 __device__ __forceinline__ double bcast(double v, int src) {
     const uint64_t b = __builtin_bit_cast(uint64_t, v);
@@ -44,6 +55,19 @@ __device__ __forceinline__ double wave_reduce(double v, double identity, Op op) 
     return bcast(v, 63);
 }
 
+// wave_reduce with identity +0.0 (sums; maxima of values whose result is only used
+// through fmax(result, positive floor))
+template <class Op>
+__device__ __forceinline__ double wave_reduce0(double v, Op op) {
+    v = op(v, dpp_move0<0xB1>(v));
+    v = op(v, dpp_move0<0x4E>(v));
+    v = op(v, dpp_move0<0x114>(v));
+    v = op(v, dpp_move0<0x118>(v));
+    v = op(v, dpp_move0<0x142>(v));
+    v = op(v, dpp_move0<0x143>(v));
+    return bcast(v, 63);
+}
+
 struct AddOp {
     __device__ __forceinline__ double operator()(double a, double b) const { return a + b; }
 };
@@ -54,7 +78,9 @@ struct MinOp {
     __device__ __forceinline__ double operator()(double a, double b) const { return fmin(a, b); }
 };
 
-__device__ __forceinline__ double wave_sum(double v) { return wave_reduce(v, 0.0, AddOp()); }
+__device__ __forceinline__ double wave_sum(double v) { return wave_reduce0(v, AddOp()); }
+// max(0, max over lanes): equal to fmax(floor, wave_max(v)) for any floor > 0
+__device__ __forceinline__ double wave_max0(double v) { return wave_reduce0(v, MaxOp()); }
 __device__ __forceinline__ double wave_max(double v) { return wave_reduce(v, -__builtin_huge_val(), MaxOp()); }
 __device__ __forceinline__ double wave_min(double v) { return wave_reduce(v, __builtin_huge_val(), MinOp()); }
 
