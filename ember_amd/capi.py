@@ -62,6 +62,7 @@ SIGNATURES = [
     ("emx_num_reactions", C.c_int, [_vp]),
     ("emx_set_options", C.c_int, [_vp, C.POINTER(Options)]),
     ("emx_set_rate_multipliers", C.c_int, [_vp, _dp]),
+    ("emx_set_rate_multiplier_function", C.c_int, [_vp, C.c_int, _dp]),
     ("emx_set_stream", C.c_int, [_vp, _vp]),
     ("emx_set_launch", C.c_int, [_vp, C.c_int, C.c_int]),
     ("emx_set_kernel_variant", C.c_int, [_vp, C.c_int]),
@@ -86,7 +87,7 @@ SIGNATURES = [
      [_vp, _sz, _sz, C.POINTER(SplitArrays), _vp, _sz, _vp, _vp, C.POINTER(Grid), _vp, _dp,
       C.c_double, C.c_double, C.c_int, _vp]),
     ("emx_correct_mass_fractions_device", C.c_int, [_vp, _sz, _sz, _vp]),
-    ("emx_heat_release_device", C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp]),
+    ("emx_heat_release_device", C.c_int, [_vp, _sz, _sz, _vp, C.c_double, _vp, _vp]),
 ]
 
 _lib = None
@@ -174,6 +175,16 @@ class Context:
     def set_rate_multipliers(self, m):
         m = None if m is None else np.ascontiguousarray(m, dtype=np.float64)
         _check(lib().emx_set_rate_multipliers(self.h, _p(m)))
+
+    def set_rate_multiplier_function(self, data):
+        """Ember's rate multiplier function: data = [T0, T1, c_0, ...] as
+        ChebyshevFunction::setCoefficients takes them (src/scalarFunction.cpp:33-51);
+        None removes it."""
+        if data is None:
+            _check(lib().emx_set_rate_multiplier_function(self.h, 0, None))
+            return
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        _check(lib().emx_set_rate_multiplier_function(self.h, len(d), _p(d)))
 
     def set_stream(self, stream_handle: int):
         _check(lib().emx_set_stream(self.h, C.c_void_p(stream_handle)))
@@ -293,9 +304,10 @@ class Context:
         N, nV = state.shape
         _check(lib().emx_correct_mass_fractions_device(self.h, nV, N, _tptr(state)))
 
-    def heat_release_device(self, state, qdot, status=None):
+    def heat_release_device(self, state, qdot, status=None, t_now=0.0):
         N, nV = state.shape
-        _check(lib().emx_heat_release_device(self.h, nV, N, _tptr(state), _tptr(qdot), _tptr(status)))
+        _check(lib().emx_heat_release_device(self.h, nV, N, _tptr(state), float(t_now), _tptr(qdot),
+                                             _tptr(status)))
 
     def tridiagonal_integrate(self, a, b, c, k, y, h, t0, t1):
         """Constant systems, (nSys, N) arrays.  Returns y after integrateToTime(t1)."""

@@ -434,7 +434,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_forward_kernel(
     const DevMech m = stage_mech(mg, perturb_g, lds, perturb);
     __syncthreads();
     const int lane = lane_id();
-    const int wave = threadIdx.x / 64;
+    const int wave = wave_in_block();
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
     WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
     init_wave_lds(L, lane, m.nRounds);
@@ -490,7 +490,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_backward_kernel(
     const DevMech m = stage_mech(mg, perturb_g, lds, perturb);
     __syncthreads();
     const int lane = lane_id();
-    const int wave = threadIdx.x / 64;
+    const int wave = wave_in_block();
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds) + adj_lds_doubles(m.nRounds);
     double* wbase = lds + mech_lds_doubles(m) + wave * per_wave;
     WaveLds L = carve_lds(wbase, m.nRounds, m.nChunkRounds);

@@ -37,13 +37,13 @@ struct BdfSystem {
 // SourceSystemCVODE::f for this lane's variable.  Returns false if rates failed.
 template <int NR, bool INTERP>
 __device__ inline bool bdf_f(const DevMech& m, int lane, double yv, BdfSystem& S, RateRegs<NR>& R,
-                             const double* perturb, const WaveLds& L, double& ydot) {
+                             const double* perturb, double fa, const WaveLds& L, double& ydot) {
     S.nfe++;
     const double T = bcast(yv, 1);
     GasState g = set_state(m, lane, yv, S.rW, T, true, L);
     double hRT, cpR;
     bool ok = update_T<NR, INTERP>(m, lane, g, R, L, hRT, cpR);
-    rates_of_progress<NR>(m, lane, g, R, perturb, L);
+    rates_of_progress<NR>(m, lane, g, R, perturb, fa, L);
     double wQ, wD;
     species_rates(m, lane, L, wQ, wD);
     const bool sp = is_species_lane(lane, m);
@@ -164,7 +164,8 @@ __device__ double lu_solve(const double* M, int n, int lane, int step_of, double
 // SourceSystemCVODE::fdJacobian: column i written to J[i*64 + lane]
 template <int NR, bool INTERP>
 __device__ bool fd_jacobian(const DevMech& m, int lane, int n, double yv, double ydot, BdfSystem& S,
-                            RateRegs<NR>& R, const double* perturb, const WaveLds& L, double* J) {
+                            RateRegs<NR>& R, const double* perturb, double fa, const WaveLds& L,
+                            double* J) {
     const double eps = 1.4901161193847656e-08;  // sqrt(DBL_EPSILON)
     const double atl = 2.220446049250313e-16;   // DBL_EPSILON
     bool ok = true;
@@ -174,7 +175,7 @@ __device__ bool fd_jacobian(const DevMech& m, int lane, int n, double yv, double
         double yp = yv;
         if (lane == i) yp += dy;
         double yd2;
-        ok = bdf_f<NR, INTERP>(m, lane, yp, S, R, perturb, L, yd2) && ok;
+        ok = bdf_f<NR, INTERP>(m, lane, yp, S, R, perturb, fa, L, yd2) && ok;
         J[i * 64 + lane] = (yd2 - ydot) / dy;
     }
     return ok;
@@ -192,14 +193,15 @@ __device__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
     nsteps = 0;
     bool ok = true;
     double f0;
-    ok = bdf_f<NR, INTERP>(m, lane, y, S, R, perturb, L, f0) && ok;
+    // the right-hand sides see t_start + t (CVODE's t0 = tInitial, src/sourceSystem.cpp:298)
+    ok = bdf_f<NR, INTERP>(m, lane, y, S, R, perturb, fn_mult(o, 0.0), L, f0) && ok;
     double qprev = S.qdot, qint = 0.0;
     double scale = S.atol + fabs(y) * rtol;
     const double d0 = rms(y, scale, active, n), d1 = rms(f0, scale, active, n);
     double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
     h0 = fmin(h0, tf);
     double f1;
-    ok = bdf_f<NR, INTERP>(m, lane, y + h0 * f0, S, R, perturb, L, f1) && ok;
+    ok = bdf_f<NR, INTERP>(m, lane, y + h0 * f0, S, R, perturb, fn_mult(o, h0), L, f1) && ok;
     const double d2 = rms(f1 - f0, scale, active, n) / h0;
     const double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? fmax(1e-6, h0 * 1e-3)
                                                    : pow(0.01 / fmax(d1, d2), 1.0 / 2.0);
@@ -213,7 +215,7 @@ __device__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
     D[1] = f0 * h_abs;
     bool lu_valid = false;
     int step_of = -1;
-    ok = fd_jacobian<NR, INTERP>(m, lane, n, y, f0, S, R, perturb, L, J) && ok;
+    ok = fd_jacobian<NR, INTERP>(m, lane, n, y, f0, S, R, perturb, fn_mult(o, 0.0), L, J) && ok;
     if (!ok) return -2;
     double t = 0.0;
     double error_norm = 0.0, safety = 0.0;
@@ -261,7 +263,7 @@ __device__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
                 int k;
                 for (k = 0; k < kNewtonMaxIter; k++) {
                     double fy;
-                    ok = bdf_f<NR, INTERP>(m, lane, ynew, S, R, perturb, L, fy) && ok;
+                    ok = bdf_f<NR, INTERP>(m, lane, ynew, S, R, perturb, fn_mult(o, t_new), L, fy) && ok;
                     const int bad = wave_or(active && !isfinite(fy) ? 1 : 0);
                     if (bad) break;
                     double dy = lu_solve(M, n, lane, step_of, c * fy - psi - dvec);
@@ -281,8 +283,9 @@ __device__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
                 if (!converged) {
                     if (current_jac) break;
                     double fp;
-                    ok = bdf_f<NR, INTERP>(m, lane, ypred, S, R, perturb, L, fp) && ok;
-                    ok = fd_jacobian<NR, INTERP>(m, lane, n, ypred, fp, S, R, perturb, L, J) && ok;
+                    const double fa = fn_mult(o, t_new);
+                    ok = bdf_f<NR, INTERP>(m, lane, ypred, S, R, perturb, fa, L, fp) && ok;
+                    ok = fd_jacobian<NR, INTERP>(m, lane, n, ypred, fp, S, R, perturb, fa, L, J) && ok;
                     lu_valid = false;
                     current_jac = true;
                 }
@@ -315,7 +318,7 @@ __device__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
         y = ynew;
         if (qint_out) {
             double ftmp;
-            ok = bdf_f<NR, INTERP>(m, lane, y, S, R, perturb, L, ftmp) && ok;
+            ok = bdf_f<NR, INTERP>(m, lane, y, S, R, perturb, fn_mult(o, t), L, ftmp) && ok;
             S.nfe--;
             qint += 0.5 * hstep * (qprev + S.qdot);
             qprev = S.qdot;

@@ -84,6 +84,10 @@ struct emx_context {
     int adj_cap = 64;       // checkpoints per point
     size_t adj_n = 0;       // points of the last adjoint forward stage (0: none)
     double adj_dt = 0.0;
+    // rate multiplier function: [T0, T1, c_0 .. c_{n-1}] on the device, n = ratefn_n (0: none);
+    // while set, the kernels take unit per-slot multipliers (`ones`) times a(t)
+    DevBuf ratefn, ones;
+    int ratefn_n = 0;
 };
 
 namespace {
@@ -104,6 +108,9 @@ DevOpts to_dev(const emx_options& o) {
     d.atol_species = o.integrator_species_abs_tol;
     d.min_step = o.integrator_min_timestep;
     d.max_steps = o.integrator_max_steps;
+    d.ratefn = nullptr;
+    d.ratefn_n = 0;
+    d.t_start = 0.0;
     return d;
 }
 
@@ -192,8 +199,23 @@ int check_ctx(emx_context* c) {
     return 0;
 }
 
-int grid_blocks(emx_context* c, int nPoints, bool bdf) {
-    const int wpb = c->waves_per_block;
+// per-slot multipliers for the production kernels: ones while the function is set
+const double* stage_perturb(const emx_context* c) {
+    return c->ratefn_n > 0 ? c->ones.as<double>() : c->perturb;
+}
+
+// kernel options of a stage starting at absolute time t_start
+DevOpts stage_opts(const emx_context* c, double t_start) {
+    DevOpts d = c->dopts;
+    if (c->ratefn_n > 0) {
+        d.ratefn = c->ratefn.as<double>();
+        d.ratefn_n = c->ratefn_n;
+    }
+    d.t_start = t_start;
+    return d;
+}
+
+int grid_blocks(emx_context* c, int nPoints, bool bdf, int wpb) {
     size_t lds_per_block = bdf ? bdf_lds_bytes(c->dm, wpb) : qss_lds_bytes(c->dm, wpb);
     int per_cu = c->blocks_per_cu;
     if (per_cu <= 0) {
@@ -274,7 +296,8 @@ void emx_destroy(emx_context* c) {
     for (DevBuf* b : {&c->b_in, &c->b_out, &c->b_sc, &c->b_ql, &c->b_status, &c->b_s1, &c->b_s2,
                       &c->d_B, &c->d_Brow, &c->d_D, &c->d_sc, &c->d_x, &c->d_y, &c->d_h, &c->d_ns,
                       &c->d_work, &c->d_hmin, &c->t_a, &c->t_b, &c->t_c, &c->t_k, &c->jac, &c->plain_ws,
-                      &c->cost, &c->order, &c->ck_y, &c->ck_s, &c->nckpt, &c->record})
+                      &c->cost, &c->order, &c->ck_y, &c->ck_s, &c->nckpt, &c->record, &c->ratefn,
+                      &c->ones})
         b->release();
     delete c;
 }
@@ -298,6 +321,36 @@ int emx_set_rate_multipliers(emx_context* c, const double* m) {
     if (m)
         for (int i = 0; i < c->hm.I; i++) p[c->lay.slot_of_rxn[i]] = m[i];
     HIP_TRY(hipMemcpy(c->perturb, p.data(), p.size() * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int emx_set_rate_multiplier_function(emx_context* c, int n, const double* data) {
+    if (int rc = check_ctx(c)) return rc;
+    if (n == 0 || !data) {
+        c->ratefn_n = 0;
+        return 0;
+    }
+    if (n < 2) return fail(-1, "the rate multiplier function needs n >= 2 (T0, T1, coefficients)");
+    if (n - 2 > kMaxRateFnCoeffs) return fail(-1, "too many Chebyshev coefficients");
+    if (!(data[1] != data[0])) return fail(-1, "the rate multiplier function needs T1 != T0");
+    // ChebyshevFunction::setCoefficients: at least two coefficients, zero-filled
+    std::vector<double> f = {data[0], data[1]};
+    if (n >= 4)
+        f.insert(f.end(), data + 2, data + n);
+    else if (n == 3)
+        f.insert(f.end(), {data[2], 0.0});
+    else
+        f.insert(f.end(), {0.0, 0.0});
+    HIP_TRY(c->ratefn.ensure((2 + kMaxRateFnCoeffs) * sizeof(double)));
+    if (!c->ones.p) {
+        const std::vector<double> one(c->lay.nRounds * 64, 1.0);
+        HIP_TRY(c->ones.ensure(one.size() * sizeof(double)));
+        HIP_TRY(hipMemcpy(c->ones.p, one.data(), one.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    // stages already queued on the context stream read the previous coefficients
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(c->ratefn.p, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->ratefn_n = (int)f.size() - 2;
     return 0;
 }
 
@@ -335,15 +388,15 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
                                 const double* split_const, const double* qloss, double t_stage_start,
                                 double dt, double* state_out, int32_t* status, int32_t* stat1,
                                 int32_t* stat2, double* qint) {
-    (void)t_stage_start;
     if (int rc = check_ctx(c)) return rc;
     if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
     if (nPoints == 0) return 0;
     if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
     if (!state_in || !state_out) return fail(-1, "state pointers are NULL");
     if (c->variant == EMX_VARIANT_PLAIN) {
-        if (c->opts.chemistry_integrator != EMX_QSS || c->opts.qss_stability_check)
-            return fail(-1, "the plain baseline kernel implements QSS without stability check only");
+        if (c->opts.chemistry_integrator != EMX_QSS || c->opts.qss_stability_check || c->ratefn_n)
+            return fail(-1, "the plain baseline kernel implements QSS without stability check "
+                            "or rate multiplier function only");
         HIP_TRY(c->plain_ws.ensure(plain_workspace_doubles(c->dm, (int)nPoints) * sizeof(double)));
         int rc = launch_qss_plain(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
                                   (int)nPoints, state_out, status, stat1, stat2,
@@ -365,9 +418,9 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
             return fail(-1, "the per-point wavefront variant implements QSS only");
         if (bdf_lds_bytes(c->dm, c->waves_per_block) > 160 * 1024)
             return fail(-1, "BDF workgroup needs more than 160 KB of LDS; use fewer waves per block");
-        const int blocks = grid_blocks(c, (int)nPoints, true);
+        const int blocks = grid_blocks(c, (int)nPoints, true, c->waves_per_block);
         HIP_TRY(c->jac.ensure((size_t)blocks * c->waves_per_block * 64 * 64 * sizeof(double)));
-        int rc = launch_bdf_stage(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
+        int rc = launch_bdf_stage(c->dm, stage_opts(c, t_stage_start), state_in, split_const, qloss, stage_perturb(c), dt,
                                   (int)nPoints, order, c->queue, state_out, status, stat1, stat2,
                                   c->cost.as<int>(), qint, blocks, c->waves_per_block,
                                   c->jac.as<double>(), c->stream);
@@ -376,12 +429,15 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
         return 0;
     }
     const bool per_point = c->variant == EMX_VARIANT_WAVE_PER_POINT;
-    const int qblocks = per_point ? (int)((nPoints + c->waves_per_block - 1) / c->waves_per_block)
-                                  : grid_blocks(c, (int)nPoints, false);
-    int rc = launch_qss_stage(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
-                              (int)nPoints, per_point ? nullptr : order, per_point ? nullptr : c->queue,
+    // the rate multiplier function's kernels run at up to 8 waves per workgroup
+    const int wpb = c->ratefn_n > 0 ? std::min(c->waves_per_block, 8) : c->waves_per_block;
+    const int qblocks = per_point ? (int)((nPoints + wpb - 1) / wpb)
+                                  : grid_blocks(c, (int)nPoints, false, wpb);
+    int rc = launch_qss_stage(c->dm, stage_opts(c, t_stage_start), state_in, split_const, qloss,
+                              stage_perturb(c), dt, (int)nPoints, per_point ? nullptr : order,
+                              per_point ? nullptr : c->queue,
                               state_out, status, stat1, stat2, c->cost.as<int>(), qint, qblocks,
-                              c->waves_per_block, c->stream);
+                              wpb, c->stream);
     if (rc) return fail(-10, std::string("qss launch: ") + hipGetErrorString((hipError_t)rc));
     c->cost_n = nPoints;
     return 0;
@@ -398,6 +454,9 @@ static int adjoint_check(emx_context* c, size_t nPoints, size_t nVars) {
         c->opts.qss_stability_check)
         return fail(-1, "the adjoint supports the QSS integrator with qss_iteration_count = 1 and "
                         "no stability check");
+    if (c->ratefn_n)
+        return fail(-1, "the adjoint's ln-multiplier parameters need per-reaction multipliers; "
+                        "remove the rate multiplier function");
     return 0;
 }
 
@@ -748,13 +807,14 @@ int emx_correct_mass_fractions_device(emx_context* c, size_t nVars, size_t nPoin
 }
 
 int emx_heat_release_device(emx_context* c, size_t nVars, size_t nPoints, const double* state,
-                            double* qdot, int32_t* status) {
+                            double t_now, double* qdot, int32_t* status) {
     if (int rc = check_ctx(c)) return rc;
     if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
     if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
     if (nPoints == 0) return 0;
     if (!state || !qdot) return fail(-1, "state or qdot is NULL");
-    int e = launch_heat_release(c->dm, state, c->perturb, (int)nPoints, qdot, status, c->stream);
+    int e = launch_heat_release(c->dm, stage_opts(c, t_now), state, stage_perturb(c), (int)nPoints, qdot,
+                                status, c->stream);
     return e ? fail(-10, std::string("heat release launch: ") + hipGetErrorString((hipError_t)e)) : 0;
 }
 

@@ -119,6 +119,31 @@ __device__ inline DevMech stage_mech(const DevMech& m, const double* perturb, do
     return v;
 }
 
+// ChebyshevFunction::a (src/scalarFunction.cpp:53-72): t mapped onto [-1, 1] over
+// f = [T0, T1, c_0 .. c_{nc-1}] (nc >= 2), then sum c_i T_i(x) by the three-term
+// recurrence.  Wave-uniform.
+__device__ inline double rate_fn_eval(const double* f, int nc, double t) {
+    const double x = 2 * (t - f[0]) / (f[1] - f[0]) - 1;
+    double Cnm1 = 1, Cn = x;
+    double y = f[2] + x * f[3];
+    for (int i = 2; i < nc; i++) {
+        const double Cnp1 = 2 * x * Cn - Cnm1;
+        y += Cnp1 * f[2 + i];
+        Cnm1 = Cn;
+        Cn = Cnp1;
+    }
+    return y;
+}
+
+// The rate multipliers of a right-hand side are perturb[s] * fa: the per-slot multipliers
+// (Cantera's m_perturb) with fa = 1, or, while Ember's rate multiplier function is set,
+// unit per-slot multipliers with fa = a(t) for every reaction (CanteraGas::setRateMultiplier,
+// src/chemistry0d.cpp:634-642, called from src/sourceSystem.cpp:133-135 and :453-455).
+// Multiplying by an exact 1 keeps either product bit for bit.
+__device__ __forceinline__ double fn_mult(const DevOpts& o, double t) {
+    return o.ratefn ? rate_fn_eval(o.ratefn, o.ratefn_n, o.t_start + t) : 1.0;
+}
+
 // T-dependent reaction data held in registers (one set per owned slot).
 // NR = compile-time bound on reaction rounds (>= DevMech::nRounds).
 template <int NR>
@@ -295,10 +320,10 @@ __device__ inline void third_body_conc(const DevMech& m, int lane, const GasStat
 }
 
 // Rates of progress into LDS (composition-dependent part of updateROP).
-// perturb: per-slot rate multipliers (Cantera m_perturb).
+// Rate multipliers perturb[s] * fa (Cantera m_perturb; fa: the rate multiplier function).
 template <int NR>
 __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasState& g,
-                                         const RateRegs<NR>& R, const double* perturb,
+                                         const RateRegs<NR>& R, const double* perturb, double fa,
                                          const WaveLds& L) {
     third_body_conc(m, lane, g, L);
 #pragma unroll
@@ -326,7 +351,7 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
                     pr *= F / (1.0 + pr);
                     kf = pr * khigh;
                 }
-                double rfn = kf * perturb[s];
+                double rfn = kf * perturb[s] * fa;
                 ropf = rfn;
                 if (kind == kThreeBody) ropf *= L.Mg[info_group(w)];
                 ropr = ropf;

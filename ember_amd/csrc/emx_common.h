@@ -109,6 +109,8 @@ struct DevMech {
     const double* kc_s;
 };
 
+constexpr int kMaxRateFnCoeffs = 32;  // Chebyshev coefficients of the rate multiplier function
+
 // Integrator options as seen by the kernels
 struct DevOpts {
     double qss_epsmin, qss_epsmax, qss_dtmin, qss_dtmax;
@@ -117,6 +119,12 @@ struct DevOpts {
     int qss_stability;
     double rtol, atol_momentum, atol_energy, atol_species, min_step;
     int max_steps;
+    // Ember's rate multiplier function (a ChebyshevFunction): [T0, T1, c_0 .. c_{n-1}] in
+    // device memory, NULL when none is set; the right-hand sides of a stage evaluate it at
+    // t_start plus their time from the stage start
+    const double* ratefn;
+    int ratefn_n;
+    double t_start;
 };
 
 }  // namespace emx

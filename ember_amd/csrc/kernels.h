@@ -29,7 +29,7 @@ int launch_time_derivatives(size_t n, double dt, const double* delta_diff, const
                             const double* sc_diff, const double* sc_prod, double* ddt_diff,
                             double* ddt_prod, hipStream_t st);
 int launch_correct_mass_fractions(int nVars, int nPoints, double* state, hipStream_t st);
-int launch_heat_release(const DevMech& m, const double* state, const double* perturb, int n,
+int launch_heat_release(const DevMech& m, const DevOpts& o, const double* state, const double* perturb, int n,
                         double* qdot, int* status, hipStream_t stream);
 
 }  // namespace emx

@@ -52,7 +52,7 @@ struct QssSystem {
 template <int NR, bool INTERP>
 __device__ inline bool qss_odefun(const DevMech& m, int lane, double y, bool corrector,
                                   QssSystem& S, RateRegs<NR>& R, const double* perturb,
-                                  const WaveLds& L, double& q, double& d) {
+                                  double fa, const WaveLds& L, double& q, double& d) {
     if (!corrector) S.T = bcast(y, 1);  // unroll_y: T not updated in correctors
     GasState g = set_state(m, lane, y, S.rW, S.T, false, L);
     bool ok = true;
@@ -66,7 +66,7 @@ __device__ inline bool qss_odefun(const DevMech& m, int lane, double y, bool cor
         double cp_mole = kGasConstant * (g.mmw * s);
         S.cp = cp_mole / g.mmw;
     }
-    rates_of_progress<NR>(m, lane, g, R, perturb, L);
+    rates_of_progress<NR>(m, lane, g, R, perturb, fa, L);
     double wQ, wD;
     species_rates(m, lane, L, wQ, wD);
     S.wQ = wQ;
@@ -113,7 +113,9 @@ constexpr double kNegInf = -__builtin_huge_val();
 // a single call site of the ODE function: the predictor evaluation starts a step,
 // every further evaluation is a corrector.  The control flow and arithmetic follow
 // src/qssintegrator.cpp:98-265 line by line.
-template <int NR, bool INTERP, class Hooks>
+// FN: a rate multiplier function is set (o.ratefn); without one the multiplier factor is
+// the constant 1 and compiles away.
+template <int NR, bool INTERP, class Hooks, bool FN = false>
 __device__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool active,
                              QssLane& Z, QssSystem& S, RateRegs<NR>& R, const double* perturb,
                              const WaveLds& L, double tf, QssResume start, Hooks& H, int& gcount,
@@ -133,7 +135,9 @@ __device__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
     while (true) {
         if (--budget < 0) return -3;
         if (predict && !H.step_start(lane, Z, tn, dt, firstStep)) return 1;
-        if (!qss_odefun<NR, INTERP>(m, lane, Z.y, !predict, S, R, perturb, L, Z.q, Z.d)) return -2;
+        // SourceSystemQSS::odefun(tn + tstart) (src/qssintegrator.cpp:113, :159)
+        const double fa = FN ? fn_mult(o, tn) : 1.0;
+        if (!qss_odefun<NR, INTERP>(m, lane, Z.y, !predict, S, R, perturb, fa, L, Z.q, Z.d)) return -2;
         gcount += 1;
         if (predict) H.predicted(lane, Z, S);
         else if (!H.corrected(lane, Z, S, dt)) return -5;

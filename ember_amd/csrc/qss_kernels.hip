@@ -17,8 +17,9 @@ namespace emx {
 namespace {
 
 // OCC = waves per SIMD the register allocation targets: 2 (up to 8 waves per workgroup,
-// no spills) or 3 (9..12 waves per workgroup, register budget 168).
-template <int NR, bool INTERP, int OCC>
+// no spills) or 3 (9..12 waves per workgroup, register budget 168).  FN: Ember's rate
+// multiplier function is set (o.ratefn).
+template <int NR, bool INTERP, int OCC, bool FN>
 __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
     DevMech mg, DevOpts o, const double* __restrict__ state_in,
     const double* __restrict__ split_const, const double* __restrict__ qloss,
@@ -31,7 +32,7 @@ __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
     const DevMech m = stage_mech(mg, perturb_g, lds, perturb);
     __syncthreads();
     const int lane = lane_id();
-    const int wave = threadIdx.x / 64;
+    const int wave = wave_in_block();
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
     WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
     init_wave_lds(L, lane, m.nRounds);
@@ -69,8 +70,8 @@ __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
         int g = 0, r = 0;
         double qi = 0.0;
         NoHooks H;
-        int err = qss_integrate<NR, INTERP>(m, o, lane, active, Z, S, R, perturb, L, dt,
-                                            QssResume{0.0, 0.0, true}, H, g, r, qi);
+        int err = qss_integrate<NR, INTERP, NoHooks, FN>(m, o, lane, active, Z, S, R, perturb, L, dt,
+                                                         QssResume{0.0, 0.0, true}, H, g, r, qi);
         double* yout = state_out + (size_t)p * nv;
         if (active) yout[lane] = (err >= 0) ? Z.y : y0;
         if (lane == 0) {
@@ -99,7 +100,7 @@ __global__ void __launch_bounds__(256) rates_kernel(DevMech mg, const double* __
     const DevMech m = stage_mech(mg, perturb_g, lds, perturb);
     __syncthreads();
     const int lane = lane_id();
-    const int wave = threadIdx.x / 64;
+    const int wave = wave_in_block();
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
     WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
     init_wave_lds(L, lane, m.nRounds);
@@ -113,7 +114,7 @@ __global__ void __launch_bounds__(256) rates_kernel(DevMech mg, const double* __
     GasState g = set_state(m, lane, y, rW, T[p], nonorm != 0, L);
     double hRT, cpR;
     bool ok = update_T<NR, INTERP>(m, lane, g, R, L, hRT, cpR);
-    rates_of_progress<NR>(m, lane, g, R, perturb, L);
+    rates_of_progress<NR>(m, lane, g, R, perturb, 1.0, L);
     double wQ, wD;
     species_rates(m, lane, L, wQ, wD);
     if (is_species_lane(lane, m)) {
@@ -125,19 +126,20 @@ __global__ void __launch_bounds__(256) rates_kernel(DevMech mg, const double* __
     if (lane == 0 && status) status[p] = ok ? 0 : -2;
 }
 
-// FlameSolver::calculateQdot (src/flameSolver.cpp:926-939) per point: setStateMass,
-// getEnthalpies, getReactionRates, qdot = -sum_k wdot_k h_k [W/m^3].  One wavefront per
-// point over Ember's point-contiguous state (U, T, Y).
+// FlameSolver::calculateQdot (src/flameSolver.cpp:926-939) per point: the rate multiplier
+// function's a(t_now) when one is set, setStateMass, getEnthalpies, getReactionRates,
+// qdot = -sum_k wdot_k h_k [W/m^3].  One wavefront per point over Ember's
+// point-contiguous state (U, T, Y).
 template <int NR, bool INTERP>
 __global__ void __launch_bounds__(256) heat_release_kernel(DevMech mg, const double* __restrict__ state,
                                                            const double* __restrict__ perturb_g, int n,
-                                                           double* qdot, int* status) {
+                                                           DevOpts o, double* qdot, int* status) {
     extern __shared__ double lds[];
     const double* perturb;
     const DevMech m = stage_mech(mg, perturb_g, lds, perturb);
     __syncthreads();
     const int lane = lane_id();
-    const int wave = threadIdx.x / 64;
+    const int wave = wave_in_block();
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
     WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
     init_wave_lds(L, lane, m.nRounds);
@@ -153,7 +155,7 @@ __global__ void __launch_bounds__(256) heat_release_kernel(DevMech mg, const dou
     GasState g = set_state(m, lane, y, rW, T, false, L);
     double hRT, cpR;
     const bool ok = update_T<NR, INTERP>(m, lane, g, R, L, hRT, cpR);
-    rates_of_progress<NR>(m, lane, g, R, perturb, L);
+    rates_of_progress<NR>(m, lane, g, R, perturb, fn_mult(o, 0.0), L);
     double wQ, wD;
     species_rates(m, lane, L, wQ, wD);
     const double hk = sp ? hRT * kGasConstant * T : 0.0;  // J/kmol
@@ -178,16 +180,22 @@ int launch_qss_stage(const DevMech& m, const DevOpts& o, const double* state_in,
     size_t lds = ((size_t)mech_lds_doubles(m) + (size_t)waves_per_block * wave_lds_doubles(m.nRounds, m.nChunkRounds)) * sizeof(double);
     if (waves_per_block < 1 || waves_per_block > 12) return (int)hipErrorInvalidConfiguration;
     if (queue) hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, stream, queue);
-#define EMX_QSS_LAUNCH(INTERP, OCC)                                                                    \
-    hipLaunchKernelGGL((qss_stage_kernel<NR, INTERP, OCC>), dim3(grid_blocks), dim3(64 * waves_per_block), \
-                       lds, stream, m, o, state_in, split_const, qloss, perturb, dt, nPoints, order,  \
-                       queue, state_out, status, stat1, stat2, cost, qint)
+#define EMX_QSS_LAUNCH(INTERP, OCC, FN)                                                               \
+    hipLaunchKernelGGL((qss_stage_kernel<NR, INTERP, OCC, FN>), dim3(grid_blocks),                     \
+                       dim3(64 * waves_per_block), lds, stream, m, o, state_in, split_const, qloss,   \
+                       perturb, dt, nPoints, order, queue, state_out, status, stat1, stat2, cost, qint)
     const bool occ3 = waves_per_block > 8;
+    const bool fn = o.ratefn != nullptr;
+    if (fn && occ3) return (int)hipErrorInvalidConfiguration;  // the function runs at up to 8 waves
     EMX_DISPATCH_NR(m.nRounds, {
         if (m.interp) {
-            if (occ3) EMX_QSS_LAUNCH(true, 3); else EMX_QSS_LAUNCH(true, 2);
+            if (fn) EMX_QSS_LAUNCH(true, 2, true);
+            else if (occ3) EMX_QSS_LAUNCH(true, 3, false);
+            else EMX_QSS_LAUNCH(true, 2, false);
         } else {
-            if (occ3) EMX_QSS_LAUNCH(false, 3); else EMX_QSS_LAUNCH(false, 2);
+            if (fn) EMX_QSS_LAUNCH(false, 2, true);
+            else if (occ3) EMX_QSS_LAUNCH(false, 3, false);
+            else EMX_QSS_LAUNCH(false, 2, false);
         }
     });
 #undef EMX_QSS_LAUNCH
@@ -211,18 +219,18 @@ int launch_rates(const DevMech& m, const double* T, const double* Y, int nonorm,
     return (int)hipGetLastError();
 }
 
-int launch_heat_release(const DevMech& m, const double* state, const double* perturb, int n,
-                        double* qdot, int* status, hipStream_t stream) {
+int launch_heat_release(const DevMech& m, const DevOpts& o, const double* state, const double* perturb,
+                        int n, double* qdot, int* status, hipStream_t stream) {
     const int wpb = 4;
     const size_t lds = qss_lds_bytes(m, wpb);
     const int blocks = (n + wpb - 1) / wpb;
     EMX_DISPATCH_NR(m.nRounds, {
         if (m.interp)
             hipLaunchKernelGGL((heat_release_kernel<NR, true>), dim3(blocks), dim3(64 * wpb), lds,
-                               stream, m, state, perturb, n, qdot, status);
+                               stream, m, state, perturb, n, o, qdot, status);
         else
             hipLaunchKernelGGL((heat_release_kernel<NR, false>), dim3(blocks), dim3(64 * wpb), lds,
-                               stream, m, state, perturb, n, qdot, status);
+                               stream, m, state, perturb, n, o, qdot, status);
     });
     return (int)hipGetLastError();
 }

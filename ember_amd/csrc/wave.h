@@ -8,6 +8,10 @@
 namespace emx {
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+// index of this wavefront within its workgroup, as a wave-uniform (scalar) value
+__device__ __forceinline__ int wave_in_block() {
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+}
 
 // ---- DPP data movement on 64-bit values (two 32-bit halves moved with one pattern).
 // `identity` is written where the pattern has no source lane.

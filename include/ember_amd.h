@@ -87,6 +87,21 @@ int emx_set_options(emx_context* ctx, const emx_options* opts);
  * CanteraGas::setRateMultiplier (src/chemistry0d.cpp:634-642).  NULL resets to 1. */
 int emx_set_rate_multipliers(emx_context* ctx, const double* multipliers);
 
+/* Ember's rate multiplier function (FlameSolver::rateMultiplierFunction, a
+ * ChebyshevFunction whose coefficients the Python layer sets before every step,
+ * python/ember/_ember.pyx:533-543): data = [T0, T1, c_0, ..., c_{n-3}] with the meaning of
+ * ChebyshevFunction::setCoefficients(n, data) (src/scalarFunction.cpp:33-51), at most 32
+ * coefficients.  While set, every right-hand side of the production stages multiplies all
+ * reactions by a(t) = sum c_i T_i(2 (t - T0)/(T1 - T0) - 1) at its absolute time t
+ * (t_stage_start plus the integrator's time, src/sourceSystem.cpp:133-135, :453-455), in
+ * place of the per-reaction multipliers; emx_heat_release_device uses a(t_now)
+ * (src/flameSolver.cpp:929-931).  n = 0 or data NULL removes it.  Waits for the context
+ * stream before replacing the coefficients (a captured graph keeps the t_stage_start of
+ * its capture but reads the current coefficients).  QSS stages with a function run at
+ * most 8 waves per workgroup.  The adjoint entry points refuse to run while a function is
+ * set, and the plain kernel variant does not support it. */
+int emx_set_rate_multiplier_function(emx_context* ctx, int n, const double* data);
+
 /* HIP stream (hipStream_t) for the *_device entry points; NULL = default stream. */
 int emx_set_stream(emx_context* ctx, void* stream);
 
@@ -241,9 +256,10 @@ int emx_correct_mass_fractions_device(emx_context* ctx, size_t nVars, size_t nPo
 
 /* FlameSolver::calculateQdot (src/flameSolver.cpp:926-939): qdot[j] = -sum_k wdot_k h_k
  * [W/m^3] at each point's (T, Y) of the device state matrix; qdot (nPoints) device.
+ * t_now: the solver time, for the rate multiplier function (ignored without one).
  * status (nPoints, device, nullable): 0, or -2 if T is outside the interpolation table. */
 int emx_heat_release_device(emx_context* ctx, size_t nVars, size_t nPoints, const double* state,
-                            double* qdot, int32_t* status);
+                            double t_now, double* qdot, int32_t* status);
 
 /* TridiagonalIntegrator::integrateToTime for nSys constant systems y' = A y + k
  * (test/tridiagonalIntegrator.cpp ConstantTridiagonalODE).  a, b, c, k, y: nSys x N,

@@ -57,6 +57,10 @@ struct Mech {
     std::vector<double> W, Tmid, low, high;  // low/high: K x 7
     std::vector<Rxn> rxn;
     std::vector<double> perturb;
+    // Ember's rate multiplier function (a ChebyshevFunction): [T0, T1, c_0, c_1, ...] as
+    // ChebyshevFunction::setCoefficients stores them (src/scalarFunction.cpp:33-51); empty
+    // when none is set
+    std::vector<double> ratefn;
     std::vector<double> dn;
     // Cantera StoichManagerN lists: grouped by molecule count (1, 2, 3)
     struct Stoich { int rxn; int n; int k[3]; };
@@ -70,6 +74,23 @@ struct Mech {
     bool interp_ready = false;
     double interp_P = -1;
 };
+
+// ChebyshevFunction::a (src/scalarFunction.cpp:53-72): t mapped onto [-1, 1] over
+// [T0, T1], then the sum of c_i T_i(x) with the three-term recurrence
+double rate_fn_eval(const Mech& m, double t) {
+    const double* f = m.ratefn.data();
+    const int nc = (int)m.ratefn.size() - 2;
+    const double x = 2 * (t - f[0]) / (f[1] - f[0]) - 1;
+    double Cnm1 = 1, Cn = x;
+    double y = f[2] + x * f[3];
+    for (int i = 2; i < nc; i++) {
+        const double Cnp1 = 2 * x * Cn - Cnm1;
+        y += Cnp1 * f[2 + i];
+        Cnm1 = Cn;
+        Cn = Cnp1;
+    }
+    return y;
+}
 
 void build_stoich(Mech& m) {
     m.reac.clear(); m.prod.clear(); m.revprod.clear();
@@ -184,6 +205,16 @@ struct Gas {
     std::vector<double> kf0, rkcn, concm, rfn, ropf, ropr, ropnet;
     double thermoT = -1.0, rateT = -1.0;
     const double* perturb = nullptr;  // per-thread rate multipliers (NULL: the mechanism's)
+    // CanteraGas::setRateMultiplier(a) (src/chemistry0d.cpp:634-642): every reaction's
+    // multiplier set to one value, in place of the per-reaction ones
+    bool uniform_mult = false;
+    double mult_all = 1.0;
+    // the multipliers at time t when the mechanism has a rate multiplier function
+    // (SourceSystemQSS::odefun / SourceSystemCVODE::f, src/sourceSystem.cpp:453-455, :133-135)
+    void rate_function_at(double t) {
+        uniform_mult = !m.ratefn.empty();
+        if (uniform_mult) mult_all = rate_fn_eval(m, t);
+    }
 
     Gas(Mech& mech, int kin, double pressure)
         : m(mech), kinetics(kin), P(pressure), y(m.K), ym(m.K), conc(m.K), cpR(m.K), hRT(m.K),
@@ -365,7 +396,10 @@ struct Gas {
             kf0[i] = pr * khigh;
         }
         const double* pt = perturb ? perturb : m.perturb.data();
-        for (int i = 0; i < m.I; i++) rfn[i] = kf0[i] * pt[i];
+        if (uniform_mult)
+            for (int i = 0; i < m.I; i++) rfn[i] = kf0[i] * mult_all;
+        else
+            for (int i = 0; i < m.I; i++) rfn[i] = kf0[i] * pt[i];
         for (int i = 0; i < m.I; i++) ropf[i] = rfn[i];
         for (int i = 0; i < m.I; i++)
             if (m.rxn[i].type == 1) ropf[i] *= concm[m.tb_slot[i]];
@@ -603,8 +637,8 @@ struct SourceQSS : QssOde {
     }
 
     void odefun(double t, const double* yy, double* qq, double* dd, bool corrector) override {
-        (void)t;
         unroll_y(yy, corrector);
+        gas.rate_function_at(t);
         gas.setMassFractions(Yv.data());
         gas.setState_TP(T);
         if (!gas.creationRates(wQ.data())) failed_rates = true;
@@ -627,7 +661,8 @@ struct SourceQSS : QssOde {
             dd[2 + k] = wD[k] * gas.m.W[k] / rho;
         }
     }
-    void setState(const double* yIn) { integrator.setState(yIn, 0.0); }
+    double tStart = 0.0;
+    void setState(const double* yIn) { integrator.setState(yIn, tStart); }
     int integrateToTime(double tf) { return integrator.integrateToTime(tf); }
 };
 
@@ -679,9 +714,12 @@ struct SourceBDF {
         for (int k = 0; k < K; k++) atol[2 + k] = o.atol_species;
     }
 
-    // SourceSystemCVODE::f (src/sourceSystem.cpp:128-164)
-    void f(const double* y, double* ydot) {
+    double tStart = 0.0;  // absolute time of the integration's t = 0 (CVODE's t0 = tInitial)
+
+    // SourceSystemCVODE::f (src/sourceSystem.cpp:128-164) at t (from the stage start)
+    void f(double t, const double* y, double* ydot) {
         nfe++;
+        gas.rate_function_at(tStart + t);
         double T = y[1];
         for (int k = 0; k < K; k++) Yv[k] = y[2 + k];
         gas.setMassFractions_NoNorm(Yv.data());
@@ -700,7 +738,7 @@ struct SourceBDF {
     }
 
     // SourceSystemCVODE::fdJacobian (src/sourceSystem.cpp:268-293); J column-major
-    void fdJacobian(const double* y, const double* ydot, double* J) {
+    void fdJacobian(double t, const double* y, const double* ydot, double* J) {
         njac++;
         std::vector<double> yp(y, y + N), yd2(N);
         const double eps = std::sqrt(std::numeric_limits<double>::epsilon());
@@ -709,7 +747,7 @@ struct SourceBDF {
             for (int k = 0; k < N; k++) yp[k] = y[k];
             double dy = (std::abs(y[i]) > atl) ? std::abs(y[i]) * eps : std::abs(y[i]) * eps + atl;
             yp[i] += dy;
-            f(yp.data(), yd2.data());
+            f(t, yp.data(), yd2.data());
             for (int k = 0; k < N; k++) J[(size_t)i * N + k] = (yd2[k] - ydot[k]) / dy;
         }
     }
@@ -806,7 +844,7 @@ struct SourceBDF {
 
         std::vector<double> f0(N), scale(N), f1(N), y1(N);
         double t = 0.0;
-        f(y, f0.data());
+        f(t, y, f0.data());
         double qprev = qDot;
         double qint = 0.0;
         // initial step (Hairer/Shampine select_initial_step, order 1)
@@ -815,7 +853,7 @@ struct SourceBDF {
         double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
         h0 = std::min(h0, tf);
         for (int i = 0; i < N; i++) y1[i] = y[i] + h0 * f0[i];
-        f(y1.data(), f1.data());
+        f(t + h0, y1.data(), f1.data());  // select_initial_step: f(t0 + h0, y0 + h0 f0)
         for (int i = 0; i < N; i++) y1[i] = f1[i] - f0[i];
         double d2 = rms(y1.data(), scale.data()) / h0;
         double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? std::max(1e-6, h0 * 1e-3)
@@ -829,7 +867,7 @@ struct SourceBDF {
         std::vector<double> J((size_t)N * N), LU((size_t)N * N);
         std::vector<int> piv(N);
         bool lu_valid = false;
-        fdJacobian(y, f0.data(), J.data());
+        fdJacobian(t, y, f0.data(), J.data());
         std::vector<double> ypred(N), psi(N), ynew(N), dvec(N), dy(N), fy(N), err(N);
 
         while (t < tf) {
@@ -882,7 +920,7 @@ struct SourceBDF {
                     converged = false;
                     int k;
                     for (k = 0; k < NEWTON_MAXITER; k++) {
-                        f(ynew.data(), fy.data());
+                        f(t_new, ynew.data(), fy.data());
                         bool finite = true;
                         for (int v = 0; v < N; v++) finite = finite && std::isfinite(fy[v]);
                         if (!finite) break;
@@ -903,8 +941,8 @@ struct SourceBDF {
                     if (!converged) {
                         if (current_jac) break;
                         std::vector<double> fp(N);
-                        f(ypred.data(), fp.data());
-                        fdJacobian(ypred.data(), fp.data(), J.data());
+                        f(t_new, ypred.data(), fp.data());
+                        fdJacobian(t_new, ypred.data(), fp.data(), J.data());
                         lu_valid = false;
                         current_jac = true;
                     }
@@ -939,7 +977,7 @@ struct SourceBDF {
             // heat release on the accepted step (trapezoid); not counted in nfe
             {
                 std::vector<double> tmp(N);
-                f(y, tmp.data());
+                f(t, y, tmp.data());
                 nfe--;
                 qint += 0.5 * hstep * (qprev + qDot);
                 qprev = qDot;
@@ -1110,6 +1148,7 @@ void orc_default_options(orc_options* o) {
     o->atol_species = 1e-13;
     o->min_step = 1e-18;
     o->max_steps = 1000000;
+    o->t_start = 0.0;
 }
 
 void* orc_mech_load(const char* path) { return load_mech(path); }
@@ -1119,6 +1158,27 @@ int orc_mech_nreactions(void* mech) { return static_cast<Mech*>(mech)->I; }
 void orc_mech_set_multipliers(void* mech, const double* m) {
     Mech* me = static_cast<Mech*>(mech);
     for (int i = 0; i < me->I; i++) me->perturb[i] = m ? m[i] : 1.0;
+}
+
+int orc_mech_set_rate_function(void* mech, int n, const double* data) {
+    Mech* me = static_cast<Mech*>(mech);
+    me->ratefn.clear();
+    if (n == 0 || !data) return 0;
+    if (n < 2) return -1;
+    // ChebyshevFunction::setCoefficients: at least two coefficients, zero-filled
+    me->ratefn = {data[0], data[1]};
+    if (n >= 4)
+        me->ratefn.insert(me->ratefn.end(), data + 2, data + n);
+    else if (n == 3)
+        me->ratefn.insert(me->ratefn.end(), {data[2], 0.0});
+    else
+        me->ratefn.insert(me->ratefn.end(), {0.0, 0.0});
+    return 0;
+}
+
+double orc_rate_function_eval(void* mech, double t) {
+    const Mech* me = static_cast<Mech*>(mech);
+    return me->ratefn.empty() ? 1.0 : rate_fn_eval(*me, t);
 }
 
 int orc_rates(void* mech, const orc_options* o, int n, const double* T, const double* Y, int nonorm,
@@ -1171,6 +1231,8 @@ int orc_source_integrate(void* mech, const orc_options* o, int nPoints, const do
         Gas gas(m, o->kinetics, o->pressure);  // per-thread gas (gases.local())
         SourceQSS qss(gas, *o);
         SourceBDF bdf(gas, *o);
+        qss.tStart = o->t_start;
+        bdf.tStart = o->t_start;
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 1)
 #endif
@@ -1247,7 +1309,8 @@ int orc_source_rhs(void* mech, const orc_options* o, const double* y, const doub
     SourceBDF bdf(gas, *o);
     for (int i = 0; i < m.K + 2; i++) bdf.splitConst[i] = split_const ? split_const[i] : 0.0;
     bdf.qLoss = qloss;
-    bdf.f(y, ydot);
+    bdf.tStart = o->t_start;
+    bdf.f(0.0, y, ydot);
     return bdf.failed_rates ? -2 : 0;
 }
 
@@ -1330,6 +1393,7 @@ int orc_qss_adjoint(void* mech, const orc_options* o, int nPoints, const double*
                     double* grad_lnk, int* status, int nthreads) {
     Mech& m = *static_cast<Mech*>(mech);
     if (!adjoint_options_ok(o)) return -6;
+    if (!m.ratefn.empty()) return -8;  // the ln-multiplier parameters need per-reaction multipliers
     const int N = m.K + 2;
     prepare_interp(m, o);
 #ifdef _OPENMP
@@ -1382,6 +1446,7 @@ int orc_qss_fd_gradient(void* mech, const orc_options* o, int nPoints, const dou
                         double* grad_lnk, int nthreads) {
     Mech& m = *static_cast<Mech*>(mech);
     if (!adjoint_options_ok(o)) return -6;
+    if (!m.ratefn.empty()) return -8;  // the ln-multiplier parameters need per-reaction multipliers
     const int N = m.K + 2;
     prepare_interp(m, o);
 #ifdef _OPENMP

@@ -29,6 +29,9 @@ typedef struct orc_options {
     /* BDF tolerances, python/ember/input.py:729-746 */
     double rtol, atol_momentum, atol_energy, atol_species, min_step;
     int max_steps;
+    /* absolute time of the stage start (SourceSystem::setState tInitial); the right-hand
+     * sides of a stage see t_start + t (only the rate multiplier function depends on it) */
+    double t_start;
 } orc_options;
 
 void orc_default_options(orc_options* o);
@@ -39,6 +42,14 @@ int orc_mech_nspecies(void* mech);
 int orc_mech_nreactions(void* mech);
 /* Per-reaction rate multipliers (Cantera setMultiplier / m_perturb). */
 void orc_mech_set_multipliers(void* mech, const double* m);
+/* Ember's rate multiplier function, FlameSolver::rateMultiplierFunction: a ChebyshevFunction
+ * whose coefficients ChebyshevFunction::setCoefficients(n, data) sets (data = [T0, T1,
+ * c_0, ...], src/scalarFunction.cpp:33-51).  While set, every right-hand side multiplies
+ * all reactions by a(t) at its own time, in place of the per-reaction multipliers
+ * (src/sourceSystem.cpp:133-135, :453-455).  n = 0 or data NULL removes it; -1 for n = 1. */
+int orc_mech_set_rate_function(void* mech, int n, const double* data);
+/* a(t) of the function set on mech (1 when none is set) */
+double orc_rate_function_eval(void* mech, double t);
 
 /* Rates at n states.  nonorm=0: setState_TPY (clip + normalise);
  * nonorm=1: setMassFractions_NoNorm + setState_TP.  Any output may be NULL. */
@@ -90,7 +101,8 @@ int orc_diffusion_integrate(int nVars, int N, const double* B, int nB, const int
  * over accepted steps) and the gradients of J = Qbar*Q + ybar_final . y_final with the
  * step sequence frozen: grad_y0 (nPoints x nVars) w.r.t. the initial state and grad_lnk
  * (nPoints x nReactions) w.r.t. the logs of the rate multipliers.  Qbar NULL = ones,
- * ybar_final NULL = zeros.  status as orc_source_integrate. */
+ * ybar_final NULL = zeros.  status as orc_source_integrate.  -6: unsupported options,
+ * -8: a rate multiplier function is set. */
 int orc_qss_adjoint(void* mech, const orc_options* o, int nPoints, const double* state_in,
                     const double* split_const, const double* qloss, double dt, const double* Qbar,
                     const double* ybar_final, double* state_out, double* Q, double* grad_y0,

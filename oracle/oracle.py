@@ -24,7 +24,8 @@ class Options(C.Structure):
                 ("qss_dtmax", C.c_double), ("qss_itermax", C.c_int), ("qss_abstol", C.c_double),
                 ("qss_minval", C.c_double), ("qss_stability_check", C.c_int),
                 ("rtol", C.c_double), ("atol_momentum", C.c_double), ("atol_energy", C.c_double),
-                ("atol_species", C.c_double), ("min_step", C.c_double), ("max_steps", C.c_int)]
+                ("atol_species", C.c_double), ("min_step", C.c_double), ("max_steps", C.c_int),
+                ("t_start", C.c_double)]
 
 
 def build() -> str:
@@ -48,6 +49,9 @@ def lib():
         L.orc_mech_nspecies.argtypes = [C.c_void_p]
         L.orc_mech_nreactions.argtypes = [C.c_void_p]
         L.orc_mech_set_multipliers.argtypes = [C.c_void_p, _dp]
+        L.orc_mech_set_rate_function.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_rate_function_eval.restype = C.c_double
+        L.orc_rate_function_eval.argtypes = [C.c_void_p, C.c_double]
         L.orc_rates.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, C.c_int,
                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_source_integrate.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, _dp,
@@ -101,6 +105,20 @@ class Mech:
     def set_multipliers(self, m):
         m = None if m is None else np.ascontiguousarray(m, dtype=np.float64)
         lib().orc_mech_set_multipliers(self.h, _p(m))
+
+    def set_rate_function(self, data):
+        """ChebyshevFunction::setCoefficients(len(data), data), data = [T0, T1, c0, ...];
+        None removes the function."""
+        if data is None:
+            rc = lib().orc_mech_set_rate_function(self.h, 0, None)
+        else:
+            d = np.ascontiguousarray(data, dtype=np.float64)
+            rc = lib().orc_mech_set_rate_function(self.h, len(d), _p(d))
+        if rc:
+            raise ValueError(f"orc_mech_set_rate_function failed: {rc}")
+
+    def rate_function(self, t):
+        return lib().orc_rate_function_eval(self.h, float(t))
 
     def rates(self, T, Y, opts=None, nonorm=False):
         opts = opts or options()
