@@ -38,6 +38,9 @@ class Options(C.Structure):
         ("integrator_energy_abs_tol", C.c_double), ("integrator_species_abs_tol", C.c_double),
         ("integrator_min_timestep", C.c_double), ("integrator_max_steps", C.c_int32),
         ("diffusion_timestep_multiplier", C.c_double), ("global_timestep", C.c_double),
+        ("ignition_t_start", C.c_double), ("ignition_duration", C.c_double),
+        ("ignition_energy", C.c_double), ("ignition_center", C.c_double),
+        ("ignition_stddev", C.c_double),
     ]
 
 
@@ -63,6 +66,7 @@ SIGNATURES = [
     ("emx_set_options", C.c_int, [_vp, C.POINTER(Options)]),
     ("emx_set_rate_multipliers", C.c_int, [_vp, _dp]),
     ("emx_set_rate_multiplier_function", C.c_int, [_vp, C.c_int, _dp]),
+    ("emx_set_point_positions", C.c_int, [_vp, _sz, _dp]),
     ("emx_set_stream", C.c_int, [_vp, _vp]),
     ("emx_set_launch", C.c_int, [_vp, C.c_int, C.c_int]),
     ("emx_set_kernel_variant", C.c_int, [_vp, C.c_int]),
@@ -185,6 +189,15 @@ class Context:
             return
         d = np.ascontiguousarray(data, dtype=np.float64)
         _check(lib().emx_set_rate_multiplier_function(self.h, len(d), _p(d)))
+
+    def set_point_positions(self, x):
+        """Grid positions of the stage's points (SourceSystem::setPosition), for the
+        igniter; None clears."""
+        if x is None:
+            _check(lib().emx_set_point_positions(self.h, 0, None))
+            return
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        _check(lib().emx_set_point_positions(self.h, len(x), _p(x)))
 
     def set_stream(self, stream_handle: int):
         _check(lib().emx_set_stream(self.h, C.c_void_p(stream_handle)))

@@ -30,6 +30,7 @@ constexpr int kNewtonMaxIter = 4;
 
 struct BdfSystem {
     double W, rW, sc, qloss, atol;
+    double qign;  // igniter amplitude at this point's position (0: none)
     double qdot;
     int nfe;
 };
@@ -37,7 +38,8 @@ struct BdfSystem {
 // SourceSystemCVODE::f for this lane's variable.  Returns false if rates failed.
 template <int NR, bool INTERP>
 __device__ inline bool bdf_f(const DevMech& m, int lane, double yv, BdfSystem& S, RateRegs<NR>& R,
-                             const double* perturb, double fa, const WaveLds& L, double& ydot) {
+                             const double* perturb, double fa, double qi, const WaveLds& L,
+                             double& ydot) {
     S.nfe++;
     const double T = bcast(yv, 1);
     GasState g = set_state(m, lane, yv, S.rW, T, true, L);
@@ -52,7 +54,7 @@ __device__ inline bool bdf_f(const DevMech& m, int lane, double yv, BdfSystem& S
     const double rho = g.dens;
     const double s = wave_sum(g.ym * cpR);
     const double cp = (kGasConstant * (g.mmw * s)) / g.mmw;
-    double qdot = -wave_sum(sp ? wdot * hk : 0.0);
+    double qdot = -wave_sum(sp ? wdot * hk : 0.0) + qi;  // + getQdotIgniter(t)
     qdot -= S.qloss;
     S.qdot = qdot;
     if (lane == 0) ydot = S.sc;
@@ -164,8 +166,8 @@ __device__ double lu_solve(const double* M, int n, int lane, int step_of, double
 // SourceSystemCVODE::fdJacobian: column i written to J[i*64 + lane]
 template <int NR, bool INTERP>
 __device__ bool fd_jacobian(const DevMech& m, int lane, int n, double yv, double ydot, BdfSystem& S,
-                            RateRegs<NR>& R, const double* perturb, double fa, const WaveLds& L,
-                            double* J) {
+                            RateRegs<NR>& R, const double* perturb, double fa, double qi,
+                            const WaveLds& L, double* J) {
     const double eps = 1.4901161193847656e-08;  // sqrt(DBL_EPSILON)
     const double atl = 2.220446049250313e-16;   // DBL_EPSILON
     bool ok = true;
@@ -175,7 +177,7 @@ __device__ bool fd_jacobian(const DevMech& m, int lane, int n, double yv, double
         double yp = yv;
         if (lane == i) yp += dy;
         double yd2;
-        ok = bdf_f<NR, INTERP>(m, lane, yp, S, R, perturb, fa, L, yd2) && ok;
+        ok = bdf_f<NR, INTERP>(m, lane, yp, S, R, perturb, fa, qi, L, yd2) && ok;
         J[i * 64 + lane] = (yd2 - ydot) / dy;
     }
     return ok;
@@ -194,14 +196,15 @@ __device__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
     bool ok = true;
     double f0;
     // the right-hand sides see t_start + t (CVODE's t0 = tInitial, src/sourceSystem.cpp:298)
-    ok = bdf_f<NR, INTERP>(m, lane, y, S, R, perturb, fn_mult(o, 0.0), L, f0) && ok;
+    ok = bdf_f<NR, INTERP>(m, lane, y, S, R, perturb, fn_mult(o, 0.0), igniter(o, S.qign, 0.0), L, f0) && ok;
     double qprev = S.qdot, qint = 0.0;
     double scale = S.atol + fabs(y) * rtol;
     const double d0 = rms(y, scale, active, n), d1 = rms(f0, scale, active, n);
     double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
     h0 = fmin(h0, tf);
     double f1;
-    ok = bdf_f<NR, INTERP>(m, lane, y + h0 * f0, S, R, perturb, fn_mult(o, h0), L, f1) && ok;
+    ok = bdf_f<NR, INTERP>(m, lane, y + h0 * f0, S, R, perturb, fn_mult(o, h0), igniter(o, S.qign, h0), L,
+                           f1) && ok;
     const double d2 = rms(f1 - f0, scale, active, n) / h0;
     const double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? fmax(1e-6, h0 * 1e-3)
                                                    : pow(0.01 / fmax(d1, d2), 1.0 / 2.0);
@@ -215,7 +218,8 @@ __device__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
     D[1] = f0 * h_abs;
     bool lu_valid = false;
     int step_of = -1;
-    ok = fd_jacobian<NR, INTERP>(m, lane, n, y, f0, S, R, perturb, fn_mult(o, 0.0), L, J) && ok;
+    ok = fd_jacobian<NR, INTERP>(m, lane, n, y, f0, S, R, perturb, fn_mult(o, 0.0), igniter(o, S.qign, 0.0),
+                                 L, J) && ok;
     if (!ok) return -2;
     double t = 0.0;
     double error_norm = 0.0, safety = 0.0;
@@ -263,7 +267,8 @@ __device__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
                 int k;
                 for (k = 0; k < kNewtonMaxIter; k++) {
                     double fy;
-                    ok = bdf_f<NR, INTERP>(m, lane, ynew, S, R, perturb, fn_mult(o, t_new), L, fy) && ok;
+                    ok = bdf_f<NR, INTERP>(m, lane, ynew, S, R, perturb, fn_mult(o, t_new),
+                                           igniter(o, S.qign, t_new), L, fy) && ok;
                     const int bad = wave_or(active && !isfinite(fy) ? 1 : 0);
                     if (bad) break;
                     double dy = lu_solve(M, n, lane, step_of, c * fy - psi - dvec);
@@ -283,9 +288,9 @@ __device__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
                 if (!converged) {
                     if (current_jac) break;
                     double fp;
-                    const double fa = fn_mult(o, t_new);
-                    ok = bdf_f<NR, INTERP>(m, lane, ypred, S, R, perturb, fa, L, fp) && ok;
-                    ok = fd_jacobian<NR, INTERP>(m, lane, n, ypred, fp, S, R, perturb, fa, L, J) && ok;
+                    const double fa = fn_mult(o, t_new), qi = igniter(o, S.qign, t_new);
+                    ok = bdf_f<NR, INTERP>(m, lane, ypred, S, R, perturb, fa, qi, L, fp) && ok;
+                    ok = fd_jacobian<NR, INTERP>(m, lane, n, ypred, fp, S, R, perturb, fa, qi, L, J) && ok;
                     lu_valid = false;
                     current_jac = true;
                 }
@@ -318,7 +323,7 @@ __device__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
         y = ynew;
         if (qint_out) {
             double ftmp;
-            ok = bdf_f<NR, INTERP>(m, lane, y, S, R, perturb, fn_mult(o, t), L, ftmp) && ok;
+            ok = bdf_f<NR, INTERP>(m, lane, y, S, R, perturb, fn_mult(o, t), igniter(o, S.qign, t), L, ftmp) && ok;
             S.nfe--;
             qint += 0.5 * hstep * (qprev + S.qdot);
             qprev = S.qdot;
@@ -399,6 +404,7 @@ __global__ void __launch_bounds__(256) bdf_stage_kernel(
         double y0 = active ? state_in[(size_t)p * nv + lane] : 0.0;
         S.sc = (active && split_const) ? split_const[(size_t)p * nv + lane] : 0.0;
         S.qloss = qloss ? qloss[p] : 0.0;
+        S.qign = o.qign ? o.qign[p] : 0.0;
         S.nfe = 0;
         R.T = -1.0;
         double y = y0;

@@ -88,6 +88,11 @@ struct emx_context {
     // while set, the kernels take unit per-slot multipliers (`ones`) times a(t)
     DevBuf ratefn, ones;
     int ratefn_n = 0;
+    // igniter: point positions (host) and the per-point pulse amplitudes (device), rebuilt
+    // when the positions or the ignition options change
+    std::vector<double> xpos;
+    DevBuf qign;
+    bool qign_valid = false;
 };
 
 namespace {
@@ -111,6 +116,9 @@ DevOpts to_dev(const emx_options& o) {
     d.ratefn = nullptr;
     d.ratefn_n = 0;
     d.t_start = 0.0;
+    d.qign = nullptr;
+    d.ign_t0 = o.ignition_t_start;
+    d.ign_t1 = o.ignition_t_start + o.ignition_duration;  // as getQdotIgniter forms the bound
     return d;
 }
 
@@ -204,15 +212,36 @@ const double* stage_perturb(const emx_context* c) {
     return c->ratefn_n > 0 ? c->ones.as<double>() : c->perturb;
 }
 
-// kernel options of a stage starting at absolute time t_start
+// kernel options of a stage starting at absolute time t_start (igniter amplitudes as
+// prepared by igniter_ready)
 DevOpts stage_opts(const emx_context* c, double t_start) {
     DevOpts d = c->dopts;
     if (c->ratefn_n > 0) {
         d.ratefn = c->ratefn.as<double>();
         d.ratefn_n = c->ratefn_n;
     }
+    if (c->opts.ignition_energy != 0.0) d.qign = c->qign.as<double>();
     d.t_start = t_start;
     return d;
+}
+
+// The igniter's per-point amplitudes for a stage over nPoints points: getQdotIgniter's
+// spatial factor at each position (src/sourceSystem.cpp:38-41), evaluated on the host.
+int igniter_ready(emx_context* c, size_t nPoints) {
+    if (c->opts.ignition_energy == 0.0) return 0;
+    if (c->xpos.size() != nPoints)
+        return fail(-1, "the igniter needs emx_set_point_positions for the stage's points");
+    if (c->qign_valid) return 0;
+    const emx_options& o = c->opts;
+    std::vector<double> a(nPoints);
+    for (size_t j = 0; j < nPoints; j++)
+        a[j] = o.ignition_energy / (o.ignition_stddev * std::sqrt(2 * M_PI) * o.ignition_duration) *
+               std::exp(-std::pow(c->xpos[j] - o.ignition_center, 2) / (2 * std::pow(o.ignition_stddev, 2)));
+    HIP_TRY(c->qign.ensure(nPoints * sizeof(double)));
+    HIP_TRY(hipStreamSynchronize(c->stream));  // stages in flight read the previous amplitudes
+    HIP_TRY(hipMemcpy(c->qign.p, a.data(), nPoints * sizeof(double), hipMemcpyHostToDevice));
+    c->qign_valid = true;
+    return 0;
 }
 
 int grid_blocks(emx_context* c, int nPoints, bool bdf, int wpb) {
@@ -250,6 +279,11 @@ void emx_default_options(emx_options* o) {
     o->integrator_max_steps = 1000000;
     o->diffusion_timestep_multiplier = 10.0;
     o->global_timestep = 2e-5;
+    o->ignition_t_start = 0.0;  // python/ember/input.py class Ignition
+    o->ignition_duration = 1e-3;
+    o->ignition_energy = 0.0;
+    o->ignition_center = 0.0;
+    o->ignition_stddev = 1e-4;
 }
 
 const char* emx_last_error(void) { return g_err.c_str(); }
@@ -297,7 +331,7 @@ void emx_destroy(emx_context* c) {
                       &c->d_B, &c->d_Brow, &c->d_D, &c->d_sc, &c->d_x, &c->d_y, &c->d_h, &c->d_ns,
                       &c->d_work, &c->d_hmin, &c->t_a, &c->t_b, &c->t_c, &c->t_k, &c->jac, &c->plain_ws,
                       &c->cost, &c->order, &c->ck_y, &c->ck_s, &c->nckpt, &c->record, &c->ratefn,
-                      &c->ones})
+                      &c->ones, &c->qign})
         b->release();
     delete c;
 }
@@ -311,7 +345,18 @@ int emx_set_options(emx_context* c, const emx_options* o) {
     bool interp_changed = o->kinetics_model != c->opts.kinetics_model;
     c->opts = *o;
     c->dopts = to_dev(*o);
+    c->qign_valid = false;
     if (interp_changed) return upload_mech(c);
+    return 0;
+}
+
+int emx_set_point_positions(emx_context* c, size_t nPoints, const double* x) {
+    if (int rc = check_ctx(c)) return rc;
+    if (x)
+        c->xpos.assign(x, x + nPoints);
+    else
+        c->xpos.clear();
+    c->qign_valid = false;
     return 0;
 }
 
@@ -394,9 +439,10 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
     if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
     if (!state_in || !state_out) return fail(-1, "state pointers are NULL");
     if (c->variant == EMX_VARIANT_PLAIN) {
-        if (c->opts.chemistry_integrator != EMX_QSS || c->opts.qss_stability_check || c->ratefn_n)
-            return fail(-1, "the plain baseline kernel implements QSS without stability check "
-                            "or rate multiplier function only");
+        if (c->opts.chemistry_integrator != EMX_QSS || c->opts.qss_stability_check || c->ratefn_n ||
+            c->opts.ignition_energy != 0.0)
+            return fail(-1, "the plain baseline kernel implements QSS without stability check, "
+                            "rate multiplier function or igniter only");
         HIP_TRY(c->plain_ws.ensure(plain_workspace_doubles(c->dm, (int)nPoints) * sizeof(double)));
         int rc = launch_qss_plain(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
                                   (int)nPoints, state_out, status, stat1, stat2,
@@ -404,6 +450,7 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
         if (rc) return fail(-10, std::string("plain launch: ") + hipGetErrorString((hipError_t)rc));
         return 0;
     }
+    if (int rc = igniter_ready(c, nPoints)) return rc;
     // work queue order: descending cost of the previous stage over the same points
     HIP_TRY(c->cost.ensure(nPoints * sizeof(int)));
     HIP_TRY(c->order.ensure(nPoints * sizeof(int)));
@@ -429,8 +476,10 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
         return 0;
     }
     const bool per_point = c->variant == EMX_VARIANT_WAVE_PER_POINT;
-    // the rate multiplier function's kernels run at up to 8 waves per workgroup
-    const int wpb = c->ratefn_n > 0 ? std::min(c->waves_per_block, 8) : c->waves_per_block;
+    // with the rate multiplier function or the igniter the kernels run at up to 8 waves
+    // per workgroup
+    const bool ext = c->ratefn_n > 0 || c->opts.ignition_energy != 0.0;
+    const int wpb = ext ? std::min(c->waves_per_block, 8) : c->waves_per_block;
     const int qblocks = per_point ? (int)((nPoints + wpb - 1) / wpb)
                                   : grid_blocks(c, (int)nPoints, false, wpb);
     int rc = launch_qss_stage(c->dm, stage_opts(c, t_stage_start), state_in, split_const, qloss,
@@ -457,6 +506,7 @@ static int adjoint_check(emx_context* c, size_t nPoints, size_t nVars) {
     if (c->ratefn_n)
         return fail(-1, "the adjoint's ln-multiplier parameters need per-reaction multipliers; "
                         "remove the rate multiplier function");
+    if (c->opts.ignition_energy != 0.0) return fail(-1, "the adjoint does not support the igniter");
     return 0;
 }
 

@@ -144,6 +144,14 @@ __device__ __forceinline__ double fn_mult(const DevOpts& o, double t) {
     return o.ratefn ? rate_fn_eval(o.ratefn, o.ratefn_n, o.t_start + t) : 1.0;
 }
 
+// SourceSystem::getQdotIgniter (src/sourceSystem.cpp:33-45) at time t from the stage start,
+// for a point whose pulse amplitude energy / (stddev sqrt(2 pi) duration) *
+// exp(-(x - center)^2 / (2 stddev^2)) is `amp` (precomputed on the host per position)
+__device__ __forceinline__ double igniter(const DevOpts& o, double amp, double t) {
+    const double ta = o.t_start + t;
+    return (ta >= o.ign_t0 && ta < o.ign_t1) ? amp : 0.0;
+}
+
 // T-dependent reaction data held in registers (one set per owned slot).
 // NR = compile-time bound on reaction rounds (>= DevMech::nRounds).
 template <int NR>

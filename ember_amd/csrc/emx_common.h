@@ -125,6 +125,10 @@ struct DevOpts {
     const double* ratefn;
     int ratefn_n;
     double t_start;
+    // igniter (SourceSystem::getQdotIgniter): per-point amplitudes in device memory (NULL:
+    // off), applied for absolute times in [ign_t0, ign_t1)
+    const double* qign;
+    double ign_t0, ign_t1;
 };
 
 }  // namespace emx

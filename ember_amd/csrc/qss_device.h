@@ -42,6 +42,7 @@ struct QssSystem {
     double W, rW;      // species lanes
     double sc;         // splitConst for this lane's variable
     double qloss;
+    double qign;       // igniter amplitude at this point's position (0: none)
     double T;          // temperature used by the rates (frozen in correctors)
     double rho, cp;    // from the last non-corrector call (SourceSystem::updateThermo)
     double hk;         // J/kmol, species lanes
@@ -49,10 +50,12 @@ struct QssSystem {
     double wQ, wD;     // creation / destruction rate of this lane's species (last evaluation)
 };
 
-template <int NR, bool INTERP>
+// EXT: the call carries the time-dependent external terms (rate multiplier factor fa,
+// igniter heat release qi); without them fa = 1 and qi = 0 compile away.
+template <int NR, bool INTERP, bool EXT = false>
 __device__ inline bool qss_odefun(const DevMech& m, int lane, double y, bool corrector,
                                   QssSystem& S, RateRegs<NR>& R, const double* perturb,
-                                  double fa, const WaveLds& L, double& q, double& d) {
+                                  double fa, double qi, const WaveLds& L, double& q, double& d) {
     if (!corrector) S.T = bcast(y, 1);  // unroll_y: T not updated in correctors
     GasState g = set_state(m, lane, y, S.rW, S.T, false, L);
     bool ok = true;
@@ -72,6 +75,7 @@ __device__ inline bool qss_odefun(const DevMech& m, int lane, double y, bool cor
     S.wQ = wQ;
     S.wD = wD;
     S.qdot = -wave_sum((wQ - wD) * S.hk);
+    if (EXT) S.qdot += qi;  // + getQdotIgniter(t) (src/sourceSystem.cpp:465)
     const double rho = S.rho, cp = S.cp;
     if (lane == 0) {
         q = S.sc;
@@ -113,9 +117,9 @@ constexpr double kNegInf = -__builtin_huge_val();
 // a single call site of the ODE function: the predictor evaluation starts a step,
 // every further evaluation is a corrector.  The control flow and arithmetic follow
 // src/qssintegrator.cpp:98-265 line by line.
-// FN: a rate multiplier function is set (o.ratefn); without one the multiplier factor is
-// the constant 1 and compiles away.
-template <int NR, bool INTERP, class Hooks, bool FN = false>
+// EXT: a rate multiplier function (o.ratefn) or the igniter (o.qign) is on; without them
+// the multiplier factor 1 and the igniter term compile away.
+template <int NR, bool INTERP, class Hooks, bool EXT = false>
 __device__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool active,
                              QssLane& Z, QssSystem& S, RateRegs<NR>& R, const double* perturb,
                              const WaveLds& L, double tf, QssResume start, Hooks& H, int& gcount,
@@ -136,8 +140,10 @@ __device__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool 
         if (--budget < 0) return -3;
         if (predict && !H.step_start(lane, Z, tn, dt, firstStep)) return 1;
         // SourceSystemQSS::odefun(tn + tstart) (src/qssintegrator.cpp:113, :159)
-        const double fa = FN ? fn_mult(o, tn) : 1.0;
-        if (!qss_odefun<NR, INTERP>(m, lane, Z.y, !predict, S, R, perturb, fa, L, Z.q, Z.d)) return -2;
+        const double fa = EXT ? fn_mult(o, tn) : 1.0;
+        const double qi = EXT ? igniter(o, S.qign, tn) : 0.0;
+        if (!qss_odefun<NR, INTERP, EXT>(m, lane, Z.y, !predict, S, R, perturb, fa, qi, L, Z.q, Z.d))
+            return -2;
         gcount += 1;
         if (predict) H.predicted(lane, Z, S);
         else if (!H.corrected(lane, Z, S, dt)) return -5;

@@ -17,9 +17,9 @@ namespace emx {
 namespace {
 
 // OCC = waves per SIMD the register allocation targets: 2 (up to 8 waves per workgroup,
-// no spills) or 3 (9..12 waves per workgroup, register budget 168).  FN: Ember's rate
-// multiplier function is set (o.ratefn).
-template <int NR, bool INTERP, int OCC, bool FN>
+// no spills) or 3 (9..12 waves per workgroup, register budget 168).  EXT: Ember's rate
+// multiplier function (o.ratefn) or the igniter (o.qign) is on.
+template <int NR, bool INTERP, int OCC, bool EXT>
 __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
     DevMech mg, DevOpts o, const double* __restrict__ state_in,
     const double* __restrict__ split_const, const double* __restrict__ qloss,
@@ -60,6 +60,7 @@ __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
         double y0 = active ? yin[lane] : 0.0;
         S.sc = (active && split_const) ? split_const[(size_t)p * nv + lane] : 0.0;
         S.qloss = qloss ? qloss[p] : 0.0;
+        S.qign = (EXT && o.qign) ? o.qign[p] : 0.0;
         R.T = -1.0;
         QssLane Z;
         Z.enforce = active && lane != 0;
@@ -70,8 +71,8 @@ __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
         int g = 0, r = 0;
         double qi = 0.0;
         NoHooks H;
-        int err = qss_integrate<NR, INTERP, NoHooks, FN>(m, o, lane, active, Z, S, R, perturb, L, dt,
-                                                         QssResume{0.0, 0.0, true}, H, g, r, qi);
+        int err = qss_integrate<NR, INTERP, NoHooks, EXT>(m, o, lane, active, Z, S, R, perturb, L, dt,
+                                                          QssResume{0.0, 0.0, true}, H, g, r, qi);
         double* yout = state_out + (size_t)p * nv;
         if (active) yout[lane] = (err >= 0) ? Z.y : y0;
         if (lane == 0) {
@@ -180,13 +181,13 @@ int launch_qss_stage(const DevMech& m, const DevOpts& o, const double* state_in,
     size_t lds = ((size_t)mech_lds_doubles(m) + (size_t)waves_per_block * wave_lds_doubles(m.nRounds, m.nChunkRounds)) * sizeof(double);
     if (waves_per_block < 1 || waves_per_block > 12) return (int)hipErrorInvalidConfiguration;
     if (queue) hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, stream, queue);
-#define EMX_QSS_LAUNCH(INTERP, OCC, FN)                                                               \
-    hipLaunchKernelGGL((qss_stage_kernel<NR, INTERP, OCC, FN>), dim3(grid_blocks),                     \
+#define EMX_QSS_LAUNCH(INTERP, OCC, EXT)                                                              \
+    hipLaunchKernelGGL((qss_stage_kernel<NR, INTERP, OCC, EXT>), dim3(grid_blocks),                    \
                        dim3(64 * waves_per_block), lds, stream, m, o, state_in, split_const, qloss,   \
                        perturb, dt, nPoints, order, queue, state_out, status, stat1, stat2, cost, qint)
     const bool occ3 = waves_per_block > 8;
-    const bool fn = o.ratefn != nullptr;
-    if (fn && occ3) return (int)hipErrorInvalidConfiguration;  // the function runs at up to 8 waves
+    const bool fn = o.ratefn != nullptr || o.qign != nullptr;
+    if (fn && occ3) return (int)hipErrorInvalidConfiguration;  // external terms: up to 8 waves
     EMX_DISPATCH_NR(m.nRounds, {
         if (m.interp) {
             if (fn) EMX_QSS_LAUNCH(true, 2, true);

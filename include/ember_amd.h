@@ -55,6 +55,13 @@ typedef struct emx_options {
     int32_t integrator_max_steps;
     double diffusion_timestep_multiplier;
     double global_timestep;
+    /* igniter, ConfigOptions ignition_* (src/readConfig.h:135-139; python/ember/input.py
+     * class Ignition): SourceSystem::getQdotIgniter (src/sourceSystem.cpp:33-45) adds
+     * energy / (stddev sqrt(2 pi) duration) exp(-(x - center)^2 / (2 stddev^2)) [W/m^3] to
+     * the production stages' qDot for absolute times in [t_start, t_start + duration).
+     * Off while energy = 0 (the default); on, it needs emx_set_point_positions. */
+    double ignition_t_start, ignition_duration, ignition_energy, ignition_center,
+        ignition_stddev;
 } emx_options;
 
 /* One-dimensional grid for the diffusion systems (OneDimGrid, src/grid.h:120-126). */
@@ -101,6 +108,12 @@ int emx_set_rate_multipliers(emx_context* ctx, const double* multipliers);
  * most 8 waves per workgroup.  The adjoint entry points refuse to run while a function is
  * set, and the plain kernel variant does not support it. */
 int emx_set_rate_multiplier_function(emx_context* ctx, int n, const double* data);
+
+/* Grid positions x of the points of the production stages (SourceSystem::setPosition,
+ * src/sourceSystem.cpp:73, called from src/flameSolver.cpp:501, :539), for the igniter.
+ * Host array of nPoints; NULL clears.  The stages' nPoints must match while the igniter is
+ * on. */
+int emx_set_point_positions(emx_context* ctx, size_t nPoints, const double* x);
 
 /* HIP stream (hipStream_t) for the *_device entry points; NULL = default stream. */
 int emx_set_stream(emx_context* ctx, void* stream);

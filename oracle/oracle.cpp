@@ -603,6 +603,16 @@ struct QssIntegrator {
     }
 };
 
+// SourceSystem::getQdotIgniter (src/sourceSystem.cpp:33-45) at position x, time t
+double qdot_igniter(const orc_options& opt, double x, double t) {
+    if (t >= opt.ignition_t_start && t < opt.ignition_t_start + opt.ignition_duration) {
+        return opt.ignition_energy /
+               (opt.ignition_stddev * std::sqrt(2 * M_PI) * opt.ignition_duration) *
+               std::exp(-std::pow(x - opt.ignition_center, 2) / (2 * std::pow(opt.ignition_stddev, 2)));
+    }
+    return 0.0;
+}
+
 // SourceSystemQSS (src/sourceSystem.cpp:398-531)
 struct SourceQSS : QssOde {
     Gas& gas;
@@ -610,10 +620,12 @@ struct SourceQSS : QssOde {
     QssIntegrator integrator;
     std::vector<double> splitConst, Yv, hk, wQ, wD;
     double U = 0, T = 0, rho = 0, cp = 0, qLoss = 0, qDot = 0;
+    double x = 0;  // SourceSystem::setPosition
     bool failed_rates = false;
+    const orc_options& opt;
 
     SourceQSS(Gas& g, const orc_options& o)
-        : gas(g), K(g.m.K), N(g.m.K + 2), splitConst(N), Yv(K), hk(K), wQ(K), wD(K) {
+        : gas(g), K(g.m.K), N(g.m.K + 2), splitConst(N), Yv(K), hk(K), wQ(K), wD(K), opt(o) {
         integrator.ode = this;
         integrator.initialize(N);
         integrator.enforce[0] = 0;                 // initialize: enforce_ymin[kMomentum] = false
@@ -650,7 +662,7 @@ struct SourceQSS : QssOde {
         }
         double s = 0.0;
         for (int k = 0; k < K; k++) s += (wQ[k] - wD[k]) * hk[k];
-        qDot = -s;
+        qDot = -s + qdot_igniter(opt, x, t);
         double dUdtQ = splitConst[0], dUdtD = 0.0;
         double dTdtQ = qDot / (rho * cp) + splitConst[1];
         double dTdtD = qLoss / (rho * cp);
@@ -704,6 +716,7 @@ struct SourceBDF {
     int K, N;
     std::vector<double> splitConst, Yv, hk, wdot, atol;
     double qLoss = 0, qDot = 0;
+    double x = 0;  // SourceSystem::setPosition
     int nfe = 0, nsteps = 0, njac = 0;
     bool failed_rates = false;
 
@@ -730,7 +743,7 @@ struct SourceBDF {
         double cp = gas.cp_mass();
         double s = 0.0;
         for (int k = 0; k < K; k++) s += wdot[k] * hk[k];
-        qDot = -s;
+        qDot = -s + qdot_igniter(o, x, tStart + t);
         qDot -= qLoss;
         ydot[0] = splitConst[0];
         ydot[1] = qDot / (rho * cp) + splitConst[1];
@@ -1149,6 +1162,12 @@ void orc_default_options(orc_options* o) {
     o->min_step = 1e-18;
     o->max_steps = 1000000;
     o->t_start = 0.0;
+    o->ignition_t_start = 0.0;  // python/ember/input.py class Ignition defaults
+    o->ignition_duration = 1e-3;
+    o->ignition_energy = 0.0;
+    o->ignition_center = 0.0;
+    o->ignition_stddev = 1e-4;
+    o->positions = nullptr;
 }
 
 void* orc_mech_load(const char* path) { return load_mech(path); }
@@ -1244,6 +1263,7 @@ int orc_source_integrate(void* mech, const orc_options* o, int nPoints, const do
             if (o->integrator == 0) {
                 for (int i = 0; i < N; i++) qss.splitConst[i] = split_const ? split_const[(size_t)j * N + i] : 0.0;
                 qss.qLoss = qloss ? qloss[j] : 0.0;
+                qss.x = o->positions ? o->positions[j] : 0.0;
                 qss.failed_rates = false;
                 qss.setState(yin);
                 err = qss.integrateToTime(dt);
@@ -1254,6 +1274,7 @@ int orc_source_integrate(void* mech, const orc_options* o, int nPoints, const do
             } else {
                 for (int i = 0; i < N; i++) bdf.splitConst[i] = split_const ? split_const[(size_t)j * N + i] : 0.0;
                 bdf.qLoss = qloss ? qloss[j] : 0.0;
+                bdf.x = o->positions ? o->positions[j] : 0.0;
                 bdf.failed_rates = false;
                 bdf.nfe = bdf.nsteps = bdf.njac = 0;
                 std::vector<double> y(yin, yin + N);
@@ -1394,6 +1415,7 @@ int orc_qss_adjoint(void* mech, const orc_options* o, int nPoints, const double*
     Mech& m = *static_cast<Mech*>(mech);
     if (!adjoint_options_ok(o)) return -6;
     if (!m.ratefn.empty()) return -8;  // the ln-multiplier parameters need per-reaction multipliers
+    if (o->ignition_energy != 0.0) return -9;
     const int N = m.K + 2;
     prepare_interp(m, o);
 #ifdef _OPENMP
@@ -1447,6 +1469,7 @@ int orc_qss_fd_gradient(void* mech, const orc_options* o, int nPoints, const dou
     Mech& m = *static_cast<Mech*>(mech);
     if (!adjoint_options_ok(o)) return -6;
     if (!m.ratefn.empty()) return -8;  // the ln-multiplier parameters need per-reaction multipliers
+    if (o->ignition_energy != 0.0) return -9;
     const int N = m.K + 2;
     prepare_interp(m, o);
 #ifdef _OPENMP

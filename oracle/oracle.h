@@ -30,8 +30,16 @@ typedef struct orc_options {
     double rtol, atol_momentum, atol_energy, atol_species, min_step;
     int max_steps;
     /* absolute time of the stage start (SourceSystem::setState tInitial); the right-hand
-     * sides of a stage see t_start + t (only the rate multiplier function depends on it) */
+     * sides of a stage see t_start + t (the rate multiplier function and the igniter depend
+     * on it) */
     double t_start;
+    /* igniter, ConfigOptions ignition_* (src/readConfig.h:135-139, python/ember/input.py
+     * class Ignition): SourceSystem::getQdotIgniter (src/sourceSystem.cpp:33-45) adds
+     * energy / (stddev sqrt(2 pi) duration) exp(-(x - center)^2 / (2 stddev^2)) to qDot for
+     * t in [t_start_ign, t_start_ign + duration).  positions: nPoints grid positions x
+     * (SourceSystem::setPosition); NULL = 0 for every point. */
+    double ignition_t_start, ignition_duration, ignition_energy, ignition_center, ignition_stddev;
+    const double* positions;
 } orc_options;
 
 void orc_default_options(orc_options* o);
@@ -102,7 +110,7 @@ int orc_diffusion_integrate(int nVars, int N, const double* B, int nB, const int
  * step sequence frozen: grad_y0 (nPoints x nVars) w.r.t. the initial state and grad_lnk
  * (nPoints x nReactions) w.r.t. the logs of the rate multipliers.  Qbar NULL = ones,
  * ybar_final NULL = zeros.  status as orc_source_integrate.  -6: unsupported options,
- * -8: a rate multiplier function is set. */
+ * -8: a rate multiplier function is set, -9: the igniter is on. */
 int orc_qss_adjoint(void* mech, const orc_options* o, int nPoints, const double* state_in,
                     const double* split_const, const double* qloss, double dt, const double* Qbar,
                     const double* ybar_final, double* state_out, double* Q, double* grad_y0,

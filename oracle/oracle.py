@@ -25,7 +25,10 @@ class Options(C.Structure):
                 ("qss_minval", C.c_double), ("qss_stability_check", C.c_int),
                 ("rtol", C.c_double), ("atol_momentum", C.c_double), ("atol_energy", C.c_double),
                 ("atol_species", C.c_double), ("min_step", C.c_double), ("max_steps", C.c_int),
-                ("t_start", C.c_double)]
+                ("t_start", C.c_double), ("ignition_t_start", C.c_double),
+                ("ignition_duration", C.c_double), ("ignition_energy", C.c_double),
+                ("ignition_center", C.c_double), ("ignition_stddev", C.c_double),
+                ("positions", C.c_void_p)]
 
 
 def build() -> str:
