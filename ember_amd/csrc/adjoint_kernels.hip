@@ -175,7 +175,7 @@ __device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, d
                 if (rev) roprb += A.wDb[p0] + A.wDb[p1] + A.wDb[p2];
                 const double PCr = C[r0] * C[r1] * C[r2];
                 const double PCp = C[p0] * C[p1] * C[p2];
-                const double Mgv = (kind != kElementary) ? L.Mg[info_group(w)] : 1.0;
+                const double Mgv = (kind >= kThreeBody && kind <= kTroe) ? L.Mg[info_group(w)] : 1.0;
                 // k_f as the forward computed it
                 double kf = R.kf[j];
                 double klow = 0.0, khigh = 0.0, work = 0.0, pr0 = 0.0, F = 1.0, f1 = 0.0, u = 0.0,

@@ -155,6 +155,11 @@ int upload_mech(emx_context* c) {
     d.lnRPref = std::log(kGasConstant / m.Pref);
     std::vector<double> kf_c, kf_s, kc_c, kc_s;
     if (d.interp) build_interp_tables(m, L, c->P, kf_c, kf_s, kc_c, kc_s);
+    std::vector<int32_t> aux;
+    std::vector<double> auxd;
+    build_pressure_tables(m, L, c->P, aux, auxd);
+    d.has_pdep = 0;
+    for (const auto& r : m.rxn) d.has_pdep |= (r.type == 4 || r.type == 5);
     Packer pk;
     pk.add(m.W, &d.W);
     pk.add(L.rW, &d.rW);
@@ -189,6 +194,8 @@ int upload_mech(emx_context* c) {
     pk.add(L.sp_grp, &d.sp_grp);
     pk.add(L.sp_grp_eff, &d.sp_grp_eff);
     pk.add(L.chunk_sp, &d.chunk_sp);
+    pk.add(aux, &d.aux);
+    pk.add(auxd, &d.auxd);
     pk.add(kf_c, &d.kf_c);
     pk.add(kf_s, &d.kf_s);
     pk.add(kc_c, &d.kc_c);
@@ -440,9 +447,9 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
     if (!state_in || !state_out) return fail(-1, "state pointers are NULL");
     if (c->variant == EMX_VARIANT_PLAIN) {
         if (c->opts.chemistry_integrator != EMX_QSS || c->opts.qss_stability_check || c->ratefn_n ||
-            c->opts.ignition_energy != 0.0)
+            c->opts.ignition_energy != 0.0 || c->dm.has_pdep)
             return fail(-1, "the plain baseline kernel implements QSS without stability check, "
-                            "rate multiplier function or igniter only");
+                            "rate multiplier function, igniter or pressure-dependent rates only");
         HIP_TRY(c->plain_ws.ensure(plain_workspace_doubles(c->dm, (int)nPoints) * sizeof(double)));
         int rc = launch_qss_plain(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
                                   (int)nPoints, state_out, status, stat1, stat2,
@@ -507,6 +514,7 @@ static int adjoint_check(emx_context* c, size_t nPoints, size_t nVars) {
         return fail(-1, "the adjoint's ln-multiplier parameters need per-reaction multipliers; "
                         "remove the rate multiplier function");
     if (c->opts.ignition_energy != 0.0) return fail(-1, "the adjoint does not support the igniter");
+    if (c->dm.has_pdep) return fail(-1, "the adjoint does not support PLOG or Chebyshev reactions");
     return 0;
 }
 

@@ -152,6 +152,36 @@ __device__ __forceinline__ double igniter(const DevOpts& o, double amp, double t
     return (ta >= o.ign_t0 && ta < o.ign_t1) ? amp : 0.0;
 }
 
+// k_f of a PLOG or Chebyshev reaction slot at the context pressure (tables from
+// build_pressure_tables): Cantera's PlogRate (log-linear interpolation in ln P between the
+// bracketing levels, each level's rates summed) and ChebyshevRate (log10 k as a Chebyshev
+// series in the reduced inverse temperature, the pressure sums precomputed).
+__device__ inline double pdep_kf(const DevMech& m, int s, int kind, double logT, double recipT) {
+    const double* d = m.auxd + m.aux[s];
+    if (kind == kPlog) {
+        const int n1 = (int)d[1], n2 = (int)d[2];
+        auto level_log = [&](const double* q, int n) {  // q: (A, ln A, b, Ea/R) per rate
+            if (n == 1) return q[1] + q[2] * logT - q[3] * recipT;
+            double k = 1e-300;
+            for (int i = 0; i < n; i++) k += q[4 * i] * emx_exp(q[4 * i + 2] * logT - q[4 * i + 3] * recipT, m.exp2t);
+            return emx_log(k);
+        };
+        const double l1 = level_log(d + 3, n1);
+        const double l2 = level_log(d + 3 + 4 * n1, n2);
+        return emx_exp(l1 + (l2 - l1) * d[0], m.exp2t);
+    }
+    const int nT = (int)d[0];
+    const double Tr = (2 * recipT + d[1]) * d[2];
+    double Cnm1 = Tr, Cn = 1.0, logk = 0.0;
+    for (int i = 0; i < nT; i++) {
+        const double Cnp1 = 2 * Tr * Cn - Cnm1;
+        logk += Cn * d[3 + i];
+        Cnm1 = Cn;
+        Cn = Cnp1;
+    }
+    return emx_exp(logk * kLn10, m.exp2t);  // 10^logk
+}
+
 // T-dependent reaction data held in registers (one set per owned slot).
 // NR = compile-time bound on reaction rounds (>= DevMech::nRounds).
 template <int NR>
@@ -260,6 +290,12 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
                 size_t o = n * stride + s;
                 R.rkc[j] = m.kc_c[o] + m.kc_s[o] * dT;
                 R.kf[j] = m.kf_c[o] + m.kf_s[o] * dT;
+                // PLOG and Chebyshev rates keep their own evaluation (only Arrhenius rates
+                // are interpolated, src/chemistry0d.cpp:264-269)
+                if (m.has_pdep) {
+                    const int kind = info_kind(m.info[s]);
+                    if (kind >= kPlog) R.kf[j] = pdep_kf(m, s, kind, emx_log(T), 1.0 / T);
+                }
             }
         }
     } else {
@@ -283,6 +319,7 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
                 if (kind != kNone) {
                     if (info_rev(w)) rkc = fmin(inv_kc(m.reac[s], m.prod[s], L), kBigNumber);
                     if (kind <= kThreeBody) kf = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT, m.exp2t);
+                    else if (kind >= kPlog) kf = pdep_kf(m, s, kind, logT, recipT);
                 }
                 R.rkc[j] = rkc;
                 R.kf[j] = kf;

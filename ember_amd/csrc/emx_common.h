@@ -22,8 +22,10 @@ constexpr double kBigNumber = 1e300;
 constexpr double kLn10 = 2.302585092994045684;  // ln 10: Troe's 10^x evaluated as exp(x ln 10)
 constexpr double kInvLn10 = 0.43429448190325182765;  // 1 / ln 10: log10(x) = ln(x) / ln 10
 
-// reaction kinds in device order (falloff first, then three-body, then elementary)
-enum RxnKind : int { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3, kNone = 4 };
+// reaction kinds in device order (falloff first, then three-body, then elementary with the
+// pressure-dependent PLOG and Chebyshev rates)
+enum RxnKind : int { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3, kNone = 4,
+                     kPlog = 5, kCheb = 6 };
 
 // info word of a reaction slot
 __host__ __device__ inline int info_kind(int32_t w) { return w & 7; }
@@ -102,6 +104,11 @@ struct DevMech {
     const int32_t* sp_grp;
     const double* sp_grp_eff;
     const int32_t* chunk_sp;      // [nChunkRounds * 64]
+    // pressure-dependent rates (PLOG, Chebyshev) at the context pressure: per slot offset
+    // into auxd (-1: none); layouts in build_pressure_tables (mech_host.h)
+    const int32_t* aux;
+    const double* auxd;
+    int has_pdep;  // 1 if the mechanism has PLOG or Chebyshev reactions
     // interp tables [kInterpN][nRounds*64]
     const double* kf_c;
     const double* kf_s;

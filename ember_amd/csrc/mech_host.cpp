@@ -58,7 +58,32 @@ std::string read_table(const std::string& path, HostMech& m) {
                 if (fscanf(f, "%d %lf", &k, &e) != 2) { err = "bad efficiencies"; break; }
                 r.eff.emplace_back(k, e);
             }
+            if (r.type < 0 || r.type > 5) err = "unknown reaction type";
             m.rxn.push_back(r);
+        } else if (!strcmp(tag, "PLOG")) {
+            // the PLOG rates of the preceding reaction
+            int n = 0;
+            if (m.rxn.empty() || m.rxn.back().type != 4 || fscanf(f, "%d", &n) != 1 || n < 1) {
+                err = "bad PLOG record";
+                break;
+            }
+            std::vector<double>& v = m.rxn.back().plog;
+            v.resize(4 * (size_t)n);
+            for (int j = 0; j < 4 * n && err.empty(); j++)
+                if (fscanf(f, "%lf", &v[j]) != 1) err = "bad PLOG data";
+            for (int j = 1; j < n && err.empty(); j++)
+                if (v[4 * j] < v[4 * (j - 1)]) err = "PLOG pressures must not decrease";
+        } else if (!strcmp(tag, "CHEB")) {
+            HostRxn* r = m.rxn.empty() ? nullptr : &m.rxn.back();
+            if (!r || r->type != 5 ||
+                fscanf(f, "%d %d %lf %lf %lf %lf", &r->chebT, &r->chebP, &r->chebTmin, &r->chebTmax,
+                       &r->chebPmin, &r->chebPmax) != 6 || r->chebT < 1 || r->chebP < 1) {
+                err = "bad CHEB record";
+                break;
+            }
+            r->cheb.resize((size_t)r->chebT * r->chebP);
+            for (auto& c : r->cheb)
+                if (err.empty() && fscanf(f, "%lf", &c) != 1) err = "bad CHEB data";
         } else {
             err = std::string("unknown record ") + tag;
         }
@@ -83,11 +108,11 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
     for (int i = 0; i < m.I; i++) order[i] = i;
     auto rank = [&](int i) {
         int t = m.rxn[i].type;
-        return t == 3 ? 0 : t == 2 ? 1 : t == 1 ? 2 : 3;
+        return t == 3 ? 0 : t == 2 ? 1 : t == 1 ? 2 : 3;  // PLOG, Chebyshev with elementary
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rank(a) < rank(b); });
     int nFall = 0;
-    for (int i = 0; i < m.I; i++) nFall += (m.rxn[i].type >= 2);
+    for (int i = 0; i < m.I; i++) nFall += (m.rxn[i].type == 2 || m.rxn[i].type == 3);
     L.nRounds = (m.I + 63) / 64;
     L.nFallRounds = (nFall + 63) / 64;
     if (L.nRounds > kMaxRounds) return "mechanism has more than 512 reactions";
@@ -119,7 +144,7 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
         L.reac[s] = pack_stoich(r.nr, r.r);
         L.prod[s] = pack_stoich(r.np, r.p);
         int group = 255;
-        if (r.type != 0) {
+        if (r.type >= 1 && r.type <= 3) {
             std::vector<std::pair<int, double>> e;
             for (auto& pe : r.eff) e.emplace_back(pe.first, pe.second - r.defeff);
             auto key = std::make_pair(r.defeff, e);
@@ -132,7 +157,8 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
                 group = it->second;
             }
         }
-        int kind = r.type == 3 ? kTroe : r.type == 2 ? kLindemann : r.type == 1 ? kThreeBody : kElementary;
+        int kind = r.type == 3 ? kTroe : r.type == 2 ? kLindemann : r.type == 1 ? kThreeBody
+                 : r.type == 4 ? kPlog : r.type == 5 ? kCheb : kElementary;
         L.info[s] = kind | (r.rev ? 8 : 0) | (r.hasT2 ? 16 : 0) | (group << 8);
     }
     if ((int)glist.size() > 64) return "more than 64 distinct third-body efficiency sets";
@@ -230,6 +256,71 @@ static void host_nasa(const HostMech& m, int k, double T, double& cpR, double& h
     cpR = ct0 + ct1 + ct2 + ct3 + ct4;
     hRT = ct0 + 0.5 * ct1 + 1.0 / 3.0 * ct2 + 0.25 * ct3 + 0.2 * ct4 + c[5] * (1.0 / T);
     sR = ct0 * std::log(T) + ct1 + 0.5 * ct2 + 1.0 / 3.0 * ct3 + 0.25 * ct4 + c[6];
+}
+
+void build_pressure_tables(const HostMech& m, const DevLayout& L, double P, std::vector<int32_t>& aux,
+                           std::vector<double>& auxd) {
+    const int S = L.nRounds * 64;
+    aux.assign(S, -1);
+    auxd.clear();
+    const double logP = std::log(P), log10P = std::log10(P);
+    for (int s = 0; s < m.I; s++) {
+        const HostRxn& r = m.rxn[L.orig[s]];
+        if (r.type == 4) {
+            // PlogRate at fixed P: the levels (distinct pressures) bracketing P, the end
+            // level on both sides outside the listed range
+            const int n = (int)r.plog.size() / 4;
+            std::vector<int> first;  // index of the first rate of each level
+            for (int j = 0; j < n; j++)
+                if (j == 0 || r.plog[4 * j] != r.plog[4 * (j - 1)]) first.push_back(j);
+            const int nl = (int)first.size();
+            auto level_end = [&](int l) { return l + 1 < nl ? first[l + 1] : n; };
+            int lo = 0, hi = 0;
+            double wP = 0.0;
+            if (logP >= std::log(r.plog[4 * first[nl - 1]])) {
+                lo = hi = nl - 1;
+            } else if (logP >= std::log(r.plog[0])) {
+                hi = 0;
+                while (std::log(r.plog[4 * first[hi]]) <= logP) hi++;  // upper_bound
+                lo = hi - 1;
+                const double logP1 = std::log(r.plog[4 * first[lo]]), logP2 = std::log(r.plog[4 * first[hi]]);
+                const double rDeltaP = 1.0 / (logP2 - logP1);
+                wP = (logP - logP1) * rDeltaP;
+            }
+            aux[s] = (int32_t)auxd.size();
+            const int n1 = level_end(lo) - first[lo], n2 = level_end(hi) - first[hi];
+            auxd.push_back(wP);
+            auxd.push_back(n1);
+            auxd.push_back(n2);
+            for (int l : {lo, hi})
+                for (int j = first[l]; j < level_end(l); j++) {
+                    const double A = r.plog[4 * j + 1];
+                    auxd.push_back(A);
+                    auxd.push_back(A > 0 ? std::log(A) : -INFINITY);
+                    auxd.push_back(r.plog[4 * j + 2]);
+                    auxd.push_back(r.plog[4 * j + 3]);
+                }
+        } else if (r.type == 5) {
+            // ChebyshevRate at fixed P: the pressure sums d_t = sum_p a_tp T_p(Pr)
+            const double PrNum = -std::log10(r.chebPmin) - std::log10(r.chebPmax);
+            const double PrDen = 1.0 / (std::log10(r.chebPmax) - std::log10(r.chebPmin));
+            const double Pr = (2 * log10P + PrNum) * PrDen;
+            std::vector<double> d(r.chebT, 0.0);
+            double Cnm1 = Pr, Cn = 1.0;
+            for (int p = 0; p < r.chebP; p++) {
+                const double Cnp1 = 2 * Pr * Cn - Cnm1;
+                for (int t = 0; t < r.chebT; t++) d[t] += Cn * r.cheb[(size_t)t * r.chebP + p];
+                Cnm1 = Cn;
+                Cn = Cnp1;
+            }
+            aux[s] = (int32_t)auxd.size();
+            auxd.push_back(r.chebT);
+            auxd.push_back(-1.0 / r.chebTmin - 1.0 / r.chebTmax);
+            auxd.push_back(1.0 / (1.0 / r.chebTmax - 1.0 / r.chebTmin));
+            auxd.insert(auxd.end(), d.begin(), d.end());
+        }
+    }
+    if (auxd.empty()) auxd.push_back(0.0);
 }
 
 void build_interp_tables(const HostMech& m, const DevLayout& L, double P, std::vector<double>& kf_c,

@@ -18,6 +18,12 @@ struct HostRxn {
     int hasT2 = 0;
     double defeff = 1.0;
     std::vector<std::pair<int, double>> eff;
+    // type 4 (PLOG): (P [Pa], A, b, Ea/R) per rate, in file order
+    std::vector<double> plog;
+    // type 5 (Chebyshev): nT x nP coefficients of log10 k, row-major, and the ranges
+    int chebT = 0, chebP = 0;
+    double chebTmin = 0, chebTmax = 0, chebPmin = 0, chebPmax = 0;
+    std::vector<double> cheb;
 };
 
 struct HostMech {
@@ -50,6 +56,15 @@ struct DevLayout {
 };
 
 std::string build_layout(const HostMech& m, DevLayout& L);
+
+// Pressure-dependent rates at the context pressure P, per slot: aux[s] = offset into
+// auxd (-1: none).  PLOG (Cantera PlogRate at fixed P): [wP, n1, n2, (A, ln A, b, Ea/R) x
+// (n1 + n2)] for the rates at the bracketing pressures P1 <= P < P2 (the end level
+// twice outside the listed range), wP = (ln P - ln P1) / (ln P2 - ln P1) (0 for one
+// level).  Chebyshev (ChebyshevRate at fixed P): [nT, -1/Tmin - 1/Tmax,
+// 1 / (1/Tmax - 1/Tmin), d_0 .. d_{nT-1}] with d_t = sum_p a_tp T_p(Pr).
+void build_pressure_tables(const HostMech& m, const DevLayout& L, double P, std::vector<int32_t>& aux,
+                           std::vector<double>& auxd);
 
 // InterpKinetics / MultiArrheniusInterp tables at pressure P, [kInterpN][nRounds*64]
 void build_interp_tables(const HostMech& m, const DevLayout& L, double P, std::vector<double>& kf_c,

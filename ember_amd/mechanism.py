@@ -5,7 +5,8 @@ src/chemistry0d.cpp:455-485) and Cantera parses it.  Cantera is not part of
 this build, so this module restates the subset of Cantera's YAML reader that
 the hot path needs: ideal-gas phases, NASA7 two-range thermo, elementary,
 three-body (with efficiencies) and falloff (Lindemann / Troe) reactions,
-reversible and irreversible, duplicates.
+pressure-dependent Arrhenius (PLOG) and Chebyshev reactions, reversible and
+irreversible, duplicates.
 
 The output is a plain-text table file (``.emxm``) that the C-ABI library reads
 in ``emx_create``.  All quantities are in SI/kmol units, matching Cantera's
@@ -33,7 +34,11 @@ CAL = 4.184
 ATOMIC_WEIGHTS = {"H": 1.008, "C": 12.011, "N": 14.007, "O": 15.999,
                   "Ar": 39.95, "He": 4.002602}
 
-TYPE_ELEMENTARY, TYPE_THREE_BODY, TYPE_LINDEMANN, TYPE_TROE = 0, 1, 2, 3
+TYPE_ELEMENTARY, TYPE_THREE_BODY, TYPE_LINDEMANN, TYPE_TROE, TYPE_PLOG, TYPE_CHEBYSHEV = 0, 1, 2, 3, 4, 5
+
+# pressure units of Cantera YAML quantities, in Pa
+PRESSURE_UNITS = {"Pa": 1.0, "kPa": 1e3, "MPa": 1e6, "bar": 1e5, "atm": ONE_ATM,
+                  "torr": ONE_ATM / 760.0, "Torr": ONE_ATM / 760.0, "dyn/cm^2": 0.1}
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
@@ -56,6 +61,12 @@ class Reaction:
     default_eff: float = 1.0
     efficiencies: Dict[int, float] = field(default_factory=dict)
     duplicate: bool = False
+    # PLOG: (P [Pa], A [SI kmol units], b, Ea/R [K]) per rate, in file order
+    plog: List[Tuple[float, float, float, float]] = field(default_factory=list)
+    # Chebyshev: (Tmin, Tmax, Pmin [Pa], Pmax [Pa]) and the coefficients of log10 k
+    # (nT x nP, k in SI kmol units)
+    cheb_range: Tuple[float, float, float, float] = (0.0, 0.0, 0.0, 0.0)
+    cheb: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -134,6 +145,13 @@ class Mechanism:
             for k, e in eff:
                 line += f" {k} {e!r}"
             out.append(line)
+            if r.rtype == TYPE_PLOG:
+                out.append(f"PLOG {len(r.plog)} " +
+                           " ".join(repr(float(v)) for lvl in r.plog for v in lvl))
+            elif r.rtype == TYPE_CHEBYSHEV:
+                nT, nP = r.cheb.shape
+                out.append(f"CHEB {nT} {nP} " + " ".join(repr(float(v)) for v in r.cheb_range) + " " +
+                           " ".join(repr(float(v)) for v in r.cheb.ravel()))
         out.append("END")
         tmp = path + ".tmp"
         with open(tmp, "w") as f:
@@ -176,6 +194,14 @@ def _parse_side(text: str, names: List[str]) -> Tuple[List[int], bool, bool]:
     for sp in sorted(counts):
         idx += [names.index(sp)] * counts[sp]
     return idx, has_M, falloff
+
+
+def _pressure(v, units: dict) -> float:
+    """A Cantera YAML pressure: a number in the file's pressure unit, or 'value unit'."""
+    if isinstance(v, str):
+        num, unit = v.split()
+        return float(num) * PRESSURE_UNITS[unit]
+    return float(v) * PRESSURE_UNITS[units.get("pressure", "Pa")]
 
 
 def _unit_factors(units: dict) -> Tuple[float, float]:
@@ -277,6 +303,24 @@ def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
             k = rd["rate-constant"]
             r = Reaction(eq, TYPE_ELEMENTARY, rev, R, P,
                          float(k["A"]) * conc_unit ** (order - 1), float(k["b"]), ea_R(k["Ea"]))
+        elif rtype_s == "pressure-dependent-Arrhenius":
+            # Cantera PlogRate: Arrhenius expressions at listed pressures
+            r = Reaction(eq, TYPE_PLOG, rev, R, P, 0.0, 0.0, 0.0)
+            for k in rd["rate-constants"]:
+                r.plog.append((_pressure(k["P"], units), float(k["A"]) * conc_unit ** (order - 1),
+                               float(k["b"]), ea_R(k["Ea"])))
+        elif rtype_s == "Chebyshev":
+            # Cantera ChebyshevRate: log10 k = sum_tp a_tp T_t(Tr) T_p(Pr); the unit
+            # conversion of k enters as log10 of its factor on a_00
+            r = Reaction(eq, TYPE_CHEBYSHEV, rev, R, P, 0.0, 0.0, 0.0)
+            Tmin, Tmax = (float(v) for v in rd["temperature-range"])
+            Pmin, Pmax = (_pressure(v, units) for v in rd["pressure-range"])
+            r.cheb_range = (Tmin, Tmax, Pmin, Pmax)
+            c = np.array(rd["data"], dtype=float)
+            if c.ndim != 2:
+                raise ValueError(f"Chebyshev data must be a matrix: {eq}")
+            c[0, 0] += np.log10(conc_unit ** (order - 1))
+            r.cheb = c
         else:
             raise ValueError(f"unsupported reaction type {rtype_s!r}: {eq}")
         if r.rtype != TYPE_ELEMENTARY:

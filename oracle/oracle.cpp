@@ -48,6 +48,12 @@ struct Rxn {
     int hasT2 = 0;
     double defeff = 1.0;
     std::vector<std::pair<int, double>> eff;
+    // type 4, pressure-dependent Arrhenius (Cantera PlogRate): (P, A, b, Ea/R) per rate
+    std::vector<double> plog;
+    // type 5, Cantera ChebyshevRate: nT x nP coefficients of log10 k (row-major)
+    int chebT = 0, chebP = 0;
+    double chebTmin = 0, chebTmax = 0, chebPmin = 0, chebPmax = 0;
+    std::vector<double> cheb;
 };
 
 struct Mech {
@@ -114,7 +120,7 @@ void build_stoich(Mech& m) {
     m.tb_slot.assign(m.I, -1);
     for (int i = 0; i < m.I; i++) {
         m.dn[i] = double(m.rxn[i].np) - double(m.rxn[i].nr);
-        if (m.rxn[i].type != 0) {
+        if (m.rxn[i].type >= 1 && m.rxn[i].type <= 3) {  // three-body and falloff
             m.tb_slot[i] = (int)m.thirdbody.size();
             m.thirdbody.push_back(i);
         }
@@ -162,6 +168,18 @@ Mech* load_mech(const char* path) {
                 r.eff.emplace_back(k, e);
             }
             m->rxn.push_back(r);
+        } else if (!strcmp(tag, "PLOG")) {
+            int n = 0;
+            ok = !m->rxn.empty() && m->rxn.back().type == 4 && fscanf(f, "%d", &n) == 1 && n >= 1;
+            if (ok) m->rxn.back().plog.resize(4 * (size_t)n);
+            for (int j = 0; j < 4 * n && ok; j++) ok = fscanf(f, "%lf", &m->rxn.back().plog[j]) == 1;
+        } else if (!strcmp(tag, "CHEB")) {
+            Rxn* r = m->rxn.empty() ? nullptr : &m->rxn.back();
+            ok = r && r->type == 5 &&
+                 fscanf(f, "%d %d %lf %lf %lf %lf", &r->chebT, &r->chebP, &r->chebTmin, &r->chebTmax,
+                        &r->chebPmin, &r->chebPmax) == 6 && r->chebT >= 1 && r->chebP >= 1;
+            if (ok) r->cheb.resize((size_t)r->chebT * r->chebP);
+            for (size_t j = 0; ok && j < r->cheb.size(); j++) ok = fscanf(f, "%lf", &r->cheb[j]) == 1;
         } else {
             ok = false;
         }
@@ -298,6 +316,54 @@ struct Gas {
         return A * std::exp(b * logT - EaR * recipT);
     }
 
+    // Cantera PlogRate at pressure P: ln k interpolated linearly in ln P between the rates
+    // of the two listed pressures that bracket P (P1 <= P < P2); below the first or from
+    // the last pressure on, that level alone.  A level's rates are summed (one rate: its
+    // logarithm directly).
+    static double plog_kf(const Rxn& r, double P, double logT, double recipT) {
+        const int n = (int)r.plog.size() / 4;
+        std::vector<int> lev;  // first rate of each distinct pressure
+        for (int j = 0; j < n; j++)
+            if (j == 0 || r.plog[4 * j] != r.plog[4 * (j - 1)]) lev.push_back(j);
+        lev.push_back(n);
+        const int nl = (int)lev.size() - 1;
+        auto level_log = [&](int l) {
+            const int a = lev[l], e = lev[l + 1];
+            if (e - a == 1) return std::log(r.plog[4 * a + 1]) + r.plog[4 * a + 2] * logT - r.plog[4 * a + 3] * recipT;
+            double k = 1e-300;
+            for (int j = a; j < e; j++) k += arrhenius(r.plog[4 * j + 1], r.plog[4 * j + 2], r.plog[4 * j + 3], logT, recipT);
+            return std::log(k);
+        };
+        const double logP = std::log(P);
+        if (logP < std::log(r.plog[0])) return std::exp(level_log(0));
+        if (logP >= std::log(r.plog[4 * lev[nl - 1]])) return std::exp(level_log(nl - 1));
+        int hi = 0;
+        while (std::log(r.plog[4 * lev[hi]]) <= logP) hi++;
+        const double logP1 = std::log(r.plog[4 * lev[hi - 1]]), logP2 = std::log(r.plog[4 * lev[hi]]);
+        const double k1 = level_log(hi - 1), k2 = level_log(hi);
+        return std::exp(k1 + (k2 - k1) * ((logP - logP1) * (1.0 / (logP2 - logP1))));
+    }
+
+    // Cantera ChebyshevRate: log10 k = sum_t sum_p a_tp T_t(Tr) T_p(Pr) with
+    // Tr = (2/T - 1/Tmin - 1/Tmax) / (1/Tmax - 1/Tmin) and
+    // Pr = (2 log10 P - log10 Pmin - log10 Pmax) / (log10 Pmax - log10 Pmin)
+    static double cheb_kf(const Rxn& r, double P, double recipT) {
+        const double Pr = (2 * std::log10(P) - std::log10(r.chebPmin) - std::log10(r.chebPmax)) /
+                          (std::log10(r.chebPmax) - std::log10(r.chebPmin));
+        const double Tr = (2 * recipT - 1.0 / r.chebTmin - 1.0 / r.chebTmax) / (1.0 / r.chebTmax - 1.0 / r.chebTmin);
+        std::vector<double> Tt(r.chebT), Tp(r.chebP);
+        for (int i = 0; i < r.chebT; i++) Tt[i] = i == 0 ? 1.0 : i == 1 ? Tr : 2 * Tr * Tt[i - 1] - Tt[i - 2];
+        for (int i = 0; i < r.chebP; i++) Tp[i] = i == 0 ? 1.0 : i == 1 ? Pr : 2 * Pr * Tp[i - 1] - Tp[i - 2];
+        double logk = 0.0;
+        for (int t = 0; t < r.chebT; t++)
+            for (int p = 0; p < r.chebP; p++) logk += r.cheb[(size_t)t * r.chebP + p] * Tt[t] * Tp[p];
+        return std::pow(10.0, logk);
+    }
+
+    double pdep_kf(const Rxn& r, double logT, double recipT) const {
+        return r.type == 4 ? plog_kf(r, P, logT, recipT) : cheb_kf(r, P, recipT);
+    }
+
     // InterpKinetics / MultiArrheniusInterp tables over [TminInt, TmaxInt], frozen.
     void build_interp() {
         if (m.interp_ready && m.interp_P == P) return;
@@ -339,10 +405,13 @@ struct Gas {
             if (T < m.TminInt || T >= m.TmaxInt) return false;
             size_t n = (size_t)std::floor((T - m.TminInt) / (m.TmaxInt - m.TminInt) * (m.nInterp - 1));
             double dT = T - n * (m.TmaxInt - m.TminInt) / (m.nInterp - 1) - m.TminInt;
+            const double logT = std::log(T), recipT = 1.0 / T;
             for (int i = 0; i < m.I; i++) {
                 rkcn[i] = m.kc_const[n * m.I + i] + m.kc_slope[n * m.I + i] * dT;
                 if (m.rxn[i].type <= 1)
                     kf0[i] = m.kf_const[n * m.I + i] + m.kf_slope[n * m.I + i] * dT;
+                else if (m.rxn[i].type >= 4)  // only Arrhenius rates are interpolated
+                    kf0[i] = pdep_kf(m.rxn[i], logT, recipT);
             }
         } else {
             rkc_at(rkcn.data());
@@ -350,6 +419,7 @@ struct Gas {
             for (int i = 0; i < m.I; i++) {
                 const Rxn& r = m.rxn[i];
                 if (r.type <= 1) kf0[i] = arrhenius(r.A, r.b, r.EaR, logT, recipT);
+                else if (r.type >= 4) kf0[i] = pdep_kf(r, logT, recipT);
             }
         }
         rateT = T;
@@ -372,7 +442,7 @@ struct Gas {
         double logT = std::log(T), recipT = 1.0 / T;
         for (int i = 0; i < m.I; i++) {
             const Rxn& r = m.rxn[i];
-            if (r.type < 2) continue;
+            if (r.type != 2 && r.type != 3) continue;
             double work = 0.0;
             if (r.type == 3) {
                 double rt3 = 1.0 / r.T3, rt1 = 1.0 / r.T1;
@@ -1416,6 +1486,8 @@ int orc_qss_adjoint(void* mech, const orc_options* o, int nPoints, const double*
     if (!adjoint_options_ok(o)) return -6;
     if (!m.ratefn.empty()) return -8;  // the ln-multiplier parameters need per-reaction multipliers
     if (o->ignition_energy != 0.0) return -9;
+    for (const Rxn& r : m.rxn)
+        if (r.type >= 4) return -10;  // PLOG / Chebyshev: no adjoint
     const int N = m.K + 2;
     prepare_interp(m, o);
 #ifdef _OPENMP
@@ -1470,6 +1542,8 @@ int orc_qss_fd_gradient(void* mech, const orc_options* o, int nPoints, const dou
     if (!adjoint_options_ok(o)) return -6;
     if (!m.ratefn.empty()) return -8;  // the ln-multiplier parameters need per-reaction multipliers
     if (o->ignition_energy != 0.0) return -9;
+    for (const Rxn& r : m.rxn)
+        if (r.type >= 4) return -10;  // PLOG / Chebyshev: no adjoint
     const int N = m.K + 2;
     prepare_interp(m, o);
 #ifdef _OPENMP
