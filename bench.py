@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 # counter's reading for a plain copy of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0):
 # profiles/qss_gri30_n4096_stage.md (stage kernel, 4096 GRI points) and
 # profiles/diffusion_k53_n65536.md (diff_step + pcr kernels, one solve of 53 x 65536).
-STAGE_TRAFFIC = 1501 * 2 * 1024 + 2354 * 1024
+STAGE_TRAFFIC = 1555 * 2 * 1024 + 2355 * 1024
 DIFFUSION_TRAFFIC_PER_SOLVE = (48650 + 5220) * 2 * 1024 + (42200 + 1696) * 1024
 
 
