@@ -700,22 +700,24 @@ __global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const dou
 }
 
 // Separators of one variable per workgroup of kPcrThreads threads, from the reduced
-// tridiagonal system of the chunk summaries (C rows, C <= kPcrMax).  The rows are read
-// coalesced into LDS; each thread owns a run of G = ceil(C / kPcrThreads) <= 16
-// consecutive rows and eliminates its interior towards both ends (the same
-// downward/upward elimination as the chunks); the <= kPcrThreads run separators are
-// solved by parallel cyclic reduction (each level couples row i with rows i -/+ stride and
-// doubles the stride; two buffers, one barrier per level), and each thread back-substitutes
-// its run.  Few long runs keep the O(S log S) reduction small: the workgroup is the only
-// one of its variable, so its VALU work is the critical path.
-constexpr int kPcrThreads = 256;
-constexpr int kRunMax = kPcrMax / kPcrThreads;
+// tridiagonal system of the chunk summaries (C rows, C <= kPcrMax).  All threads read
+// the rows coalesced into LDS (8 waves keep twice the loads of 4 in flight); then each of
+// the first kPcrRuns threads owns a run of G = ceil(C / kPcrRuns) <= 16 consecutive rows
+// and eliminates its interior towards both ends (the same downward/upward elimination as
+// the chunks); the <= kPcrRuns run separators are solved by parallel cyclic reduction
+// (each level couples row i with rows i -/+ stride and doubles the stride; two buffers,
+// one barrier per level), and each run thread back-substitutes its run.  Few long runs
+// keep the O(S log S) reduction small: the workgroup is the only one of its variable, so
+// its latency is the critical path.
+constexpr int kPcrThreads = 512;
+constexpr int kPcrRuns = 256;
+constexpr int kRunMax = kPcrMax / kPcrRuns;
 
 // LDS index of reduced row i: one pad double per 16 rows, so that the rows of consecutive
 // runs (16 apart) fall into different banks
 __host__ __device__ __forceinline__ int pcr_slot(int i) { return i + (i >> 4); }
 
-size_t pcr_lds_doubles(int C) { return 4 * (size_t)pcr_slot(C) + 2 * 4 * kPcrThreads; }
+size_t pcr_lds_doubles(int C) { return 4 * (size_t)pcr_slot(C) + 2 * 4 * kPcrRuns; }
 
 __global__ void __launch_bounds__(kPcrThreads) pcr_kernel(const double* own, const double* nxt, int C,
                                                           int nV, const int* nsteps, int q,
@@ -728,8 +730,8 @@ __global__ void __launch_bounds__(kPcrThreads) pcr_kernel(const double* own, con
     double* b = a + CS;
     double* c = b + CS;
     double* r = c + CS;
-    double* sm = r + CS;  // separator system: buffer p, array k at sm + (4 p + k) * kPcrThreads
-    auto sep = [&](int p, int k) { return sm + (4 * p + k) * kPcrThreads; };
+    double* sm = r + CS;  // separator system: buffer p, array k at sm + (4 p + k) * kPcrRuns
+    auto sep = [&](int p, int k) { return sm + (4 * p + k) * kPcrRuns; };
     const size_t pl = (size_t)nV * C;
     const double* ov = own + (size_t)v * C;
     const double* nv = nxt + (size_t)v * C;
@@ -737,7 +739,7 @@ __global__ void __launch_bounds__(kPcrThreads) pcr_kernel(const double* own, con
     const int tid = threadIdx.x;
     // reduced row i: own part of chunk i plus the interior contribution of chunk i + 1
     // (batches of kLoadBatch rows per thread, all loads of a batch in flight together)
-    constexpr int kLoadBatch = 16;
+    constexpr int kLoadBatch = kPcrMax / kPcrThreads;
     for (int kb = 0; kb * kPcrThreads < C; kb += kLoadBatch) {
         double la[kLoadBatch], lb[kLoadBatch], lc[kLoadBatch], lr[kLoadBatch];
 #pragma unroll
@@ -761,7 +763,7 @@ __global__ void __launch_bounds__(kPcrThreads) pcr_kernel(const double* own, con
             }
         }
     }
-    const int G = (C + kPcrThreads - 1) / kPcrThreads;  // rows per run
+    const int G = (C + kPcrRuns - 1) / kPcrRuns;  // rows per run
     const int S = (C + G - 1) / G;                      // runs (separator rows)
     const int s0 = tid * G;
     const int e0 = min(s0 + G, C) - 1;  // separator of this run
