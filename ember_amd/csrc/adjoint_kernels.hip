@@ -180,7 +180,7 @@ __device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, d
                 double kf = R.kf[j];
                 double klow = 0.0, khigh = 0.0, work = 0.0, pr0 = 0.0, F = 1.0, f1 = 0.0, u = 0.0,
                        nn = 0.0;
-                const bool fall = j < RateRegs<NR>::NF && (kind == kLindemann || kind == kTroe);
+                const bool fall = j < RateRegs<NR>::NF && (kind == kLindemann || kind == kTroe);  // SRI: refused
                 if (fall) {
                     klow = R.klow[j];
                     khigh = R.khigh[j];

@@ -214,6 +214,13 @@ int check_ctx(emx_context* c) {
     return 0;
 }
 
+// PLOG, Chebyshev or SRI reactions (no adjoint, no plain baseline kernel for them)
+bool has_kinds_beyond_troe(const HostMech& m) {
+    for (const auto& r : m.rxn)
+        if (r.type >= 4) return true;
+    return false;
+}
+
 // per-slot multipliers for the production kernels: ones while the function is set
 const double* stage_perturb(const emx_context* c) {
     return c->ratefn_n > 0 ? c->ones.as<double>() : c->perturb;
@@ -447,9 +454,9 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
     if (!state_in || !state_out) return fail(-1, "state pointers are NULL");
     if (c->variant == EMX_VARIANT_PLAIN) {
         if (c->opts.chemistry_integrator != EMX_QSS || c->opts.qss_stability_check || c->ratefn_n ||
-            c->opts.ignition_energy != 0.0 || c->dm.has_pdep)
+            c->opts.ignition_energy != 0.0 || has_kinds_beyond_troe(c->hm))
             return fail(-1, "the plain baseline kernel implements QSS without stability check, "
-                            "rate multiplier function, igniter or pressure-dependent rates only");
+                            "rate multiplier function, igniter, PLOG, Chebyshev or SRI rates only");
         HIP_TRY(c->plain_ws.ensure(plain_workspace_doubles(c->dm, (int)nPoints) * sizeof(double)));
         int rc = launch_qss_plain(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
                                   (int)nPoints, state_out, status, stat1, stat2,
@@ -514,7 +521,8 @@ static int adjoint_check(emx_context* c, size_t nPoints, size_t nVars) {
         return fail(-1, "the adjoint's ln-multiplier parameters need per-reaction multipliers; "
                         "remove the rate multiplier function");
     if (c->opts.ignition_energy != 0.0) return fail(-1, "the adjoint does not support the igniter");
-    if (c->dm.has_pdep) return fail(-1, "the adjoint does not support PLOG or Chebyshev reactions");
+    if (has_kinds_beyond_troe(c->hm))
+        return fail(-1, "the adjoint does not support PLOG, Chebyshev or SRI reactions");
     return 0;
 }
 

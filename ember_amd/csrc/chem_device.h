@@ -191,7 +191,8 @@ struct RateRegs {
     double rkc[NR];
     double klow[NF];
     double khigh[NF];
-    double work[NF];  // log10(Fcent) for Troe
+    double work[NF];   // log10(Fcent) for Troe; log10 of a exp(-b/T) + exp(-T/c) for SRI
+    double work2[NF];  // d T^e for SRI
     double T;         // temperature these were computed at (-1: none)
 };
 
@@ -278,6 +279,15 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
     cpR = 0.0;
     if (sp) nasa(m, lane - 2, T, logT, cpR, hRT, sR);
     if (R.T == T) return true;
+    // PLOG and Chebyshev k_f into the forward-rate LDS row (free until rates_of_progress),
+    // in one rolled loop so that their code exists once instead of once per round
+    if (m.has_pdep) {
+        for (int j = 0; j < m.nRounds; j++) {
+            const int s = j * 64 + lane;
+            const int kind = info_kind(m.info[s]);
+            if (kind == kPlog || kind == kCheb) L.ropf[s] = pdep_kf(m, s, kind, logT, recipT);
+        }
+    }
     if (INTERP) {
         if (!(T >= kInterpTmin && T < kInterpTmax)) return false;
         size_t n = (size_t)floor((T - kInterpTmin) / (kInterpTmax - kInterpTmin) * (kInterpN - 1));
@@ -294,7 +304,7 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
                 // are interpolated, src/chemistry0d.cpp:264-269)
                 if (m.has_pdep) {
                     const int kind = info_kind(m.info[s]);
-                    if (kind >= kPlog) R.kf[j] = pdep_kf(m, s, kind, emx_log(T), 1.0 / T);
+                    if (kind == kPlog || kind == kCheb) R.kf[j] = L.ropf[s];
                 }
             }
         }
@@ -319,32 +329,39 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
                 if (kind != kNone) {
                     if (info_rev(w)) rkc = fmin(inv_kc(m.reac[s], m.prod[s], L), kBigNumber);
                     if (kind <= kThreeBody) kf = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT, m.exp2t);
-                    else if (kind >= kPlog) kf = pdep_kf(m, s, kind, logT, recipT);
+                    else if (kind == kPlog || kind == kCheb) kf = L.ropf[s];
                 }
                 R.rkc[j] = rkc;
                 R.kf[j] = kf;
             }
         }
     }
-    // falloff: k_low, k_high and log10(Fcent) (FalloffRate/TroeRate::updateTemp)
+    // falloff: k_low, k_high and the T-dependent parts of F (FalloffRate, TroeRate and
+    // SriRate::updateTemp)
 #pragma unroll
     for (int j = 0; j < RateRegs<NR>::NF; j++) {
         if (j < m.nFallRounds) {
             int s = j * 64 + lane;
             int kind = info_kind(m.info[s]);
-            double klow = 0.0, khigh = 0.0, work = 0.0;
-            if (kind == kLindemann || kind == kTroe) {
+            double klow = 0.0, khigh = 0.0, work = 0.0, work2 = 0.0;
+            if (kind == kLindemann || kind == kTroe || kind == kSri) {
                 klow = arrhenius(m.A0[s], m.b0[s], m.EaR0[s], logT, recipT, m.exp2t);
                 khigh = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT, m.exp2t);
                 if (kind == kTroe) {
                     double Fcent = (1.0 - m.troeA[s]) * emx_exp(-T * m.rT3[s], m.exp2t) + m.troeA[s] * emx_exp(-T * m.rT1[s], m.exp2t);
                     if (m.T2[s] != 0.0) Fcent += emx_exp(-m.T2[s] / T, m.exp2t);
                     work = emx_log(fmax(Fcent, kSmallNumber)) * kInvLn10;
+                } else if (kind == kSri) {
+                    const double* q = m.auxd + m.aux[s];  // a, b, c, d, e
+                    const double base = q[0] * emx_exp(-q[1] * recipT, m.exp2t) + emx_exp(-T / q[2], m.exp2t);
+                    work = emx_log(fmax(base, kSmallNumber)) * kInvLn10;
+                    work2 = q[4] != 0.0 ? q[3] * emx_exp(q[4] * logT, m.exp2t) : q[3];
                 }
             }
             R.klow[j] = klow;
             R.khigh[j] = khigh;
             R.work[j] = work;
+            R.work2[j] = work2;
         }
     }
     R.T = T;
@@ -380,7 +397,7 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
             double ropf = 0.0, ropr = 0.0;
             if (kind != kNone) {
                 double kf = R.kf[j];
-                if (j < RateRegs<NR>::NF && (kind == kLindemann || kind == kTroe)) {
+                if (j < RateRegs<NR>::NF && (kind == kLindemann || kind == kTroe || kind == kSri)) {
                     double khigh = R.khigh[j];
                     double pr = L.Mg[info_group(w)] * R.klow[j] / (khigh + kSmallNumber);
                     double F = 1.0;
@@ -392,6 +409,11 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
                         double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
                         double lf = work / (1.0 + f1 * f1);
                         F = emx_exp(kLn10 * lf, m.exp2t);
+                    } else if (kind == kSri) {
+                        // SriRate: F = d T^e [a exp(-b/T) + exp(-T/c)]^X, X = 1/(1 + log10^2 Pr)
+                        const double lpr = emx_log(fmax(pr, kSmallNumber)) * kInvLn10;
+                        const double X = 1.0 / (1.0 + lpr * lpr);
+                        F = R.work2[j] * emx_exp(kLn10 * (X * R.work[j]), m.exp2t);
                     }
                     pr *= F / (1.0 + pr);
                     kf = pr * khigh;

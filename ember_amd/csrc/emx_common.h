@@ -25,7 +25,7 @@ constexpr double kInvLn10 = 0.43429448190325182765;  // 1 / ln 10: log10(x) = ln
 // reaction kinds in device order (falloff first, then three-body, then elementary with the
 // pressure-dependent PLOG and Chebyshev rates)
 enum RxnKind : int { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3, kNone = 4,
-                     kPlog = 5, kCheb = 6 };
+                     kPlog = 5, kCheb = 6, kSri = 7 };
 
 // info word of a reaction slot
 __host__ __device__ inline int info_kind(int32_t w) { return w & 7; }

@@ -58,7 +58,7 @@ std::string read_table(const std::string& path, HostMech& m) {
                 if (fscanf(f, "%d %lf", &k, &e) != 2) { err = "bad efficiencies"; break; }
                 r.eff.emplace_back(k, e);
             }
-            if (r.type < 0 || r.type > 5) err = "unknown reaction type";
+            if (r.type < 0 || r.type > 6) err = "unknown reaction type";
             m.rxn.push_back(r);
         } else if (!strcmp(tag, "PLOG")) {
             // the PLOG rates of the preceding reaction
@@ -73,6 +73,13 @@ std::string read_table(const std::string& path, HostMech& m) {
                 if (fscanf(f, "%lf", &v[j]) != 1) err = "bad PLOG data";
             for (int j = 1; j < n && err.empty(); j++)
                 if (v[4 * j] < v[4 * (j - 1)]) err = "PLOG pressures must not decrease";
+        } else if (!strcmp(tag, "SRI")) {
+            HostRxn* r = m.rxn.empty() ? nullptr : &m.rxn.back();
+            if (!r || r->type != 6 ||
+                fscanf(f, "%lf %lf %lf %lf %lf", &r->sri[0], &r->sri[1], &r->sri[2], &r->sri[3], &r->sri[4]) != 5) {
+                err = "bad SRI record";
+                break;
+            }
         } else if (!strcmp(tag, "CHEB")) {
             HostRxn* r = m.rxn.empty() ? nullptr : &m.rxn.back();
             if (!r || r->type != 5 ||
@@ -108,11 +115,11 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
     for (int i = 0; i < m.I; i++) order[i] = i;
     auto rank = [&](int i) {
         int t = m.rxn[i].type;
-        return t == 3 ? 0 : t == 2 ? 1 : t == 1 ? 2 : 3;  // PLOG, Chebyshev with elementary
+        return t == 3 ? 0 : (t == 2 || t == 6) ? 1 : t == 1 ? 2 : 3;  // PLOG, Chebyshev with elementary
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rank(a) < rank(b); });
     int nFall = 0;
-    for (int i = 0; i < m.I; i++) nFall += (m.rxn[i].type == 2 || m.rxn[i].type == 3);
+    for (int i = 0; i < m.I; i++) nFall += (m.rxn[i].type == 2 || m.rxn[i].type == 3 || m.rxn[i].type == 6);
     L.nRounds = (m.I + 63) / 64;
     L.nFallRounds = (nFall + 63) / 64;
     if (L.nRounds > kMaxRounds) return "mechanism has more than 512 reactions";
@@ -144,7 +151,7 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
         L.reac[s] = pack_stoich(r.nr, r.r);
         L.prod[s] = pack_stoich(r.np, r.p);
         int group = 255;
-        if (r.type >= 1 && r.type <= 3) {
+        if ((r.type >= 1 && r.type <= 3) || r.type == 6) {
             std::vector<std::pair<int, double>> e;
             for (auto& pe : r.eff) e.emplace_back(pe.first, pe.second - r.defeff);
             auto key = std::make_pair(r.defeff, e);
@@ -158,7 +165,7 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
             }
         }
         int kind = r.type == 3 ? kTroe : r.type == 2 ? kLindemann : r.type == 1 ? kThreeBody
-                 : r.type == 4 ? kPlog : r.type == 5 ? kCheb : kElementary;
+                 : r.type == 4 ? kPlog : r.type == 5 ? kCheb : r.type == 6 ? kSri : kElementary;
         L.info[s] = kind | (r.rev ? 8 : 0) | (r.hasT2 ? 16 : 0) | (group << 8);
     }
     if ((int)glist.size() > 64) return "more than 64 distinct third-body efficiency sets";
@@ -318,6 +325,9 @@ void build_pressure_tables(const HostMech& m, const DevLayout& L, double P, std:
             auxd.push_back(-1.0 / r.chebTmin - 1.0 / r.chebTmax);
             auxd.push_back(1.0 / (1.0 / r.chebTmax - 1.0 / r.chebTmin));
             auxd.insert(auxd.end(), d.begin(), d.end());
+        } else if (r.type == 6) {
+            aux[s] = (int32_t)auxd.size();
+            auxd.insert(auxd.end(), r.sri, r.sri + 5);
         }
     }
     if (auxd.empty()) auxd.push_back(0.0);

@@ -24,6 +24,8 @@ struct HostRxn {
     int chebT = 0, chebP = 0;
     double chebTmin = 0, chebTmax = 0, chebPmin = 0, chebPmax = 0;
     std::vector<double> cheb;
+    // type 6 (SRI falloff): a, b, c, d, e
+    double sri[5] = {0, 0, 0, 1, 0};
 };
 
 struct HostMech {
@@ -62,7 +64,8 @@ std::string build_layout(const HostMech& m, DevLayout& L);
 // (n1 + n2)] for the rates at the bracketing pressures P1 <= P < P2 (the end level
 // twice outside the listed range), wP = (ln P - ln P1) / (ln P2 - ln P1) (0 for one
 // level).  Chebyshev (ChebyshevRate at fixed P): [nT, -1/Tmin - 1/Tmax,
-// 1 / (1/Tmax - 1/Tmin), d_0 .. d_{nT-1}] with d_t = sum_p a_tp T_p(Pr).
+// 1 / (1/Tmax - 1/Tmin), d_0 .. d_{nT-1}] with d_t = sum_p a_tp T_p(Pr).  SRI falloff:
+// [a, b, c, d, e].
 void build_pressure_tables(const HostMech& m, const DevLayout& L, double P, std::vector<int32_t>& aux,
                            std::vector<double>& auxd);
 

@@ -5,8 +5,8 @@ src/chemistry0d.cpp:455-485) and Cantera parses it.  Cantera is not part of
 this build, so this module restates the subset of Cantera's YAML reader that
 the hot path needs: ideal-gas phases, NASA7 two-range thermo, elementary,
 three-body (with efficiencies) and falloff (Lindemann / Troe) reactions,
-pressure-dependent Arrhenius (PLOG) and Chebyshev reactions, reversible and
-irreversible, duplicates.
+pressure-dependent Arrhenius (PLOG), Chebyshev and SRI falloff reactions, reversible
+and irreversible, duplicates.
 
 The output is a plain-text table file (``.emxm``) that the C-ABI library reads
 in ``emx_create``.  All quantities are in SI/kmol units, matching Cantera's
@@ -34,7 +34,8 @@ CAL = 4.184
 ATOMIC_WEIGHTS = {"H": 1.008, "C": 12.011, "N": 14.007, "O": 15.999,
                   "Ar": 39.95, "He": 4.002602}
 
-TYPE_ELEMENTARY, TYPE_THREE_BODY, TYPE_LINDEMANN, TYPE_TROE, TYPE_PLOG, TYPE_CHEBYSHEV = 0, 1, 2, 3, 4, 5
+TYPE_ELEMENTARY, TYPE_THREE_BODY, TYPE_LINDEMANN, TYPE_TROE, TYPE_PLOG, TYPE_CHEBYSHEV, TYPE_SRI = \
+    0, 1, 2, 3, 4, 5, 6
 
 # pressure units of Cantera YAML quantities, in Pa
 PRESSURE_UNITS = {"Pa": 1.0, "kPa": 1e3, "MPa": 1e6, "bar": 1e5, "atm": ONE_ATM,
@@ -67,6 +68,8 @@ class Reaction:
     # (nT x nP, k in SI kmol units)
     cheb_range: Tuple[float, float, float, float] = (0.0, 0.0, 0.0, 0.0)
     cheb: Optional[np.ndarray] = None
+    # SRI falloff: (a, b, c, d, e)
+    sri: Tuple[float, float, float, float, float] = (0.0, 0.0, 0.0, 1.0, 0.0)
 
 
 @dataclass
@@ -148,6 +151,8 @@ class Mechanism:
             if r.rtype == TYPE_PLOG:
                 out.append(f"PLOG {len(r.plog)} " +
                            " ".join(repr(float(v)) for lvl in r.plog for v in lvl))
+            elif r.rtype == TYPE_SRI:
+                out.append("SRI " + " ".join(repr(float(v)) for v in r.sri))
             elif r.rtype == TYPE_CHEBYSHEV:
                 nT, nP = r.cheb.shape
                 out.append(f"CHEB {nT} {nP} " + " ".join(repr(float(v)) for v in r.cheb_range) + " " +
@@ -291,7 +296,7 @@ def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
         elif rtype_s == "falloff" or fR:
             hiP = rd["high-P-rate-constant"]
             loP = rd["low-P-rate-constant"]
-            rt = TYPE_TROE if "Troe" in rd else TYPE_LINDEMANN
+            rt = TYPE_TROE if "Troe" in rd else TYPE_SRI if "SRI" in rd else TYPE_LINDEMANN
             r = Reaction(eq, rt, rev, R, P,
                          float(hiP["A"]) * conc_unit ** (order - 1), float(hiP["b"]), ea_R(hiP["Ea"]),
                          float(loP["A"]) * conc_unit ** order, float(loP["b"]), ea_R(loP["Ea"]))
@@ -299,6 +304,10 @@ def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
                 t = rd["Troe"]
                 r.troe = (float(t["A"]), float(t["T3"]), float(t["T1"]), float(t.get("T2", 0.0)))
                 r.has_T2 = "T2" in t
+            if "SRI" in rd:
+                t = rd["SRI"]
+                r.sri = (float(t["A"]), float(t["B"]), float(t["C"]), float(t.get("D", 1.0)),
+                         float(t.get("E", 0.0)))
         elif rtype_s == "elementary":
             k = rd["rate-constant"]
             r = Reaction(eq, TYPE_ELEMENTARY, rev, R, P,

@@ -54,6 +54,8 @@ struct Rxn {
     int chebT = 0, chebP = 0;
     double chebTmin = 0, chebTmax = 0, chebPmin = 0, chebPmax = 0;
     std::vector<double> cheb;
+    // type 6, Cantera SriRate falloff: a, b, c, d, e
+    double sri[5] = {0, 0, 0, 1, 0};
 };
 
 struct Mech {
@@ -120,7 +122,7 @@ void build_stoich(Mech& m) {
     m.tb_slot.assign(m.I, -1);
     for (int i = 0; i < m.I; i++) {
         m.dn[i] = double(m.rxn[i].np) - double(m.rxn[i].nr);
-        if (m.rxn[i].type >= 1 && m.rxn[i].type <= 3) {  // three-body and falloff
+        if ((m.rxn[i].type >= 1 && m.rxn[i].type <= 3) || m.rxn[i].type == 6) {  // three-body and falloff
             m.tb_slot[i] = (int)m.thirdbody.size();
             m.thirdbody.push_back(i);
         }
@@ -173,6 +175,10 @@ Mech* load_mech(const char* path) {
             ok = !m->rxn.empty() && m->rxn.back().type == 4 && fscanf(f, "%d", &n) == 1 && n >= 1;
             if (ok) m->rxn.back().plog.resize(4 * (size_t)n);
             for (int j = 0; j < 4 * n && ok; j++) ok = fscanf(f, "%lf", &m->rxn.back().plog[j]) == 1;
+        } else if (!strcmp(tag, "SRI")) {
+            Rxn* r = m->rxn.empty() ? nullptr : &m->rxn.back();
+            ok = r && r->type == 6 &&
+                 fscanf(f, "%lf %lf %lf %lf %lf", &r->sri[0], &r->sri[1], &r->sri[2], &r->sri[3], &r->sri[4]) == 5;
         } else if (!strcmp(tag, "CHEB")) {
             Rxn* r = m->rxn.empty() ? nullptr : &m->rxn.back();
             ok = r && r->type == 5 &&
@@ -410,7 +416,7 @@ struct Gas {
                 rkcn[i] = m.kc_const[n * m.I + i] + m.kc_slope[n * m.I + i] * dT;
                 if (m.rxn[i].type <= 1)
                     kf0[i] = m.kf_const[n * m.I + i] + m.kf_slope[n * m.I + i] * dT;
-                else if (m.rxn[i].type >= 4)  // only Arrhenius rates are interpolated
+                else if (m.rxn[i].type == 4 || m.rxn[i].type == 5)  // only Arrhenius rates are interpolated
                     kf0[i] = pdep_kf(m.rxn[i], logT, recipT);
             }
         } else {
@@ -419,7 +425,7 @@ struct Gas {
             for (int i = 0; i < m.I; i++) {
                 const Rxn& r = m.rxn[i];
                 if (r.type <= 1) kf0[i] = arrhenius(r.A, r.b, r.EaR, logT, recipT);
-                else if (r.type >= 4) kf0[i] = pdep_kf(r, logT, recipT);
+                else if (r.type == 4 || r.type == 5) kf0[i] = pdep_kf(r, logT, recipT);
             }
         }
         rateT = T;
@@ -442,7 +448,7 @@ struct Gas {
         double logT = std::log(T), recipT = 1.0 / T;
         for (int i = 0; i < m.I; i++) {
             const Rxn& r = m.rxn[i];
-            if (r.type != 2 && r.type != 3) continue;
+            if (r.type != 2 && r.type != 3 && r.type != 6) continue;
             double work = 0.0;
             if (r.type == 3) {
                 double rt3 = 1.0 / r.T3, rt1 = 1.0 / r.T1;
@@ -461,6 +467,12 @@ struct Gas {
                 double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
                 double lf = work / (1.0 + f1 * f1);
                 F = std::pow(10.0, lf);
+            } else if (r.type == 6) {
+                // SriRate: F = d T^e [a exp(-b/T) + exp(-T/c)]^X, X = 1 / (1 + log10^2 Pr)
+                const double base = r.sri[0] * std::exp(-r.sri[1] / T) + std::exp(-T / r.sri[2]);
+                const double lpr = std::log10(std::max(pr, kSmallNumber));
+                const double X = 1.0 / (1.0 + lpr * lpr);
+                F = r.sri[3] * std::pow(T, r.sri[4]) * std::pow(10.0, X * std::log10(std::max(base, kSmallNumber)));
             }
             pr *= F / (1.0 + pr);
             kf0[i] = pr * khigh;
@@ -1487,7 +1499,7 @@ int orc_qss_adjoint(void* mech, const orc_options* o, int nPoints, const double*
     if (!m.ratefn.empty()) return -8;  // the ln-multiplier parameters need per-reaction multipliers
     if (o->ignition_energy != 0.0) return -9;
     for (const Rxn& r : m.rxn)
-        if (r.type >= 4) return -10;  // PLOG / Chebyshev: no adjoint
+        if (r.type >= 4) return -10;  // PLOG / Chebyshev / SRI: no adjoint
     const int N = m.K + 2;
     prepare_interp(m, o);
 #ifdef _OPENMP
@@ -1543,7 +1555,7 @@ int orc_qss_fd_gradient(void* mech, const orc_options* o, int nPoints, const dou
     if (!m.ratefn.empty()) return -8;  // the ln-multiplier parameters need per-reaction multipliers
     if (o->ignition_energy != 0.0) return -9;
     for (const Rxn& r : m.rxn)
-        if (r.type >= 4) return -10;  // PLOG / Chebyshev: no adjoint
+        if (r.type >= 4) return -10;  // PLOG / Chebyshev / SRI: no adjoint
     const int N = m.K + 2;
     prepare_interp(m, o);
 #ifdef _OPENMP

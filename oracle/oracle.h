@@ -110,8 +110,8 @@ int orc_diffusion_integrate(int nVars, int N, const double* B, int nB, const int
  * step sequence frozen: grad_y0 (nPoints x nVars) w.r.t. the initial state and grad_lnk
  * (nPoints x nReactions) w.r.t. the logs of the rate multipliers.  Qbar NULL = ones,
  * ybar_final NULL = zeros.  status as orc_source_integrate.  -6: unsupported options,
- * -8: a rate multiplier function is set, -9: the igniter is on, -10: the mechanism has PLOG
- * or Chebyshev reactions. */
+ * -8: a rate multiplier function is set, -9: the igniter is on, -10: the mechanism has PLOG,
+ * Chebyshev or SRI reactions. */
 int orc_qss_adjoint(void* mech, const orc_options* o, int nPoints, const double* state_in,
                     const double* split_const, const double* qloss, double dt, const double* Qbar,
                     const double* ybar_final, double* state_out, double* Q, double* grad_y0,

@@ -1,6 +1,6 @@
-"""PLOG and Chebyshev reactions on the GPU against the oracle, on the synthetic test
-mechanism tests/golden/h2o2_pdep.yaml, at pressures below, between, on and above the PLOG
-levels.  Bars: rates within 1e-10 relative; stages within the production-stage contract,
+"""PLOG, Chebyshev and SRI-falloff reactions on the GPU against the oracle, on the
+synthetic test mechanism tests/golden/h2o2_pdep.yaml, at pressures below, between, on and
+above the PLOG levels.  Bars: rates within 1e-10 relative; stages within the production-stage contract,
 |dT| <= 1e-6 T and |dY| <= 1e-6 |Y| + 1e-12."""
 import os
 

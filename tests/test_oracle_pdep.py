@@ -1,12 +1,14 @@
-"""Pressure-dependent rates in the oracle (CPU): Cantera's PlogRate and ChebyshevRate, on
-the synthetic test mechanism tests/golden/h2o2_pdep.yaml (H + O2 <=> O + OH as PLOG with
-levels at 0.1, 1 (two rates) and 10 atm; H + HO2 <=> 2 OH as Chebyshev).
+"""Pressure-dependent rates in the oracle (CPU): Cantera's PlogRate, ChebyshevRate and
+SriRate, on the synthetic test mechanism tests/golden/h2o2_pdep.yaml (H + O2 <=> O + OH as
+PLOG with levels at 0.1, 1 (two rates) and 10 atm; H + HO2 <=> 2 OH as Chebyshev;
+H + O2 (+M) <=> HO2 (+M) as SRI falloff).
 
 Pins (k_f recovered from the oracle's forward rate of progress, k_f = ropf / (C_H C_X)):
 - PLOG at a listed pressure equals that level's Arrhenius sum; between levels ln k is
   linear in ln P; below the first and above the last level the end level applies;
 - Chebyshev equals numpy's 2-D Chebyshev series of (Tr, Pr) as defined by Cantera;
-- InterpKinetics interpolates only Arrhenius rates: both stay exact with kinetics=interp.
+- InterpKinetics interpolates only Arrhenius rates: both stay exact with kinetics=interp;
+- SRI falloff equals k_inf Pr/(1 + Pr) F with Cantera's SRI F.
 """
 import os
 
@@ -95,3 +97,26 @@ def test_adjoint_refuses_pdep(oracle_mod, om, mech):
     st[0, 2:] = 1.0 / mech.n_species
     with pytest.raises(RuntimeError):
         om.adjoint(st, 1e-6, opts=oracle_mod.options())
+
+
+def test_sri_falloff(oracle_mod, om, mech):
+    """SRI falloff (Cantera SriRate): k = k_inf Pr/(1 + Pr) F with Pr = k0 [M] / k_inf and
+    F = d T^e [a exp(-b/T) + exp(-T/c)]^X, X = 1/(1 + log10^2 Pr); [M] from the
+    reaction's efficiencies."""
+    i = next(j for j, r in enumerate(mech.reactions) if r.rtype == mechanism.TYPE_SRI)
+    r = mech.reactions[i]
+    a, b, c, d, e = r.sri
+    Y = np.full(mech.n_species, 1.0 / mech.n_species)
+    for T in (600.0, 1300.0, 2500.0):
+        for P in (0.1 * ATM, ATM, 30.0 * ATM):
+            out = om.rates(np.array([T]), Y[None, :], oracle_mod.options(pressure=P))
+            C = out["rho"][0] * Y / mech.W
+            M = sum(C[k] * r.efficiencies.get(k, r.default_eff) for k in range(mech.n_species))
+            kinf = _arr(r.A, r.b, r.EaR, T)
+            pr = _arr(r.A0, r.b0, r.EaR0, T) * M / kinf
+            X = 1.0 / (1.0 + np.log10(pr) ** 2)
+            F = d * T**e * (a * np.exp(-b / T) + np.exp(-T / c)) ** X
+            ref = kinf * pr / (1.0 + pr) * F
+            got = out["ropf"][0, i] / np.prod(C[r.reactants])
+            assert got == pytest.approx(ref, rel=1e-12)
+            assert abs(F - 1.0) > 1e-3  # the SRI factor matters here
