@@ -24,11 +24,16 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+// A failed HIP call is reported here; the runtime's sticky last-error is cleared so that
+// the launch checks of later calls (hipGetLastError after a launch) report their own
+// launches only.
 #define HIP_TRY(expr)                                                                   \
     do {                                                                                \
         hipError_t e_ = (expr);                                                         \
-        if (e_ != hipSuccess)                                                           \
+        if (e_ != hipSuccess) {                                                         \
+            (void)hipGetLastError();                                                    \
             return fail(-10, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+        }                                                                               \
     } while (0)
 
 // growable device buffer
@@ -607,6 +612,7 @@ int emx_source_integrate(emx_context* c, size_t nPoints, size_t nVars, const dou
     if (int rc = check_ctx(c)) return rc;
     if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
     if (nPoints == 0) return 0;
+    if (!state_in || !state_out) return fail(-1, "state pointers are NULL");
     const size_t bytes = nPoints * nVars * sizeof(double);
     HIP_TRY(c->b_in.ensure(bytes));
     HIP_TRY(c->b_out.ensure(bytes));
@@ -644,6 +650,7 @@ int emx_rates(emx_context* c, size_t n, const double* T, const double* Y, int no
               double* wD, double* wnet, int32_t* status) {
     if (int rc = check_ctx(c)) return rc;
     if (n == 0) return 0;
+    if (!T || !Y) return fail(-1, "T or Y is NULL");
     const int K = c->hm.K;
     DevBuf dT, dY, dQ, dD, dN, dS;
     auto cleanup = [&]() { dT.release(); dY.release(); dQ.release(); dD.release(); dN.release(); dS.release(); };
@@ -736,6 +743,7 @@ int emx_diffusion_integrate_device(emx_context* c, size_t nVars, size_t nPoints,
                                    const double* h_per_var, double t0, double t1, double* state) {
     if (int rc = check_diff(c, nVars, nPoints, g)) return rc;
     if (!B_row && nB != nVars) return fail(-1, "B_row is NULL but nB != nVars");
+    if (!B || !D || !state) return fail(-1, "B, D or state is NULL");
     DiffArgs a{B, B_row, D, split_const, g->x, (int)nPoints, (int)nVars, g->alpha, g->left_bc,
                g->right_bc, g->y_inf, g->wall_const, g->wall_var};
     return diffusion_run(c, a, h_per_var, t0, t1, state);
@@ -747,6 +755,7 @@ int emx_diffusion_integrate(emx_context* c, size_t nVars, size_t nPoints, const 
                             double* state) {
     if (int rc = check_diff(c, nVars, nPoints, g)) return rc;
     if (!B_row && nB != nVars) return fail(-1, "B_row is NULL but nB != nVars");
+    if (!B || !D || !state) return fail(-1, "B, D or state is NULL");
     if (B_row)
         for (size_t v = 0; v < nVars; v++)
             if (B_row[v] < 0 || (size_t)B_row[v] >= nB) return fail(-1, "B_row entry out of range");
@@ -889,7 +898,7 @@ int emx_tridiagonal_integrate(emx_context* c, size_t nSys, size_t N, const doubl
                               double t1, double* y) {
     if (int rc = check_ctx(c)) return rc;
     if (N < 2 || nSys < 1 || nSys > 64) return fail(-1, "need N >= 2 and 1 <= nSys <= 64");
-    if (!h_in) return fail(-1, "h is NULL");
+    if (!h_in || !a || !b || !cc || !k || !y) return fail(-1, "an array argument is NULL");
     hipStream_t st = c->stream;
     const size_t bytes = nSys * N * sizeof(double);
     HIP_TRY(c->t_a.ensure(bytes));
