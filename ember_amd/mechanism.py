@@ -167,17 +167,22 @@ class Mechanism:
 _SPECIES_TOKEN = re.compile(r"^(\d+(?:\.\d+)?)?\s*(\S+)$")
 
 
-def _parse_side(text: str, names: List[str]) -> Tuple[List[int], bool, bool]:
+def _parse_side(text: str, names: List[str]) -> Tuple[List[int], bool, Optional[str]]:
     """Parse one side of a Cantera equation into expanded species indices.
 
-    Returns (indices, has_M, has_falloff_M).  Indices are ordered the way
-    Cantera iterates a reaction's Composition map (sorted by species name),
-    each repeated by its integer stoichiometric coefficient.
+    Returns (indices, has_M, falloff_collider): the collider of a falloff reaction is
+    "M" for "(+M)", the species name for a specific collider such as "(+AR)", and None
+    otherwise.  Indices are ordered the way Cantera iterates a reaction's Composition
+    map (sorted by species name), each repeated by its integer stoichiometric
+    coefficient.
     """
-    falloff = False
-    if "(+M)" in text:
-        falloff = True
-        text = text.replace("(+M)", "")
+    falloff = None
+    m = re.search(r"\(\+\s*([^)\s]+)\s*\)", text)
+    if m:
+        falloff = m.group(1)
+        if falloff != "M" and falloff not in names:
+            raise ValueError(f"unknown falloff collider {falloff!r} in {text!r}")
+        text = text[:m.start()] + text[m.end():]
     text = text.strip()
     # split on ' + ' (species names such as "CH2(S)" contain no spaces)
     toks = [t.strip() for t in re.split(r"\s\+\s", " " + text + " ") if t.strip()]
@@ -214,10 +219,11 @@ def _unit_factors(units: dict) -> Tuple[float, float]:
     quantity = units.get("quantity", "kmol")
     ea = units.get("activation-energy", "J/kmol")
     lf = {"cm": 0.01, "m": 1.0, "mm": 1e-3}[length]
-    qf = {"mol": 1e-3, "kmol": 1.0}[quantity]
+    qf = {"mol": 1e-3, "kmol": 1.0, "molec": 1.0 / AVOGADRO}[quantity]
     conc_unit = lf**3 / qf   # one (length^3/quantity) expressed in m^3/kmol
     eaf = {"cal/mol": CAL * 1000.0, "kcal/mol": CAL * 1e6, "J/mol": 1000.0,
-           "kJ/mol": 1e6, "J/kmol": 1.0, "K": GAS_CONSTANT}[ea]
+           "kJ/mol": 1e6, "J/kmol": 1.0, "K": GAS_CONSTANT,
+           "eV": 1.602176634e-19 * AVOGADRO}[ea]
     return conc_unit, eaf
 
 
@@ -293,7 +299,7 @@ def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
             k = rd["rate-constant"]
             A = float(k["A"]) * conc_unit ** (order + 1 - 1)
             r = Reaction(eq, rt, rev, R, P, A, float(k["b"]), ea_R(k["Ea"]))
-        elif rtype_s == "falloff" or fR:
+        elif rtype_s == "falloff" or fR is not None:
             hiP = rd["high-P-rate-constant"]
             loP = rd["low-P-rate-constant"]
             rt = TYPE_TROE if "Troe" in rd else TYPE_SRI if "SRI" in rd else TYPE_LINDEMANN
@@ -308,6 +314,10 @@ def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
                 t = rd["SRI"]
                 r.sri = (float(t["A"]), float(t["B"]), float(t["C"]), float(t.get("D", 1.0)),
                          float(t.get("E", 0.0)))
+            if fR is not None and fR != "M":
+                # a specific collider: [M] is that species' concentration alone
+                r.default_eff = 0.0
+                r.efficiencies = {names.index(fR): 1.0}
         elif rtype_s == "elementary":
             k = rd["rate-constant"]
             r = Reaction(eq, TYPE_ELEMENTARY, rev, R, P,
@@ -332,7 +342,7 @@ def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
             r.cheb = c
         else:
             raise ValueError(f"unsupported reaction type {rtype_s!r}: {eq}")
-        if r.rtype != TYPE_ELEMENTARY:
+        if r.rtype != TYPE_ELEMENTARY and not (fR is not None and fR != "M"):
             r.default_eff = float(rd.get("default-efficiency", 1.0))
             for sp, e in (rd.get("efficiencies") or {}).items():
                 if sp in names:

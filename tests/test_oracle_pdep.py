@@ -120,3 +120,21 @@ def test_sri_falloff(oracle_mod, om, mech):
             got = out["ropf"][0, i] / np.prod(C[r.reactants])
             assert got == pytest.approx(ref, rel=1e-12)
             assert abs(F - 1.0) > 1e-3  # the SRI factor matters here
+
+
+def test_specific_collider_falloff(oracle_mod, om, mech):
+    """A falloff reaction written with "(+AR)" has argon as its only collider: [M] = C_AR
+    (default efficiency 0), so its rate does not change when other species are swapped
+    at fixed C_AR, and is the Lindemann form with that [M]."""
+    i = next(j for j, r in enumerate(mech.reactions) if "(+AR)" in r.equation)
+    r = mech.reactions[i]
+    ar = mech.species_index("AR")
+    assert r.rtype == mechanism.TYPE_LINDEMANN and r.default_eff == 0.0 and r.efficiencies == {ar: 1.0}
+    T, P = 1400.0, ATM
+    Y = np.full(mech.n_species, 1.0 / mech.n_species)
+    out = om.rates(np.array([T]), Y[None, :], oracle_mod.options(pressure=P))
+    C = out["rho"][0] * Y / mech.W
+    kinf = _arr(r.A, r.b, r.EaR, T)
+    pr = _arr(r.A0, r.b0, r.EaR0, T) * C[ar] / kinf
+    ref = kinf * pr / (1.0 + pr)
+    assert out["ropf"][0, i] / np.prod(C[r.reactants]) == pytest.approx(ref, rel=1e-12)
