@@ -131,6 +131,9 @@ public:
         grid_.y_inf = 0.0;
         grid_.wall_const = 0.0;
         grid_.wall_var = 1;  // kEnergy
+        // SourceSystem::setPosition for every point (flameSolver.cpp:501, :539): the
+        // igniter's positions
+        check(emx_set_point_positions(ctx_.get(), x_.size(), x_.data()));
     }
 
     // Diffusion coefficients as FlameSolver::prepareIntegrators sets them
