@@ -112,8 +112,9 @@ struct ThermoBar {
 //  qb, db: adjoints of the outputs q, d (this lane); qdotb: adjoint of qDot (uniform).
 //  yb accumulates the composition adjoint (species lanes); thetab[j] the ln-multiplier
 //  adjoint of slot j*64 + lane.
+//  Always inlined, as qss_integrate: outlined, its LDS reads become flat loads.
 template <int NR, bool INTERP>
-__device__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, double Tn,
+__device__ __forceinline__ bool rhs_vjp(const DevMech& m, int lane, double yy, bool corrector, double Tn,
                         double rho, double cp, double hk, double wQ, double wD, double qdot,
                         double qb, double db, double qdotb, double qloss, ThermoBar& thb,
                         double& Trb, double& yb, double (&thetab)[NR], RateRegs<NR>& R,

@@ -119,8 +119,10 @@ constexpr double kNegInf = -__builtin_huge_val();
 // src/qssintegrator.cpp:98-265 line by line.
 // EXT: a rate multiplier function (o.ratefn) or the igniter (o.qign) is on; without them
 // the multiplier factor 1 and the igniter term compile away.
+// Always inlined: in a called function the LDS pointers lose their address space (every
+// staged-table access becomes a flat load) and the rate-constant registers go to the stack.
 template <int NR, bool INTERP, class Hooks, bool EXT = false>
-__device__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool active,
+__device__ __forceinline__ int qss_integrate(const DevMech& m, const DevOpts& o, int lane, bool active,
                              QssLane& Z, QssSystem& S, RateRegs<NR>& R, const double* perturb,
                              const WaveLds& L, double tf, QssResume start, Hooks& H, int& gcount,
                              int& rcount, double& qint) {
