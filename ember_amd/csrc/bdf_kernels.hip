@@ -36,8 +36,10 @@ struct BdfSystem {
 };
 
 // SourceSystemCVODE::f for this lane's variable.  Returns false if rates failed.
+// Every device function below is forced inline: outlined, its LDS pointers lose their
+// address space (flat loads) and the rate-constant registers go to the stack.
 template <int NR, bool INTERP>
-__device__ inline bool bdf_f(const DevMech& m, int lane, double yv, BdfSystem& S, RateRegs<NR>& R,
+__device__ __forceinline__ bool bdf_f(const DevMech& m, int lane, double yv, BdfSystem& S, RateRegs<NR>& R,
                              const double* perturb, double fa, double qi, const WaveLds& L,
                              double& ydot) {
     S.nfe++;
@@ -77,7 +79,7 @@ __device__ __forceinline__ double gamma_k(int k) {
 
 // Difference-array rescaling D[0..order] <- (R U)^T D[0..order]
 template <int ND>
-__device__ void change_D(double (&D)[ND], int order, double factor, double* ru /*LDS 36*/) {
+__device__ __forceinline__ void change_D(double (&D)[ND], int order, double factor, double* ru /*LDS 36*/) {
     const int m1 = order + 1;
     const int lane = lane_id();
     // lane (i, j) computes RU[i][j] = sum_l R[i][l] U[l][j]
@@ -113,7 +115,7 @@ __device__ void change_D(double (&D)[ND], int order, double factor, double* ru /
 // LU of M = I - c J in LDS (M[j*64 + lane] = row `lane`, column j), row-per-lane,
 // partial pivoting: at step k the pivot is the not-yet-pivoted lane with the largest
 // |M[lane][k]| (lowest lane on ties).  step_of = k for the pivot lane of step k.
-__device__ bool lu_factor(double* M, const double* J, double c, int n, int lane, int& step_of) {
+__device__ __forceinline__ bool lu_factor(double* M, const double* J, double c, int n, int lane, int& step_of) {
     const bool row = lane < n;
     for (int j = 0; j < n; j++)
         M[j * 64 + lane] = row ? ((lane == j ? 1.0 : 0.0) - c * J[j * 64 + lane]) : 0.0;
@@ -144,7 +146,7 @@ __device__ bool lu_factor(double* M, const double* J, double c, int n, int lane,
 }
 
 // Solve M x = b; b is this lane's row value (lane = row); returns x for lane = variable.
-__device__ double lu_solve(const double* M, int n, int lane, int step_of, double b) {
+__device__ __forceinline__ double lu_solve(const double* M, int n, int lane, int step_of, double b) {
     // forward: rows in pivot order
     for (int k = 0; k < n; k++) {
         const int p = __ffsll(__ballot(step_of == k)) - 1;
@@ -165,7 +167,7 @@ __device__ double lu_solve(const double* M, int n, int lane, int step_of, double
 
 // SourceSystemCVODE::fdJacobian: column i written to J[i*64 + lane]
 template <int NR, bool INTERP>
-__device__ bool fd_jacobian(const DevMech& m, int lane, int n, double yv, double ydot, BdfSystem& S,
+__device__ __forceinline__ bool fd_jacobian(const DevMech& m, int lane, int n, double yv, double ydot, BdfSystem& S,
                             RateRegs<NR>& R, const double* perturb, double fa, double qi,
                             const WaveLds& L, double* J) {
     const double eps = 1.4901161193847656e-08;  // sqrt(DBL_EPSILON)
@@ -184,7 +186,7 @@ __device__ bool fd_jacobian(const DevMech& m, int lane, int n, double yv, double
 }
 
 template <int NR, bool INTERP>
-__device__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool active, double& y,
+__device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool active, double& y,
                              BdfSystem& S, RateRegs<NR>& R, const double* perturb, const WaveLds& L,
                              double* M, double* ru, double* J, double tf, int& nsteps,
                              double* qint_out) {
