@@ -48,7 +48,7 @@ def parse():
     p.add_argument("--bpc", type=int, default=0, help="workgroups per CU (0: as many as fit)")
     p.add_argument("--schedule", choices=["cost", "fifo"], default="cost")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
-    p.add_argument("--workload", choices=["stage", "uneven", "diffusion", "split", "copy"],
+    p.add_argument("--workload", choices=["stage", "uneven", "diffusion", "split", "copy", "thermo"],
                    default="stage")
     p.add_argument("--mech", default="gri30", help="mechanism of the split workload")
     return p.parse_args()
@@ -75,6 +75,8 @@ def main():
         return run_copy(a, dev)
     if a.workload == "split":
         return run_split(a, dev)
+    if a.workload == "thermo":
+        return run_thermo(a, dev)
     N = a.points
     integ = capi.EMX_QSS if a.integrator == "qss" else capi.EMX_BDF
     kin = capi.EMX_KIN_STANDARD if a.kinetics == "standard" else capi.EMX_KIN_INTERP
@@ -485,6 +487,69 @@ def run_split(a, dev):
         "host_driven_ms_per_step": ms_host, "speedup_vs_host_driven": ms_host / ms_dev,
         "max_abs_diff_vs_host_driven": float(np.abs(got - ref).max()),
     }
+    print(json.dumps(res))
+
+
+def run_thermo(a, dev):
+    """Thermodynamic properties of updateChemicalProperties (rho, Wmx, cp, cp_k, h_k) for
+    GRI-Mech 3.0 on 65536 points."""
+    import numpy as np
+    import torch
+
+    from ember_amd import capi, workloads
+    N = a.points if a.points != 4096 else 65536
+    st = workloads.tanh_progress_states(N, seed=2, name="gri30")
+    ctx = capi.Context("gri30")
+    s = torch.cuda.current_stream(dev)
+    ctx.set_stream(s.cuda_stream)
+    nV = st.shape[1]
+    K = nV - 2
+    f = dict(dtype=torch.float64, device=dev)
+    d = torch.from_numpy(st).to(dev)
+    rho, Wmx, cp = (torch.empty(N, **f) for _ in range(3))
+    cpk, hk = torch.empty((N, K), **f), torch.empty((N, K), **f)
+    call = lambda: ctx.thermo_properties_device(d, rho, cp, Wmx=Wmx, cpSpec=cpk, hk=hk)
+    for _ in range(a.warmup):
+        call()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(a.steps):
+        call()
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    algo = 8.0 * N * (nV + 3 + 2 * K)  # read the state, write rho, Wmx, cp, cp_k, h_k
+    gbs = algo / (ms * 1e-3) / 1e9
+    print(f"[bench] thermo properties gri30 x {N}: {ms * 1e3:.1f} us per call, {gbs:.0f} GB/s",
+          file=sys.stderr, flush=True)
+    res = {
+        "metric": "thermo properties HBM GB/s (GRI-Mech 3.0, 65536 points)", "value": gbs,
+        "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded tanh progress-variable states)",
+        "config": {"workload": "updateChemicalProperties thermo block: rho, Wmx, cp, cp_k, h_k "
+                               "for GRI-Mech 3.0 (53 species)", "points": N, "seed": 2},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
+                     "frac": gbs / 8000.0, "traffic": None, "algorithmic_bytes_per_call": algo},
+    }
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        from ember_amd import mechanism
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        om = O.Mech(mechanism.table_path("gri30"))
+        om.thermo_properties(st[:64], nthreads=threads)
+        c0 = time.perf_counter()
+        ref = om.thermo_properties(st, nthreads=threads)
+        cpu_s = time.perf_counter() - c0
+        got = {"rho": rho, "Wmx": Wmx, "cp": cp, "cpSpec": cpk, "hk": hk}
+        err = {k: float(np.abs(v.cpu().numpy() - ref[k]).max() / np.abs(ref[k]).max())
+               for k, v in got.items()}
+        res["cpu_baseline"] = {"value": algo / cpu_s / 1e9, "unit": "GB/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"all {N} points, OpenMP dynamic,1, {cpu_s * 1e3:.1f} ms"}
+        res["parity"] = {"max_abs_err_over_max": err}
     print(json.dumps(res))
 
 

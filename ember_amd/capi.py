@@ -92,6 +92,7 @@ SIGNATURES = [
       C.c_double, C.c_double, C.c_int, _vp]),
     ("emx_correct_mass_fractions_device", C.c_int, [_vp, _sz, _sz, _vp]),
     ("emx_heat_release_device", C.c_int, [_vp, _sz, _sz, _vp, C.c_double, _vp, _vp]),
+    ("emx_thermo_properties_device", C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]
 
 _lib = None
@@ -321,6 +322,13 @@ class Context:
         N, nV = state.shape
         _check(lib().emx_heat_release_device(self.h, nV, N, _tptr(state), float(t_now), _tptr(qdot),
                                              _tptr(status)))
+
+    def thermo_properties_device(self, state, rho, cp, Wmx=None, cpSpec=None, hk=None):
+        """Density, mean molecular weight, cp and the species' partial molar cp and
+        enthalpies at every point of the device state (N, nVars)."""
+        N, nV = state.shape
+        _check(lib().emx_thermo_properties_device(self.h, nV, N, _tptr(state), _tptr(rho), _tptr(Wmx),
+                                                  _tptr(cp), _tptr(cpSpec), _tptr(hk)))
 
     def tridiagonal_integrate(self, a, b, c, k, y, h, t0, t1):
         """Constant systems, (nSys, N) arrays.  Returns y after integrateToTime(t1)."""

@@ -893,6 +893,17 @@ int emx_heat_release_device(emx_context* c, size_t nVars, size_t nPoints, const 
     return e ? fail(-10, std::string("heat release launch: ") + hipGetErrorString((hipError_t)e)) : 0;
 }
 
+int emx_thermo_properties_device(emx_context* c, size_t nVars, size_t nPoints, const double* state,
+                                 double* rho, double* Wmx, double* cp, double* cpSpec, double* hk) {
+    if (int rc = check_ctx(c)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
+    if (nPoints == 0) return 0;
+    if (!state || !rho || !cp) return fail(-1, "state, rho or cp is NULL");
+    int e = launch_thermo_props(c->dm, state, (int)nPoints, rho, Wmx, cp, cpSpec, hk, c->stream);
+    return e ? fail(-10, std::string("thermo launch: ") + hipGetErrorString((hipError_t)e)) : 0;
+}
+
 int emx_tridiagonal_integrate(emx_context* c, size_t nSys, size_t N, const double* a, const double* b,
                               const double* cc, const double* k, const double* h_in, double t0,
                               double t1, double* y) {

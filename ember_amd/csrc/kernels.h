@@ -31,6 +31,8 @@ int launch_time_derivatives(size_t n, double dt, const double* delta_diff, const
 int launch_correct_mass_fractions(int nVars, int nPoints, double* state, hipStream_t st);
 int launch_heat_release(const DevMech& m, const DevOpts& o, const double* state, const double* perturb, int n,
                         double* qdot, int* status, hipStream_t stream);
+int launch_thermo_props(const DevMech& m, const double* state, int n, double* rho, double* Wmx,
+                        double* cp, double* cpSpec, double* hk, hipStream_t stream);
 
 }  // namespace emx
 

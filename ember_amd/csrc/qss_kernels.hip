@@ -167,6 +167,75 @@ __global__ void __launch_bounds__(256) heat_release_kernel(DevMech mg, const dou
     }
 }
 
+// FlameSolver::updateChemicalProperties, thermodynamic block (src/flameSolver.cpp:667-673):
+// CanteraGas::setStateMass (Phase::setMassFractions clips negatives and normalises), then
+// density, meanMolecularWeight, cp_mass, getPartialMolarCp and getPartialMolarEnthalpies,
+// with the arithmetic of the stage kernels' set_state, nasa() and odefun thermo.  Lane =
+// state variable; each wavefront takes kThermoPointsPerWave consecutive points, issuing
+// all their state loads first and reading its species' NASA7 coefficients (both ranges)
+// once.  No LDS.
+constexpr int kThermoPointsPerWave = 4;
+
+__global__ void __launch_bounds__(256) thermo_props_kernel(DevMech m, const double* __restrict__ state,
+                                                           int n, double* __restrict__ rho,
+                                                           double* __restrict__ Wmx,
+                                                           double* __restrict__ cp,
+                                                           double* __restrict__ cpSpec,
+                                                           double* __restrict__ hk) {
+    constexpr int PPW = kThermoPointsPerWave;
+    const int lane = lane_id();
+    const long p0 = ((long)blockIdx.x * (blockDim.x / 64) + wave_in_block()) * PPW;
+    if (p0 >= n) return;
+    const int nv = m.nVars, K = nv - 2;
+    const bool sp = is_species_lane(lane, m);
+    double y[PPW];
+#pragma unroll
+    for (int i = 0; i < PPW; i++)
+        y[i] = (lane < nv && p0 + i < n) ? state[(size_t)(p0 + i) * nv + lane] : 0.0;
+    const int k = sp ? lane - 2 : 0;
+    double lo[6], hi[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        lo[j] = sp ? m.nasa[14 * k + j] : 0.0;
+        hi[j] = sp ? m.nasa[14 * k + 7 + j] : 0.0;
+    }
+    const double Tmid = m.Tmid[k];
+    const double rW = sp ? m.rW[k] : 0.0;
+#pragma unroll
+    for (int i = 0; i < PPW; i++) {
+        const long p = p0 + i;
+        if (p >= n) break;  // uniform: p is the wave's
+        const double T = bcast(y[i], 1);
+        const double yc = sp ? fmax(y[i], 0.0) : 0.0;
+        const double norm = wave_sum(yc);
+        const double ym = sp ? yc * (1.0 / norm) * rW : 0.0;
+        const double mmw = 1.0 / wave_sum(ym);
+        // NasaPoly1::updateProperties (cp/R and h/RT), as nasa()
+        const bool low = T <= Tmid;
+        double c[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) c[j] = low ? lo[j] : hi[j];
+        const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+        const double ct0 = c[0];
+        const double ct1 = c[1] * T;
+        const double ct2 = c[2] * T2;
+        const double ct3 = c[3] * T3;
+        const double ct4 = c[4] * T4;
+        const double cpR = ct0 + ct1 + ct2 + ct3 + ct4;
+        const double hRT = ct0 + 0.5 * ct1 + 1.0 / 3.0 * ct2 + 0.25 * ct3 + 0.2 * ct4 + c[5] * (1.0 / T);
+        const double s = wave_sum(ym * cpR);
+        if (lane == 0) {
+            rho[p] = m.P * mmw / (T * kGasConstant);
+            if (Wmx) Wmx[p] = mmw;
+            cp[p] = kGasConstant * (mmw * s) / mmw;
+        }
+        if (sp) {
+            if (cpSpec) cpSpec[(size_t)p * K + k] = kGasConstant * cpR;  // J/kmol/K
+            if (hk) hk[(size_t)p * K + k] = hRT * kGasConstant * T;      // J/kmol
+        }
+    }
+}
+
 }  // namespace
 
 size_t qss_lds_bytes(const DevMech& m, int waves_per_block) {
@@ -233,6 +302,16 @@ int launch_heat_release(const DevMech& m, const DevOpts& o, const double* state,
             hipLaunchKernelGGL((heat_release_kernel<NR, false>), dim3(blocks), dim3(64 * wpb), lds,
                                stream, m, state, perturb, n, o, qdot, status);
     });
+    return (int)hipGetLastError();
+}
+
+int launch_thermo_props(const DevMech& m, const double* state, int n, double* rho, double* Wmx,
+                        double* cp, double* cpSpec, double* hk, hipStream_t stream) {
+    const int wpb = 4;
+    const int per_block = wpb * kThermoPointsPerWave;
+    const int blocks = (n + per_block - 1) / per_block;
+    hipLaunchKernelGGL(thermo_props_kernel, dim3(blocks), dim3(64 * wpb), 0, stream, m, state, n, rho,
+                       Wmx, cp, cpSpec, hk);
     return (int)hipGetLastError();
 }
 

@@ -274,6 +274,17 @@ int emx_correct_mass_fractions_device(emx_context* ctx, size_t nVars, size_t nPo
 int emx_heat_release_device(emx_context* ctx, size_t nVars, size_t nPoints, const double* state,
                             double t_now, double* qdot, int32_t* status);
 
+/* FlameSolver::updateChemicalProperties, thermodynamic block (src/flameSolver.cpp:667-673):
+ * CanteraGas::setStateMass(Y_j, T_j) (chemistry0d.cpp:492-495), then getDensity,
+ * getMixtureMolecularWeight, getSpecificHeatCapacity, getSpecificHeatCapacities and
+ * getEnthalpies (chemistry0d.cpp:527-535, 608-631) for every point of the device state
+ * matrix.  Device pointers: rho [kg/m^3], Wmx [kg/kmol], cp [J/kg/K] (nPoints each);
+ * cpSpec [J/kmol/K] and hk [J/kmol] (nPoints x nSpecies, point-major as Ember's
+ * column-major cpSpec/hk matrices).  Wmx, cpSpec and hk may be NULL. */
+int emx_thermo_properties_device(emx_context* ctx, size_t nVars, size_t nPoints,
+                                 const double* state, double* rho, double* Wmx, double* cp,
+                                 double* cpSpec, double* hk);
+
 /* TridiagonalIntegrator::integrateToTime for nSys constant systems y' = A y + k
  * (test/tridiagonalIntegrator.cpp ConstantTridiagonalODE).  a, b, c, k, y: nSys x N,
  * row-major (system s contiguous).  h: initial step per system.  Host pointers. */

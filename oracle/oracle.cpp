@@ -1311,6 +1311,35 @@ int orc_rates(void* mech, const orc_options* o, int n, const double* T, const do
     return 0;
 }
 
+// FlameSolver::updateChemicalProperties, thermodynamic block (src/flameSolver.cpp:667-673):
+// setStateMass, then getDensity, getMixtureMolecularWeight, getSpecificHeatCapacity,
+// getSpecificHeatCapacities (R cp_k/R) and getEnthalpies (RT h_k/RT) per point; the
+// tbb::parallel_for over points (grain 1, :653-654) as OpenMP dynamic,1.
+int orc_thermo_properties(void* mech, double pressure, int nPoints, const double* state,
+                          double* rho, double* Wmx, double* cp, double* cpSpec, double* hk,
+                          int nthreads) {
+    Mech& m = *static_cast<Mech*>(mech);
+    const int nv = m.K + 2;
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+    {
+        Gas g(m, 0, pressure);
+#pragma omp for schedule(dynamic, 1)
+        for (int j = 0; j < nPoints; j++) {
+            const double* s = state + (size_t)j * nv;
+            g.setMassFractions(s + 2);
+            g.setState_TP(s[1]);
+            rho[j] = g.dens;
+            if (Wmx) Wmx[j] = g.mmw;
+            cp[j] = g.cp_mass();
+            if (cpSpec)
+                for (int k = 0; k < m.K; k++) cpSpec[(size_t)j * m.K + k] = kGasConstant * g.cpR[k];
+            if (hk) g.enthalpies(hk + (size_t)j * m.K);
+        }
+    }
+    return 0;
+}
+
 int orc_source_integrate(void* mech, const orc_options* o, int nPoints, const double* state_in,
                          const double* split_const, const double* qloss, double dt,
                          double* state_out, int* status, int* stat1, int* stat2, double* qint,

@@ -65,6 +65,13 @@ int orc_rates(void* mech, const orc_options* o, int n, const double* T, const do
               int nonorm, double* wdotQ, double* wdotD, double* wdot_net, double* hk,
               double* rho, double* cp, double* ropf, double* ropr);
 
+/* Thermodynamic properties of FlameSolver::updateChemicalProperties for nPoints points:
+ * state point-major (nSpecies + 2 per point: U, T, Y); rho, Wmx, cp per point; cpSpec and
+ * hk (J/kmol/K, J/kmol) nPoints x nSpecies; Wmx, cpSpec, hk nullable. */
+int orc_thermo_properties(void* mech, double pressure, int nPoints, const double* state,
+                          double* rho, double* Wmx, double* cp, double* cpSpec, double* hk,
+                          int nthreads);
+
 /* One production stage for nPoints points (FlameSolver::integrateProductionTerms).
  * status: 0 ok, <0 failure (the point keeps its input state).
  * stat1/stat2: QSS gcount/rcount, BDF nsteps/nfe.  qint: integral of qdot over the

@@ -57,6 +57,8 @@ def lib():
         L.orc_rate_function_eval.argtypes = [C.c_void_p, C.c_double]
         L.orc_rates.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, C.c_int,
                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_thermo_properties.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp, _dp,
+                                            _dp, _dp, C.c_int]
         L.orc_source_integrate.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, _dp,
                                            C.c_double, _dp, _ip, _ip, _ip, _dp, C.c_int]
         L.orc_qss_test_problem.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double,
@@ -138,6 +140,20 @@ class Mech:
                              _p(out["cp"]), _p(out["ropf"]), _p(out["ropr"]))
         if rc:
             raise RuntimeError(f"orc_rates failed: {rc}")
+        return out
+
+    def thermo_properties(self, state, pressure=101325.0, nthreads=0):
+        """rho, Wmx, cp (N) and cpSpec, hk (N, K) of point-major states (N, K + 2)."""
+        state = np.ascontiguousarray(np.atleast_2d(state), dtype=np.float64)
+        n = state.shape[0]
+        out = {k: np.zeros(n) for k in ("rho", "Wmx", "cp")}
+        out["cpSpec"] = np.zeros((n, self.K))
+        out["hk"] = np.zeros((n, self.K))
+        rc = lib().orc_thermo_properties(self.h, float(pressure), n, _p(state), _p(out["rho"]),
+                                         _p(out["Wmx"]), _p(out["cp"]), _p(out["cpSpec"]),
+                                         _p(out["hk"]), int(nthreads))
+        if rc:
+            raise RuntimeError(f"orc_thermo_properties failed: {rc}")
         return out
 
     def rhs(self, y, opts=None, split_const=None, qloss=0.0):

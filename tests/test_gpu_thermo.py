@@ -1,0 +1,74 @@
+"""Thermodynamic properties of FlameSolver::updateChemicalProperties on the GPU
+(emx_thermo_properties_device) against the oracle (orc_thermo_properties) on tanh-progress
+states of both mechanisms plus edge states (a negative mass fraction, unnormalised Y,
+T at 300 K, on both sides of the 1000 K polynomial midpoint and at 3500 K), and on
+ragged point counts around the 4-wave workgroup.  Bar: rel 1e-12 on rho, Wmx, cp, cp_k,
+and |dh_k| <= 1e-12 max |h_k| (h_k crosses zero)."""
+import numpy as np
+import pytest
+
+from ember_amd import capi, mechanism
+from ember_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _states(name, n):
+    st = W.tanh_progress_states(max(n, 8), seed=21, name=name)[:n].copy()
+    edits = [(0, 2, -1e-8), (2, 1, 300.0), (3, 1, 999.9), (4, 1, 1000.1), (5, 1, 3500.0)]
+    for i, v, x in edits:
+        if i < n:
+            st[i, v] = x
+    if n > 1:
+        st[1, 2:] *= 1.01
+    return st
+
+
+def _run(ctx, st, gpu, optional=True):
+    import torch
+    N, nV = st.shape
+    f = dict(dtype=torch.float64, device=gpu)
+    d = torch.from_numpy(np.ascontiguousarray(st)).to(gpu)
+    rho, cp = torch.full((N,), -1.0, **f), torch.full((N,), -1.0, **f)
+    Wmx = torch.full((N,), -1.0, **f) if optional else None
+    cpk = torch.full((N, nV - 2), -1.0, **f) if optional else None
+    hk = torch.full((N, nV - 2), -1.0, **f) if optional else None
+    ctx.thermo_properties_device(d, rho, cp, Wmx=Wmx, cpSpec=cpk, hk=hk)
+    torch.cuda.synchronize()
+    out = {"rho": rho, "cp": cp, "Wmx": Wmx, "cpSpec": cpk, "hk": hk}
+    return {k: (v.cpu().numpy() if v is not None else None) for k, v in out.items()}
+
+
+def _check(got, ref):
+    for k in ("rho", "Wmx", "cp", "cpSpec"):
+        np.testing.assert_allclose(got[k], ref[k], rtol=1e-12, err_msg=k)
+    scale = np.abs(ref["hk"]).max() if ref["hk"].size else 1.0
+    np.testing.assert_allclose(got["hk"], ref["hk"], rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("name,n", [("h2o2", 1), ("h2o2", 5), ("h2o2", 257), ("gri30", 3),
+                                    ("gri30", 300)])
+def test_thermo_properties_parity(gpu, oracle_mod, name, n):
+    print(f"[thermo] {name} n={n}", flush=True)
+    st = _states(name, n)
+    ctx = capi.Context(name)
+    got = _run(ctx, st, gpu)
+    ref = oracle_mod.Mech(mechanism.table_path(name)).thermo_properties(st)
+    _check(got, ref)
+
+
+def test_thermo_properties_optional_outputs_and_checks(gpu, oracle_mod):
+    import torch
+    print("[thermo] optional outputs, argument checks", flush=True)
+    st = _states("gri30", 9)
+    ctx = capi.Context("gri30")
+    got = _run(ctx, st, gpu, optional=False)
+    ref = oracle_mod.Mech(mechanism.table_path("gri30")).thermo_properties(st)
+    np.testing.assert_allclose(got["rho"], ref["rho"], rtol=1e-12)
+    np.testing.assert_allclose(got["cp"], ref["cp"], rtol=1e-12)
+    d = torch.from_numpy(st[:, :-1].copy()).to(gpu)  # nVars != nSpecies + 2
+    rho = torch.empty(9, dtype=torch.float64, device=gpu)
+    with pytest.raises(Exception):
+        ctx.thermo_properties_device(d, rho, rho.clone())
+    empty = torch.empty((0, st.shape[1]), dtype=torch.float64, device=gpu)
+    ctx.thermo_properties_device(empty, rho[:0], rho[:0])  # no points: no launch
