@@ -138,12 +138,21 @@ def default_options(**kw) -> Options:
     return o
 
 
-def _tptr(t):
-    """Device pointer of a torch tensor (None passes through)."""
+_F64, _I32 = "torch.float64", "torch.int32"
+
+
+def _tptr(t, count=None, dtype=_F64):
+    """Device pointer of a torch tensor (None passes through).  count: elements the call
+    reads or writes through it; the tensor must hold at least that many, of dtype."""
     if t is None:
         return None
     if not t.is_cuda or not t.is_contiguous():
         raise EmberAmdError("device entry points need contiguous GPU tensors")
+    if count is not None:
+        if str(t.dtype) != dtype:
+            raise EmberAmdError(f"expected a {dtype} tensor, got {t.dtype}")
+        if t.numel() < count:
+            raise EmberAmdError(f"tensor holds {t.numel()} elements, the call uses {count}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -236,10 +245,11 @@ class Context:
     def source_integrate_device(self, state_in, state_out, dt, split_const=None, qloss=None,
                                 status=None, stat1=None, stat2=None, qint=None, t_start=0.0):
         """torch CUDA tensors, asynchronous on the context stream."""
-        n = state_in.shape[0]
+        n, nv = state_in.shape[0], self.n_vars
         _check(lib().emx_source_integrate_device(
-            self.h, n, self.n_vars, _tptr(state_in), _tptr(split_const), _tptr(qloss), t_start, dt,
-            _tptr(state_out), _tptr(status), _tptr(stat1), _tptr(stat2), _tptr(qint)))
+            self.h, n, nv, _tptr(state_in, n * nv), _tptr(split_const, n * nv), _tptr(qloss, n), t_start,
+            dt, _tptr(state_out, n * nv), _tptr(status, n, _I32), _tptr(stat1, n, _I32),
+            _tptr(stat2, n, _I32), _tptr(qint, n)))
 
     # ---- discrete adjoint of the QSS stage (torch CUDA tensors, context stream)
     def set_adjoint_capacity(self, max_checkpoints: int):
@@ -247,17 +257,18 @@ class Context:
 
     def adjoint_forward(self, state_in, state_out, dt, heat_release=None, status=None,
                         split_const=None, qloss=None):
-        n = state_in.shape[0]
+        n, nv = state_in.shape[0], self.n_vars
         _check(lib().emx_source_adjoint_forward(
-            self.h, n, self.n_vars, _tptr(state_in), _tptr(split_const), _tptr(qloss), dt,
-            _tptr(state_out), _tptr(status), _tptr(heat_release)))
+            self.h, n, nv, _tptr(state_in, n * nv), _tptr(split_const, n * nv), _tptr(qloss, n), dt,
+            _tptr(state_out, n * nv), _tptr(status, n, _I32), _tptr(heat_release, n)))
 
     def adjoint_backward(self, state_in, dt, grad_state, grad_lnk=None, Qbar=None, ybar=None,
                          status=None, split_const=None, qloss=None):
-        n = state_in.shape[0]
+        n, nv = state_in.shape[0], self.n_vars
         _check(lib().emx_source_adjoint_backward(
-            self.h, n, self.n_vars, _tptr(state_in), _tptr(split_const), _tptr(qloss), dt,
-            _tptr(Qbar), _tptr(ybar), _tptr(grad_state), _tptr(grad_lnk), _tptr(status)))
+            self.h, n, nv, _tptr(state_in, n * nv), _tptr(split_const, n * nv), _tptr(qloss, n), dt,
+            _tptr(Qbar, n), _tptr(ybar, n * nv), _tptr(grad_state, n * nv),
+            _tptr(grad_lnk, n * self.n_reactions), _tptr(status, n, _I32)))
 
     def rates(self, T, Y, nonorm=False):
         T = np.ascontiguousarray(np.atleast_1d(T), dtype=np.float64)
@@ -320,15 +331,17 @@ class Context:
 
     def heat_release_device(self, state, qdot, status=None, t_now=0.0):
         N, nV = state.shape
-        _check(lib().emx_heat_release_device(self.h, nV, N, _tptr(state), float(t_now), _tptr(qdot),
-                                             _tptr(status)))
+        _check(lib().emx_heat_release_device(self.h, nV, N, _tptr(state, N * nV), float(t_now),
+                                             _tptr(qdot, N), _tptr(status, N, _I32)))
 
     def thermo_properties_device(self, state, rho, cp, Wmx=None, cpSpec=None, hk=None):
         """Density, mean molecular weight, cp and the species' partial molar cp and
         enthalpies at every point of the device state (N, nVars)."""
         N, nV = state.shape
-        _check(lib().emx_thermo_properties_device(self.h, nV, N, _tptr(state), _tptr(rho), _tptr(Wmx),
-                                                  _tptr(cp), _tptr(cpSpec), _tptr(hk)))
+        K = nV - 2
+        _check(lib().emx_thermo_properties_device(self.h, nV, N, _tptr(state, N * nV), _tptr(rho, N),
+                                                  _tptr(Wmx, N), _tptr(cp, N), _tptr(cpSpec, N * K),
+                                                  _tptr(hk, N * K)))
 
     def tridiagonal_integrate(self, a, b, c, k, y, h, t0, t1):
         """Constant systems, (nSys, N) arrays.  Returns y after integrateToTime(t1)."""

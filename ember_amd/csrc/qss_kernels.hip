@@ -293,7 +293,7 @@ int launch_heat_release(const DevMech& m, const DevOpts& o, const double* state,
                         int n, double* qdot, int* status, hipStream_t stream) {
     const int wpb = 4;
     const size_t lds = qss_lds_bytes(m, wpb);
-    const int blocks = (n + wpb - 1) / wpb;
+    const int blocks = (int)(((long)n + wpb - 1) / wpb);
     EMX_DISPATCH_NR(m.nRounds, {
         if (m.interp)
             hipLaunchKernelGGL((heat_release_kernel<NR, true>), dim3(blocks), dim3(64 * wpb), lds,
@@ -309,7 +309,7 @@ int launch_thermo_props(const DevMech& m, const double* state, int n, double* rh
                         double* cp, double* cpSpec, double* hk, hipStream_t stream) {
     const int wpb = 4;
     const int per_block = wpb * kThermoPointsPerWave;
-    const int blocks = (n + per_block - 1) / per_block;
+    const int blocks = (int)(((long)n + per_block - 1) / per_block);
     hipLaunchKernelGGL(thermo_props_kernel, dim3(blocks), dim3(64 * wpb), 0, stream, m, state, n, rho,
                        Wmx, cp, cpSpec, hk);
     return (int)hipGetLastError();

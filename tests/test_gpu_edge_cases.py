@@ -89,3 +89,23 @@ def test_diffusion_argument_checks(gpu):
         ctx.diffusion_integrate(y, B, D, x, 0.0, 1e-6, B_row=np.array([0, 1], np.int32))
     out = ctx.diffusion_integrate(y, B, D, x, 0.0, 1e-6, B_row=np.zeros(2, np.int32))
     np.testing.assert_allclose(out, 0.5, rtol=1e-14)  # a constant profile stays constant
+
+
+def test_device_tensor_checks(gpu):
+    """The device wrappers refuse tensors too small for the call or of the wrong dtype
+    before anything reaches a kernel."""
+    import torch
+    print("[edge] device tensor checks", flush=True)
+    ctx = capi.Context("h2o2")
+    st = torch.from_numpy(W.tanh_progress_states(8, seed=11, name="h2o2")).to(gpu)
+    with pytest.raises(capi.EmberAmdError):  # output one row short
+        ctx.source_integrate_device(st, torch.empty_like(st)[:7], 2e-5)
+    with pytest.raises(capi.EmberAmdError):  # status of the wrong dtype
+        ctx.source_integrate_device(st, torch.empty_like(st), 2e-5,
+                                    status=torch.zeros(8, dtype=torch.int64, device=gpu))
+    with pytest.raises(capi.EmberAmdError):  # cp_k of one point too few
+        rho = torch.empty(8, dtype=torch.float64, device=gpu)
+        ctx.thermo_properties_device(st, rho, rho.clone(),
+                                     cpSpec=torch.empty((7, ctx.n_species), dtype=torch.float64, device=gpu))
+    with pytest.raises(capi.EmberAmdError):  # heat release of float32
+        ctx.heat_release_device(st, torch.empty(8, dtype=torch.float32, device=gpu))
