@@ -305,29 +305,30 @@ class Context:
                                    h_per_var=None):
         """torch CUDA tensors; state is updated in place."""
         N, nV = state.shape
-        g = Grid(C.cast(C.c_void_p(x.data_ptr()), _dp), alpha, left_bc, right_bc, y_inf, wall_const,
-                 wall_var)
+        g = Grid(C.cast(_tptr(x, N), _dp), alpha, left_bc, right_bc, y_inf, wall_const, wall_var)
         _check(lib().emx_diffusion_integrate_device(
-            self.h, nV, N, _tptr(B), B.shape[0], _tptr(B_row), _tptr(D), _tptr(split_const),
+            self.h, nV, N, _tptr(B, B.shape[0] * N), B.shape[0], _tptr(B_row, nV, _I32), _tptr(D, N * nV),
+            _tptr(split_const, N * nV),
             C.byref(g), _p(None if h_per_var is None else np.ascontiguousarray(h_per_var, dtype=np.float64)),
-            t0, t1, _tptr(state)))
+            t0, t1, _tptr(state, N * nV)))
 
     def split_step_device(self, arrays: "SplitState", B, D, x, t, dt, B_row=None, qloss=None,
                           alpha=0, left_bc=0, right_bc=0, y_inf=0.0, wall_const=0.0, wall_var=1,
                           h_per_var=None, balanced=True, status=None):
         """One device-resident split step (SplitSolver::step) on arrays.state, in place."""
         N, nV = arrays.state.shape
-        g = Grid(C.cast(C.c_void_p(x.data_ptr()), _dp), alpha, left_bc, right_bc, y_inf, wall_const,
-                 wall_var)
-        sa = SplitArrays(*(_tptr(t_) for t_ in arrays.fields()))
+        g = Grid(C.cast(_tptr(x, N), _dp), alpha, left_bc, right_bc, y_inf, wall_const, wall_var)
+        f = arrays.fields()
+        sa = SplitArrays(*(_tptr(t_, (3 if i == len(f) - 1 else 1) * N * nV) for i, t_ in enumerate(f)))
         hv = None if h_per_var is None else np.ascontiguousarray(h_per_var, dtype=np.float64)
         _check(lib().emx_split_step_device(
-            self.h, nV, N, C.byref(sa), _tptr(B), B.shape[0], _tptr(B_row), _tptr(D), C.byref(g),
-            _tptr(qloss), _p(hv), t, dt, 1 if balanced else 0, _tptr(status)))
+            self.h, nV, N, C.byref(sa), _tptr(B, B.shape[0] * N), B.shape[0], _tptr(B_row, nV, _I32),
+            _tptr(D, N * nV), C.byref(g), _tptr(qloss, N), _p(hv), t, dt, 1 if balanced else 0,
+            _tptr(status, N, _I32)))
 
     def correct_mass_fractions_device(self, state):
         N, nV = state.shape
-        _check(lib().emx_correct_mass_fractions_device(self.h, nV, N, _tptr(state)))
+        _check(lib().emx_correct_mass_fractions_device(self.h, nV, N, _tptr(state, N * nV)))
 
     def heat_release_device(self, state, qdot, status=None, t_now=0.0):
         N, nV = state.shape
