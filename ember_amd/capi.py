@@ -92,6 +92,7 @@ SIGNATURES = [
       C.c_double, C.c_double, C.c_int, _vp]),
     ("emx_correct_mass_fractions_device", C.c_int, [_vp, _sz, _sz, _vp]),
     ("emx_heat_release_device", C.c_int, [_vp, _sz, _sz, _vp, C.c_double, _vp, _vp]),
+    ("emx_thermo_properties", C.c_int, [_vp, _sz, _sz, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("emx_thermo_properties_device", C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]
 
@@ -334,6 +335,17 @@ class Context:
         N, nV = state.shape
         _check(lib().emx_heat_release_device(self.h, nV, N, _tptr(state, N * nV), float(t_now),
                                              _tptr(qdot, N), _tptr(status, N, _I32)))
+
+    def thermo_properties(self, state):
+        """Host arrays: rho, Wmx, cp (N) and cpSpec, hk (N, K) of states (N, nVars)."""
+        st = np.ascontiguousarray(np.atleast_2d(state), dtype=np.float64)
+        N, nV = st.shape
+        out = {k: np.zeros(N) for k in ("rho", "Wmx", "cp")}
+        out["cpSpec"] = np.zeros((N, nV - 2))
+        out["hk"] = np.zeros((N, nV - 2))
+        _check(lib().emx_thermo_properties(self.h, nV, N, _p(st), _p(out["rho"]), _p(out["Wmx"]),
+                                           _p(out["cp"]), _p(out["cpSpec"]), _p(out["hk"])))
+        return out
 
     def thermo_properties_device(self, state, rho, cp, Wmx=None, cpSpec=None, hk=None):
         """Density, mean molecular weight, cp and the species' partial molar cp and

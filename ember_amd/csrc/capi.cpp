@@ -893,6 +893,33 @@ int emx_heat_release_device(emx_context* c, size_t nVars, size_t nPoints, const 
     return e ? fail(-10, std::string("heat release launch: ") + hipGetErrorString((hipError_t)e)) : 0;
 }
 
+int emx_thermo_properties(emx_context* c, size_t nVars, size_t nPoints, const double* state,
+                          double* rho, double* Wmx, double* cp, double* cpSpec, double* hk) {
+    if (int rc = check_ctx(c)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
+    if (nPoints == 0) return 0;
+    if (!state || !rho || !cp) return fail(-1, "state, rho or cp is NULL");
+    const size_t N = nPoints, K = nVars - 2;
+    hipStream_t st = c->stream;
+    HIP_TRY(c->b_in.ensure(N * nVars * sizeof(double)));
+    HIP_TRY(c->b_out.ensure((3 * N + 2 * N * K) * sizeof(double)));
+    // staging: rho, Wmx, cp (N each), then cpSpec and hk (N x K each)
+    double* o = c->b_out.as<double>();
+    double *d_rho = o, *d_wmx = o + N, *d_cp = o + 2 * N, *d_cps = o + 3 * N, *d_hk = d_cps + N * K;
+    HIP_TRY(hipMemcpyAsync(c->b_in.p, state, N * nVars * sizeof(double), hipMemcpyHostToDevice, st));
+    int e = launch_thermo_props(c->dm, c->b_in.as<double>(), (int)N, d_rho, Wmx ? d_wmx : nullptr, d_cp,
+                                cpSpec ? d_cps : nullptr, hk ? d_hk : nullptr, st);
+    if (e) return fail(-10, std::string("thermo launch: ") + hipGetErrorString((hipError_t)e));
+    HIP_TRY(hipMemcpyAsync(rho, d_rho, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (Wmx) HIP_TRY(hipMemcpyAsync(Wmx, d_wmx, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(cp, d_cp, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (cpSpec) HIP_TRY(hipMemcpyAsync(cpSpec, d_cps, N * K * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (hk) HIP_TRY(hipMemcpyAsync(hk, d_hk, N * K * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
 int emx_thermo_properties_device(emx_context* c, size_t nVars, size_t nPoints, const double* state,
                                  double* rho, double* Wmx, double* cp, double* cpSpec, double* hk) {
     if (int rc = check_ctx(c)) return rc;

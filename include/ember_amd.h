@@ -285,6 +285,11 @@ int emx_thermo_properties_device(emx_context* ctx, size_t nVars, size_t nPoints,
                                  const double* state, double* rho, double* Wmx, double* cp,
                                  double* cpSpec, double* hk);
 
+/* The same on host arrays, the drop-in for the TBB loop over points (copied through the
+ * context's staging buffers; returns after the results are on the host). */
+int emx_thermo_properties(emx_context* ctx, size_t nVars, size_t nPoints, const double* state,
+                          double* rho, double* Wmx, double* cp, double* cpSpec, double* hk);
+
 /* TridiagonalIntegrator::integrateToTime for nSys constant systems y' = A y + k
  * (test/tridiagonalIntegrator.cpp ConstantTridiagonalODE).  a, b, c, k, y: nSys x N,
  * row-major (system s contiguous).  h: initial step per system.  Host pointers. */

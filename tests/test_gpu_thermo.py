@@ -72,3 +72,12 @@ def test_thermo_properties_optional_outputs_and_checks(gpu, oracle_mod):
         ctx.thermo_properties_device(d, rho, rho.clone())
     empty = torch.empty((0, st.shape[1]), dtype=torch.float64, device=gpu)
     ctx.thermo_properties_device(empty, rho[:0], rho[:0])  # no points: no launch
+
+
+@pytest.mark.parametrize("name,n", [("h2o2", 7), ("gri30", 130)])
+def test_thermo_properties_host_entry(gpu, oracle_mod, name, n):
+    print(f"[thermo] host entry {name} n={n}", flush=True)
+    st = _states(name, n)
+    got = capi.Context(name).thermo_properties(st)
+    ref = oracle_mod.Mech(mechanism.table_path(name)).thermo_properties(st)
+    _check(got, ref)
