@@ -19,6 +19,25 @@ import torch
 from . import capi
 
 
+class _multipliers:
+    """Sets the context's per-reaction multipliers to exp(ln_mult) for the adjoint calls
+    and restores the caller's multipliers afterwards (emx_set_rate_multipliers waits for
+    the work queued on the context stream before it replaces them), so that later
+    production stages on the same context are unaffected."""
+
+    def __init__(self, emx, ln_mult):
+        self.emx = emx
+        self.mult = np.exp(ln_mult.detach().double().cpu().numpy())
+
+    def __enter__(self):
+        self.prev = self.emx.rate_multipliers
+        self.emx.set_rate_multipliers(self.mult)
+
+    def __exit__(self, *exc):
+        self.emx.set_rate_multipliers(self.prev)
+        return False
+
+
 class QssStage(torch.autograd.Function):
     @staticmethod
     def forward(ctx, state, ln_mult, dt, emx, split_const=None, qloss=None):
@@ -26,12 +45,12 @@ class QssStage(torch.autograd.Function):
             raise capi.EmberAmdError("QssStage needs the states on the GPU")
         state = state.detach().contiguous()
         n = state.shape[0]
-        emx.set_rate_multipliers(np.exp(ln_mult.detach().double().cpu().numpy()))
         out = torch.empty_like(state)
         Q = torch.empty(n, dtype=torch.float64, device=state.device)
         status = torch.empty(n, dtype=torch.int32, device=state.device)
-        emx.adjoint_forward(state, out, dt, heat_release=Q, status=status,
-                            split_const=split_const, qloss=qloss)
+        with _multipliers(emx, ln_mult):
+            emx.adjoint_forward(state, out, dt, heat_release=Q, status=status,
+                                split_const=split_const, qloss=qloss)
         emx._adjoint_generation = getattr(emx, "_adjoint_generation", 0) + 1
         ctx.generation = emx._adjoint_generation
         ctx.emx, ctx.dt, ctx.split_const, ctx.qloss = emx, dt, split_const, qloss
@@ -44,14 +63,6 @@ class QssStage(torch.autograd.Function):
         state, ln_mult = ctx.saved_tensors
         emx = ctx.emx
         n = state.shape[0]
-        mult = np.exp(ln_mult.detach().double().cpu().numpy())
-        emx.set_rate_multipliers(mult)
-        if getattr(emx, "_adjoint_generation", 0) != ctx.generation:
-            # another forward pass replaced the checkpoints: recompute them for this one
-            out = torch.empty_like(state)
-            emx.adjoint_forward(state, out, ctx.dt, split_const=ctx.split_const, qloss=ctx.qloss)
-            emx._adjoint_generation += 1
-            ctx.generation = emx._adjoint_generation
         Qbar = (g_Q if g_Q is not None else torch.zeros(n, dtype=torch.float64, device=state.device))
         Qbar = Qbar.to(torch.float64).contiguous()
         ybar = None if g_out is None else g_out.to(torch.float64).contiguous()
@@ -59,8 +70,15 @@ class QssStage(torch.autograd.Function):
         need_k = ctx.needs_input_grad[1]
         grad_k = (torch.empty((n, emx.n_reactions), dtype=torch.float64, device=state.device)
                   if need_k else None)
-        emx.adjoint_backward(state, ctx.dt, grad_state, grad_lnk=grad_k, Qbar=Qbar, ybar=ybar,
-                             split_const=ctx.split_const, qloss=ctx.qloss)
+        with _multipliers(emx, ln_mult):
+            if getattr(emx, "_adjoint_generation", 0) != ctx.generation:
+                # another forward pass replaced the checkpoints: recompute them for this one
+                out = torch.empty_like(state)
+                emx.adjoint_forward(state, out, ctx.dt, split_const=ctx.split_const, qloss=ctx.qloss)
+                emx._adjoint_generation += 1
+                ctx.generation = emx._adjoint_generation
+            emx.adjoint_backward(state, ctx.dt, grad_state, grad_lnk=grad_k, Qbar=Qbar, ybar=ybar,
+                                 split_const=ctx.split_const, qloss=ctx.qloss)
         gk = None
         if need_k:
             gk = grad_k.sum(dim=0).to(device=ln_mult.device, dtype=ln_mult.dtype)

@@ -170,6 +170,7 @@ class Context:
         self.n_species = lib().emx_num_species(h)
         self.n_reactions = lib().emx_num_reactions(h)
         self.n_vars = self.n_species + 2
+        self.rate_multipliers = None  # per-reaction multipliers last set (None: all 1)
 
     def close(self):
         if getattr(self, "h", None):
@@ -190,6 +191,7 @@ class Context:
     def set_rate_multipliers(self, m):
         m = None if m is None else np.ascontiguousarray(m, dtype=np.float64)
         _check(lib().emx_set_rate_multipliers(self.h, _p(m)))
+        self.rate_multipliers = None if m is None else m.copy()
 
     def set_rate_multiplier_function(self, data):
         """Ember's rate multiplier function: data = [T0, T1, c_0, ...] as

@@ -9,7 +9,17 @@
 // the variable-order, variable-step method of Shampine & Reichelt (SIAM J. Sci.
 // Comput. 18, 1997; NDF family with kappa = 0) in difference-array form with
 // modified Newton iterations, the same algorithm as the CPU oracle's, so results
-// agree step for step.  CVODE's own step/order heuristics are not reproduced.
+// agree step for step.  The step, order and Newton logic restate scipy's
+// scipy.integrate._ivp.bdf (SciPy 1.15, BSD-3-Clause: NEWTON_MAXITER = 4, the
+// 0.2 / 10 step-factor bounds, solve_bdf_system's convergence test, change_D /
+// compute_R, select_initial_step's 0.01 d0/d1 rule, min_step = 10 ulp(t)); the
+// per-step failure bounds (10 convergence, 7 error-test failures) are CVODE's
+// defaults.  CVODE's own step/order heuristics are not reproduced.
+//
+// Per-point status: 0 ok, -2 rates failed (interpolation table range), -3 step size
+// below the minimum, -4 integrator_max_steps accepted steps, -5 singular Newton
+// matrix, -6 non-finite error estimate, -7 / -8 too many error-test / convergence
+// failures in one step, -9 invalid input state (non-finite, or T <= 0).
 //
 // Layout: lane v holds state variable v (and its column of the difference array);
 // the Jacobian lives in a per-wave global scratch slot (column j contiguous over
@@ -27,6 +37,9 @@ namespace {
 
 constexpr int kMaxOrder = 5;
 constexpr int kNewtonMaxIter = 4;
+// failed attempts allowed per step (CVODE's defaults maxncf and maxnef)
+constexpr int kMaxConvFail = 10;
+constexpr int kMaxErrFail = 7;
 
 struct BdfSystem {
     double W, rW, sc, qloss, atol;
@@ -233,6 +246,8 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
         bool accepted = false, current_jac = false;
         int n_iter = 0;
         double t_new = 0.0;
+        // failed attempts of this step, bounded as CVODE bounds them
+        int n_conv_fail = 0, n_err_fail = 0;
         while (!accepted) {
             if (h_abs < min_step) return -3;
             if (nsteps >= o.max_steps) return -4;
@@ -299,6 +314,7 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
             }
             if (!ok) return -2;
             if (!converged) {
+                if (++n_conv_fail >= kMaxConvFail) return -8;
                 const double factor = 0.5;
                 h_abs *= factor;
                 change_D(D, order, factor, ru);
@@ -309,7 +325,10 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
             safety = 0.9 * (2 * kNewtonMaxIter + 1) / (2 * kNewtonMaxIter + n_iter);
             scale = S.atol + rtol * fabs(ynew);
             error_norm = rms((1.0 / (order + 1)) * dvec, scale, active, n);
+            // a non-finite error estimate fails the point (it must not pass as a small one)
+            if (!isfinite(error_norm)) return -6;
             if (error_norm > 1) {
+                if (++n_err_fail >= kMaxErrFail) return -7;
                 const double factor = fmax(0.2, safety * pow(error_norm, -1.0 / (order + 1)));
                 h_abs *= factor;
                 change_D(D, order, factor, ru);
@@ -360,7 +379,7 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
             if (isnan(f[j])) f[j] = 0;
         for (int j = 1; j < 3; j++)
             if (f[j] > f[best]) best = j;
-        order += best - 1;
+        order = max(1, order + best - 1);
         const double factor = fmin(10.0, safety * f[best]);
         h_abs *= factor;
         change_D(D, order, factor, ru);
@@ -411,7 +430,9 @@ __global__ void __launch_bounds__(256) bdf_stage_kernel(
         R.T = -1.0;
         double y = y0;
         int nsteps = 0;
-        int err = bdf_integrate<NR, INTERP>(m, o, lane, active, y, S, R, perturb, L, M, ru, J, dt, nsteps,
+        int err = kInvalidState;
+        if (valid_input_state(lane, active, y0))
+            err = bdf_integrate<NR, INTERP>(m, o, lane, active, y, S, R, perturb, L, M, ru, J, dt, nsteps,
                                             qint_out ? qint_out + p : nullptr);
         if (active) state_out[(size_t)p * nv + lane] = (err >= 0) ? y : y0;
         if (lane == 0) {

@@ -384,6 +384,8 @@ int emx_set_rate_multipliers(emx_context* c, const double* m) {
     std::vector<double> p(c->lay.nRounds * 64, 1.0);
     if (m)
         for (int i = 0; i < c->hm.I; i++) p[c->lay.slot_of_rxn[i]] = m[i];
+    // stages already queued on the context stream read the previous multipliers
+    HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(c->perturb, p.data(), p.size() * sizeof(double), hipMemcpyHostToDevice));
     return 0;
 }

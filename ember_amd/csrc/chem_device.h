@@ -23,6 +23,17 @@
 
 namespace emx {
 
+// Status of a point whose input state Cantera's state setters would refuse
+// (Phase::setTemperature needs T > 0; a NaN or infinite value anywhere in the state).
+constexpr int kInvalidState = -9;
+
+// The whole wave agrees whether this point's input state can be integrated: every state
+// variable finite and T (lane 1) positive.
+__device__ __forceinline__ bool valid_input_state(int lane, bool active, double y0) {
+    const bool bad = active && (!isfinite(y0) || (lane == 1 && !(y0 > 0.0)));
+    return !wave_or(bad ? 1 : 0);
+}
+
 // per-wave LDS scratch (doubles)
 struct WaveLds {
     double* C;      // [64] concentrations

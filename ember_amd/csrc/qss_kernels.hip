@@ -71,7 +71,9 @@ __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
         int g = 0, r = 0;
         double qi = 0.0;
         NoHooks H;
-        int err = qss_integrate<NR, INTERP, NoHooks, EXT>(m, o, lane, active, Z, S, R, perturb, L, dt,
+        int err = kInvalidState;
+        if (valid_input_state(lane, active, y0))
+            err = qss_integrate<NR, INTERP, NoHooks, EXT>(m, o, lane, active, Z, S, R, perturb, L, dt,
                                                           QssResume{0.0, 0.0, true}, H, g, r, qi);
         double* yout = state_out + (size_t)p * nv;
         if (active) yout[lane] = (err >= 0) ? Z.y : y0;

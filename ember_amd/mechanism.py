@@ -244,6 +244,18 @@ _Yaml12Loader.add_implicit_resolver(
     list("tTfF"))
 
 
+_COMMON_KEYS = {"equation", "type", "duplicate", "note", "id"}
+_TB_KEYS = {"efficiencies", "default-efficiency"}
+# the fields each Cantera reaction type may carry here; anything else is refused
+_KNOWN_REACTION_KEYS = {
+    "elementary": _COMMON_KEYS | _TB_KEYS | {"rate-constant"},
+    "three-body": _COMMON_KEYS | _TB_KEYS | {"rate-constant"},
+    "falloff": _COMMON_KEYS | _TB_KEYS | {"high-P-rate-constant", "low-P-rate-constant", "Troe", "SRI"},
+    "pressure-dependent-Arrhenius": _COMMON_KEYS | {"rate-constants"},
+    "Chebyshev": _COMMON_KEYS | {"temperature-range", "pressure-range", "data"},
+}
+
+
 def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
     with open(path) as f:
         doc = yaml.load(f, Loader=_Yaml12Loader)
@@ -282,6 +294,14 @@ def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
     rxns: List[Reaction] = []
     for rd in doc["reactions"]:
         eq = rd["equation"]
+        rtype_s = rd.get("type", "elementary")
+        if rtype_s not in _KNOWN_REACTION_KEYS:
+            raise ValueError(f"unsupported reaction type {rtype_s!r}: {eq}")
+        unknown = set(rd) - _KNOWN_REACTION_KEYS[rtype_s]
+        if unknown:
+            # e.g. 'orders' (explicit reaction orders) or 'negative-A': compiling the
+            # reaction without them would give wrong rates, so refuse it
+            raise ValueError(f"reaction {eq!r}: unsupported field(s) {sorted(unknown)}")
         rev = "<=>" in eq or ("<=" not in eq and "=>" not in eq and "=" in eq)
         if "<=>" in eq:
             lhs, rhs = eq.split("<=>")

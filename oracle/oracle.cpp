@@ -13,8 +13,9 @@
 // BulkKinetics::updateROP, ThirdBodyCalc, FalloffRate/TroeRate, StoichManagerN),
 // restated from Cantera's published algorithms.  SUNDIALS CVODE is replaced by a
 // variable-order BDF (Shampine & Reichelt, "The MATLAB ODE Suite", SIAM J. Sci.
-// Comput. 18 (1997), NDF family with kappa = 0), applied with Ember's tolerances and
-// Ember's finite-difference Jacobian.
+// Comput. 18 (1997), NDF family with kappa = 0) restated from scipy's
+// scipy.integrate._ivp.bdf (SciPy 1.15, BSD-3-Clause), with CVODE's per-step failure
+// bounds, applied with Ember's tolerances and Ember's finite-difference Jacobian.
 //
 // Only the tests, the smoke check and the benchmark's CPU-baseline leg load this.
 #include "oracle.h"
@@ -36,6 +37,10 @@ namespace {
 constexpr double kGasConstant = 6.02214076e26 * 1.380649e-23;  // J/kmol/K (Cantera ct_defs.h)
 constexpr double kSmallNumber = 1e-300;                        // Cantera SmallNumber
 constexpr double kBigNumber = 1e300;                           // Cantera BigNumber
+// failed attempts allowed per BDF step before the point fails (CVODE's defaults maxncf and
+// maxnef, CVodeSetMaxConvFails / CVodeSetMaxErrTestFails)
+constexpr int kBdfMaxConvFail = 10;
+constexpr int kBdfMaxErrFail = 7;
 
 // ---------------------------------------------------------------------------
 // Mechanism tables (written by ember_amd/mechanism.py)
@@ -974,6 +979,9 @@ struct SourceBDF {
             double error_norm = 0, safety = 0;
             int n_iter = 0;
             double t_new = 0;
+            // failed attempts of this step, bounded as CVODE bounds them (maxncf = 10
+            // convergence failures, maxnef = 7 error-test failures per step)
+            int n_conv_fail = 0, n_err_fail = 0;
             while (!accepted) {
                 if (h_abs < min_step) return -3;
                 if (nsteps >= o.max_steps) return -4;
@@ -1044,6 +1052,7 @@ struct SourceBDF {
                 }
                 if (failed_rates) return -2;
                 if (!converged) {
+                    if (++n_conv_fail >= kBdfMaxConvFail) return -8;
                     double factor = 0.5;
                     h_abs *= factor;
                     change_D(D, order, factor);
@@ -1055,7 +1064,11 @@ struct SourceBDF {
                 for (int v = 0; v < N; v++) scale[v] = atol[v] + rtol * std::abs(ynew[v]);
                 for (int v = 0; v < N; v++) err[v] = errc[order] * dvec[v];
                 error_norm = rms(err.data(), scale.data());
+                // a non-finite error estimate (NaN or Inf in the Newton update) fails the
+                // point instead of being taken as a small error
+                if (!std::isfinite(error_norm)) return -6;
                 if (error_norm > 1) {
+                    if (++n_err_fail >= kBdfMaxErrFail) return -7;
                     double factor = std::max(MIN_FACTOR, safety * std::pow(error_norm, -1.0 / (order + 1)));
                     h_abs *= factor;
                     change_D(D, order, factor);
@@ -1101,7 +1114,7 @@ struct SourceBDF {
                 if (std::isnan(factors[j])) factors[j] = 0;
             }
             for (int j = 1; j < 3; j++) if (factors[j] > factors[best]) best = j;
-            order += best - 1;
+            order = std::max(1, order + best - 1);
             double factor = std::min(MAX_FACTOR, safety * factors[best]);
             h_abs *= factor;
             change_D(D, order, factor);
@@ -1371,7 +1384,15 @@ int orc_source_integrate(void* mech, const orc_options* o, int nPoints, const do
             double* yout = state_out + (size_t)j * N;
             int err;
             double qi = 0.0;
-            if (o->integrator == 0) {
+            // Cantera's state setters refuse a non-finite or non-positive temperature and
+            // non-finite mass fractions; the point fails before integrating
+            bool valid = yin[1] > 0.0;
+            for (int i = 0; i < N; i++) valid = valid && std::isfinite(yin[i]);
+            if (!valid) {
+                err = -9;
+                if (stat1) stat1[j] = 0;
+                if (stat2) stat2[j] = 0;
+            } else if (o->integrator == 0) {
                 for (int i = 0; i < N; i++) qss.splitConst[i] = split_const ? split_const[(size_t)j * N + i] : 0.0;
                 qss.qLoss = qloss ? qloss[j] : 0.0;
                 qss.x = o->positions ? o->positions[j] : 0.0;

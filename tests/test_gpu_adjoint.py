@@ -51,8 +51,10 @@ def test_gpu_adjoint_vs_oracle_adjoint(gpu, oracle_mod, name, n, kinetics):
 
 
 def test_gpu_adjoint_vs_finite_differences(gpu, oracle_mod):
-    print("[adjoint] gri30 vs FD", flush=True)
-    st = W.flame_front_states(2, seed=5)
+    """16 flame-front points of the adjoint workload (GRI, c in [0.3, 0.8], seed 5): every
+    gradient component within 1e-4 of central finite differences (h = 1e-6)."""
+    print("[adjoint] gri30 vs FD, 16 points", flush=True)
+    st = W.flame_front_states(16, seed=5)
     ctx = capi.Context("gri30")
     _, _, gs, gk, status = _run_gpu(ctx, st)
     assert (status == 0).all()
@@ -131,3 +133,29 @@ def test_autograd_function(gpu, oracle_mod):
     _close(gx.cpu().numpy()[:, 1:] * st[:, 1:], rgs[:, 1:] * st[:, 1:], 1e-8)
     np.testing.assert_allclose(gk.numpy(), rgk.sum(axis=0), rtol=1e-8,
                                atol=1e-8 * np.abs(rgk.sum(axis=0)).max())
+
+
+def test_autograd_restores_rate_multipliers(gpu, oracle_mod):
+    """QssStage sets exp(ln_mult) for its own calls only: a production stage on the same
+    context afterwards still runs with the multipliers the caller had set."""
+    import torch
+    from ember_amd import adjoint
+    print("[adjoint] multipliers restored", flush=True)
+    name, n = "gri30", 8
+    st = W.flame_front_states(n, seed=5, name=name)
+    ctx = capi.Context(name)
+    mult = np.random.default_rng(3).uniform(0.5, 2.0, ctx.n_reactions)
+    ctx.set_rate_multipliers(mult)
+    d = torch.device("cuda:0")
+    x = torch.from_numpy(st).to(d).requires_grad_(True)
+    lnk = torch.zeros(ctx.n_reactions, dtype=torch.float64, requires_grad=True)
+    out, Q, status = adjoint.qss_stage(x, lnk, DT, ctx)
+    torch.autograd.grad(Q.sum(), (x, lnk))
+    np.testing.assert_array_equal(ctx.rate_multipliers, mult)
+    got, gst, _, _ = ctx.source_integrate(st, DT)
+    om = oracle_mod.Mech(mechanism.table_path(name))
+    om.set_multipliers(mult)
+    ref, rst, *_ = om.integrate(st, DT, oracle_mod.options())
+    assert (gst == 0).all() and (rst == 0).all()
+    np.testing.assert_allclose(got[:, 1], ref[:, 1], rtol=1e-10)
+    np.testing.assert_allclose(got[:, 2:], ref[:, 2:], rtol=1e-6, atol=1e-12)

@@ -1,0 +1,76 @@
+"""The oracle's production stage: failure paths.
+
+A state whose right-hand side is not finite must fail the point with a negative status
+and return at once.  Before the input check and the per-step failure bounds, a NaN
+initial state made the BDF step size NaN (std::min(NaN, tf) is NaN), `h < min_step`
+never held and the attempt loop never ended.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from ember_amd import mechanism
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _h2o2_states(n):
+    return np.load(os.path.join(ROOT, "tests", "golden", "h2o2_tanh_states_4096.npy"))[:n].copy()
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_nan_state_fails_fast(integrator):
+    """T = NaN or a NaN mass fraction: status -9 (input state refused, as Cantera's state
+    setters refuse it), in a child process with a time limit so that a regression cannot
+    hang the suite."""
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, %r)\n"
+        "from oracle import oracle as O\n"
+        "from ember_amd import mechanism\n"
+        "st = np.load(%r)[:3].copy()\n"
+        "st[0, 1] = np.nan; st[1, 5] = np.nan\n"
+        "m = O.Mech(mechanism.table_path('h2o2'))\n"
+        "out, stat, *_ = m.integrate(st, 2e-5, O.options(integrator=%d), nthreads=1)\n"
+        "print(' '.join(str(int(s)) for s in stat))\n"
+        "assert np.array_equal(out[0, 2:], st[0, 2:]) and np.array_equal(out[1, :2], st[1, :2])\n"
+    ) % (ROOT, os.path.join(ROOT, "tests", "golden", "h2o2_tanh_states_4096.npy"), integrator)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    stat = [int(s) for s in r.stdout.split()]
+    assert stat[0] == -9 and stat[1] == -9, stat
+    assert stat[2] == 0, stat  # the finite neighbour is unaffected
+
+
+def test_bdf_temperature_driven_negative_fails():
+    """A split constant of -1e9 K/s drives T through zero inside the stage: the rate
+    constants become NaN there (log T), Newton fails, the step shrinks and the point fails
+    with a negative status in bounded time, keeping its input state."""
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, %r)\n"
+        "from oracle import oracle as O\n"
+        "from ember_amd import mechanism\n"
+        "st = np.load(%r)[:2].copy()\n"
+        "sc = np.zeros_like(st); sc[0, 1] = -1e9\n"
+        "m = O.Mech(mechanism.table_path('h2o2'))\n"
+        "out, stat, *_ = m.integrate(st, 2e-5, O.options(integrator=1), split_const=sc, nthreads=1)\n"
+        "print(' '.join(str(int(s)) for s in stat))\n"
+        "assert np.array_equal(out[0], st[0])\n"
+    ) % (ROOT, os.path.join(ROOT, "tests", "golden", "h2o2_tanh_states_4096.npy"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    stat = [int(s) for s in r.stdout.split()]
+    assert stat[0] < 0 and stat[1] == 0, stat
+
+
+def test_bdf_order_stays_positive_on_workload(oracle_mod):
+    """The failure bounds and the order clamp leave healthy points alone: all 256 seeded
+    GRI states integrate with status 0 and at most integrator_max_steps steps."""
+    from ember_amd import workloads
+    st = workloads.tanh_progress_states(256, seed=2, name="gri30")
+    m = oracle_mod.Mech(mechanism.table_path("gri30"))
+    out, stat, nst, nfe, _ = m.integrate(st, 2e-5, oracle_mod.options(integrator=1), nthreads=0)
+    assert (stat == 0).all()
+    assert (nst > 0).all() and (nfe > nst).all()
