@@ -5,15 +5,19 @@ tolerances.  States: tanh progress-variable blend of CH4/air (phi = 1, 300 K) an
 adiabatic equilibrium, 1% log-normal noise on minor species (seed 2).
 
 A step is one production stage over all points with the inputs resident in HBM.
-Prints one JSON line (metric point-integrations/s).
-
-Other workloads (one JSON line each, not the headline):
-  --workload uneven     N = 65536 states of very uneven stiffness (60% near-equilibrium
-                        burned, 30% flame front, 10% pre-ignition at 900-1200 K, seed 3):
-                        autotuned launch shape, hipGraph-captured stage, against the same
-                        kernel launched one wave per point and the thread-per-point kernel.
-  --workload diffusion  53 species x 65536 points diffusion stage (seeded smooth profiles,
-                        seed 4, FixedValue ends, dt = 2e-5 s, h = dt: 8 BDF1 solves), HBM GB/s.
+Prints one JSON line (metric point-integrations/s).  Beside the headline the same line
+carries one sub-object per other configuration, each with its own roofline, parity and
+CPU rate:
+  bdf        the same 4096 states with the BDF integrator (rtol 1e-6, atol 1e-12)
+  uneven     N = 65536 states of very uneven stiffness (60% near-equilibrium burned, 30%
+             flame front, 10% pre-ignition at 900-1200 K, seed 3): autotuned launch shape,
+             hipGraph-captured stage, against the same kernel launched one wave per point
+             and the thread-per-point kernel
+  diffusion  53 species x 65536 points diffusion stage (seeded smooth profiles, seed 4,
+             FixedValue ends, dt = 2e-5 s, h = dt: 8 BDF1 solves), HBM GB/s
+  thermo     thermodynamic properties of 65536 GRI points, HBM GB/s
+  adjoint    gradients of the integrated heat release on 4096 flame-front states
+--headline-only skips them; --workload NAME runs one of them alone.
 """
 import argparse
 import json
@@ -33,6 +37,12 @@ STAGE_TRAFFIC = 1555 * 2 * 1024 + 2355 * 1024
 DIFFUSION_TRAFFIC_PER_SOLVE = (48650 + 5220) * 2 * 1024 + (42200 + 1696) * 1024
 
 
+METRIC = ("GRI-scale point-integrations/s per split dt @rtol1e-6, batched vs plain GPU; "
+          "adjoint/fwd")
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector peak (AMD-published)
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -43,15 +53,50 @@ def parse():
     p.add_argument("--kinetics", choices=["standard", "interp"], default="standard")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--cpu-repeats", type=int, default=5, help="timed CPU runs (median reported)")
     p.add_argument("--no-plain", action="store_true", help="skip the plain thread-per-point kernel")
     p.add_argument("--wpb", type=int, default=0, help="waves per workgroup (0: library default)")
     p.add_argument("--bpc", type=int, default=0, help="workgroups per CU (0: as many as fit)")
     p.add_argument("--schedule", choices=["cost", "fifo"], default="cost")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
+    p.add_argument("--headline-only", action="store_true",
+                   help="skip the bdf / uneven / diffusion / thermo sub-objects")
     p.add_argument("--workload", choices=["stage", "uneven", "diffusion", "split", "copy", "thermo"],
                    default="stage")
     p.add_argument("--mech", default="gri30", help="mechanism of the split workload")
     return p.parse_args()
+
+
+def cpu_info():
+    """Host CPU of this run: threads used by the CPU legs and how they were chosen.  The
+    thread count is OMP_NUM_THREADS when set (the CPU share the machine gives one GPU),
+    otherwise every core in this process's affinity mask."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = int(omp) if omp and omp.isdigit() and int(omp) > 0 else aff
+    return {"model": model, "nproc": os.cpu_count(), "affinity_cores": aff,
+            "omp_num_threads": omp, "threads_used": threads}
+
+
+def timed_cpu(fn, repeats):
+    """1 warm-up run, then the median of `repeats` timed runs (seconds) and the last result."""
+    fn()
+    ts, out = [], None
+    for _ in range(max(1, repeats)):
+        c0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - c0)
+    ts.sort()
+    return ts[len(ts) // 2], ts, out
 
 
 def main():
@@ -60,23 +105,40 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    import torch
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    if a.workload != "stage":
+        fn = {"uneven": run_uneven, "diffusion": run_diffusion, "copy": run_copy,
+              "split": run_split, "thermo": run_thermo}[a.workload]
+        print(json.dumps(fn(a, dev)))
+        return
+    res = run_stage(a, dev)
+    if not a.headline_only and a.integrator == "qss" and a.kinetics == "standard" and a.points == 4096:
+        sub = argparse.Namespace(**vars(a))
+        sub.points = 4096
+        sub.steps = max(1, min(a.steps, 10))
+        sub.warmup = 1
+        res["bdf"] = run_bdf(sub, dev)
+        sub.points = 65536
+        res["uneven"] = _subobject(run_uneven(sub, dev))
+        res["diffusion"] = _subobject(run_diffusion(sub, dev))
+        res["thermo"] = _subobject(run_thermo(sub, dev))
+    print(json.dumps(res))
+
+
+def _subobject(r):
+    """A standalone workload's result as a sub-object of the headline line."""
+    drop = ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "dtype")
+    return {k: v for k, v in r.items() if k not in drop}
+
+
+def run_stage(a, dev):
     import numpy as np
     import torch
 
     from ember_amd import capi, flops, mechanism, workloads
-
-    torch.cuda.set_device(0)
-    dev = torch.device("cuda:0")
-    if a.workload == "uneven":
-        return run_uneven(a, dev)
-    if a.workload == "diffusion":
-        return run_diffusion(a, dev)
-    if a.workload == "copy":
-        return run_copy(a, dev)
-    if a.workload == "split":
-        return run_split(a, dev)
-    if a.workload == "thermo":
-        return run_thermo(a, dev)
     N = a.points
     integ = capi.EMX_QSS if a.integrator == "qss" else capi.EMX_BDF
     kin = capi.EMX_KIN_STANDARD if a.kinetics == "standard" else capi.EMX_KIN_INTERP
@@ -99,6 +161,7 @@ def main():
     e1 = torch.cuda.Event(enable_timing=True)
 
     def timed(steps):
+        # HIP events on the stream the library launches on (the context stream)
         e0.record(stream)
         for _ in range(steps):
             ctx.source_integrate_device(d_in, d_out, dt, status=status, stat1=s1, stat2=s2)
@@ -117,17 +180,15 @@ def main():
     # cost-ordered queue: each stage takes the points in descending order of the previous
     # stage's right-hand-side evaluation counts
     ms = ms_fifo
-    t0 = time.perf_counter()
-    wall = 0.0
     if a.schedule == "cost":
         ctx.set_schedule(capi.EMX_SCHED_COST)
         ctx.source_integrate_device(d_in, d_out, dt, status=status, stat1=s1, stat2=s2)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        ms = timed(a.steps)
-        print(f"[bench] cost-ordered queue: {ms:.3f} ms/stage", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms = timed(a.steps)
     wall = time.perf_counter() - t0
-    print(f"[bench] timed {a.steps} stages: {ms:.3f} ms/stage", file=sys.stderr, flush=True)
+    print(f"[bench] timed {a.steps} stages ({a.schedule} order): {ms:.3f} ms/stage", file=sys.stderr, flush=True)
     value = N / (ms * 1e-3)
     out = d_out.cpu().numpy()
     # the host-buffer entry point on the same states: copy in, stage, copy out (PCIe included)
@@ -147,12 +208,16 @@ def main():
         ctx.set_kernel_variant(capi.EMX_VARIANT_PLAIN)
         ctx.source_integrate_device(d_in, d_out, dt, status=status, stat1=s1, stat2=s2)
         torch.cuda.synchronize()
-        np_steps = max(1, min(a.steps, 3))
-        ms_plain = timed(np_steps)
+        ms_plain = timed(max(1, min(a.steps, 3)))
+        ctx.set_kernel_variant(capi.EMX_VARIANT_WAVE_PER_POINT)
+        ctx.source_integrate_device(d_in, d_out, dt, status=status, stat1=s1, stat2=s2)
+        torch.cuda.synchronize()
+        ms_pp = timed(max(1, min(a.steps, 5)))
         ctx.set_kernel_variant(capi.EMX_VARIANT_WAVE)
-        print(f"[bench] plain kernel: {ms_plain:.3f} ms/stage", file=sys.stderr, flush=True)
+        print(f"[bench] plain kernel: {ms_plain:.3f} ms/stage, wave-per-point launch {ms_pp:.3f} ms",
+              file=sys.stderr, flush=True)
     else:
-        ms_plain = None
+        ms_plain = ms_pp = None
 
     adj = None
     if not a.no_adjoint and integ == capi.EMX_QSS and kin == capi.EMX_KIN_STANDARD:
@@ -161,7 +226,7 @@ def main():
     mech = mechanism.load("gri30")
     fl = flops.rhs_flops(mech)
     res = {
-        "metric": "GRI-scale point-integrations/s per split dt @rtol1e-6",
+        "metric": METRIC,
         "value": value, "unit": "point-integrations/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded tanh progress-variable states)",
@@ -169,57 +234,152 @@ def main():
                    "mechanism": "gri30 (53 species, 325 reactions)", "points": N, "dt_s": dt,
                    "pressure_Pa": mechanism.ONE_ATM, "seed": 2},
         "failed_points": int((st != 0).sum()),
-        "schedule": a.schedule, "index_order_ms_per_step": ms_fifo,
+        "schedule": a.schedule,
+        "first_stage_index_order": {"ms_per_step": ms_fifo, "value": N / (ms_fifo * 1e-3),
+                                    "note": "work queue in index order, as a stage over new "
+                                            "points runs; the headline is the cost-ordered queue "
+                                            "(previous stage's evaluation counts)"},
         "host_buffer_ms_per_step": ms_e2e,
     }
     if ms_plain:
         res["plain_gpu"] = {"value": N / (ms_plain * 1e-3), "ms_per_step": ms_plain,
-                            "speedup": ms_plain / ms}
+                            "speedup": ms_plain / ms,
+                            "note": "one thread per point, 64 wavefronts at N = 4096"}
+        res["wave_per_point_launch"] = {"ms_per_step": ms_pp, "speedup": ms_pp / ms,
+                                        "note": "the same wavefront kernel, one wave per point, "
+                                                "no work queue"}
 
-    cpu = None
     if not a.no_cpu_baseline:
         from oracle import oracle as O
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        info = cpu_info()
+        threads = a.cpu_threads or info["threads_used"]
         om = O.Mech(mechanism.table_path("gri30"))
         oopts = O.options(integrator=integ, kinetics=kin,
                           **({"atol_species": 1e-12} if integ == capi.EMX_BDF else {}))
-        print(f"[bench] CPU oracle on {N} points, {threads} threads", file=sys.stderr, flush=True)
-        om.integrate(states[:64], dt, oopts, nthreads=threads)  # warm-up
-        c0 = time.perf_counter()
-        ref, rst, rg, rr, _ = om.integrate(states, dt, oopts, nthreads=threads)
-        cpu_s = time.perf_counter() - c0
+        print(f"[bench] CPU oracle on {N} points, {threads} threads, {info['model']}", file=sys.stderr, flush=True)
+        cpu_s, runs, (ref, rst, rg, rr, _) = timed_cpu(
+            lambda: om.integrate(states, dt, oopts, nthreads=threads), a.cpu_repeats)
         cpu = {"value": N / cpu_s, "unit": "point-integrations/s", "cores": threads,
-               "kind": "port", "sample": f"all {N} benchmark states, one stage, OpenMP dynamic,1 "
-                                          f"(TBB grain-1 analogue), {cpu_s:.2f} s"}
-        # one core, on the first 128 states
-        c0 = time.perf_counter()
-        om.integrate(states[:128], dt, oopts, nthreads=1)
-        cpu1_s = time.perf_counter() - c0
-        cpu["one_core"] = {"value": 128 / cpu1_s, "sample": f"first 128 benchmark states, {cpu1_s:.2f} s"}
+               "kind": "port",
+               "sample": f"all {N} benchmark states, one stage, OpenMP dynamic,1 (TBB grain-1 "
+                         f"analogue); median of {len(runs)} runs after 1 warm-up: {cpu_s:.3f} s",
+               "runs_s": runs, "host": info}
+        n1 = 128
+        cpu1_s, runs1, _ = timed_cpu(lambda: om.integrate(states[:n1], dt, oopts, nthreads=1),
+                                     a.cpu_repeats)
+        cpu["one_core"] = {"value": n1 / cpu1_s, "cores": 1,
+                           "sample": f"first {n1} benchmark states; median of {len(runs1)} runs: "
+                                     f"{cpu1_s:.3f} s"}
         res["cpu_baseline"] = cpu
         # parity of this run
         dT = np.abs(out[:, 1] - ref[:, 1]) / np.abs(ref[:, 1])
         dY = np.abs(out[:, 2:] - ref[:, 2:]) / (np.abs(ref[:, 2:]) + 1e-6)
-        # the tolerance of the stage contract: |dT| <= 1e-6 |T|, |dY| <= 1e-6 |Y| + 1e-12
+        # the tolerance of the stage contract: |dT| <= 1e-6 |T|, |dY| <= 1e-6 |Y| + floor
+        floor = 5e-11 if integ == capi.EMX_BDF else 1e-12
         ok_T = bool((np.abs(out[:, 1] - ref[:, 1]) <= 1e-6 * np.abs(ref[:, 1])).all())
-        ok_Y = bool((np.abs(out[:, 2:] - ref[:, 2:]) <= 1e-6 * np.abs(ref[:, 2:]) + 1e-12).all())
+        ok_Y = bool((np.abs(out[:, 2:] - ref[:, 2:]) <= 1e-6 * np.abs(ref[:, 2:]) + floor).all())
         res["parity"] = {"max_rel_T": float(dT.max()), "max_rel_Y_floor1e-6": float(dY.max()),
-                         "status_equal": bool((st == rst).all()),
-                         "within_rtol1e-6_atol1e-12": ok_T and ok_Y}
+                         "status_equal": bool((st == rst).all()), "y_floor": floor,
+                         "within_rtol1e-6": ok_T and ok_Y}
         g, r = rg.astype(np.float64), rr.astype(np.float64)
 
     if integ == capi.EMX_QSS:
         F = float(flops.qss_point_flops(fl, g, r).sum())
-        res["roofline"] = {"bound": "mfma", "achieved": F / (ms * 1e-3) / 1e12, "peak": 78.6,
-                           "unit": "TFLOP/s", "frac": F / (ms * 1e-3) / 1e12 / 78.6,
-                           "traffic": STAGE_TRAFFIC if (integ == capi.EMX_QSS and kin == capi.EMX_KIN_STANDARD
-                                                        and N == 4096) else None,
-                           "flops_per_launch": F, "note": "FP64 peak; exp/log count 1 op (lower bound)"}
+        res["roofline"] = {"bound": "fp64-vector", "achieved": F / (ms * 1e-3) / 1e12,
+                           "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": F / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                           "traffic": STAGE_TRAFFIC if (kin == capi.EMX_KIN_STANDARD and N == 4096) else None,
+                           "flops_per_launch": F,
+                           "note": "FP64 vector (VALU) peak; no MFMA is used; exp/log count 1 op "
+                                   "(lower bound); FLOPs from the oracle's evaluation counts"}
         res["rhs_evals_per_point"] = {"mean": float(g.mean()), "max": float(g.max()), "min": float(g.min())}
     if adj:
         res["adjoint"] = adj
     res["wall_s"] = wall
-    print(json.dumps(res))
+    return res
+
+
+def bdf_flops(fl, work, n_vars):
+    """Algorithmic FP64 operations of the BDF stage from the oracle's per-point work: every
+    right-hand side (Newton iterations and finite-difference Jacobian columns) at the full
+    count, (2/3) n^3 per LU factorisation, 2 n^2 per pair of triangular solves."""
+    import numpy as np
+    w = work.astype(np.float64)
+    return float((w[:, 0] * fl.full + w[:, 2] * (2.0 / 3.0) * n_vars ** 3
+                  + w[:, 3] * 2.0 * n_vars ** 2).sum())
+
+
+def run_bdf(a, dev):
+    """The 4096 benchmark states with the BDF integrator (SourceSystemCVODE semantics,
+    rtol 1e-6, species atol 1e-12)."""
+    import numpy as np
+    import torch
+
+    from ember_amd import capi, flops, mechanism, workloads
+    N, dt = a.points, 2e-5
+    states = workloads.tanh_progress_states(N, seed=2, name="gri30")
+    ctx = capi.Context("gri30", chemistry_integrator=capi.EMX_BDF, integrator_species_abs_tol=1e-12)
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+    d_in = torch.from_numpy(states).to(dev)
+    d_out = torch.empty_like(d_in)
+    status = torch.zeros(N, dtype=torch.int32, device=dev)
+    call = lambda: ctx.source_integrate_device(d_in, d_out, dt, status=status)
+    ctx.set_schedule(capi.EMX_SCHED_FIFO)
+    call()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    call()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_first = e0.elapsed_time(e1)
+    ctx.set_schedule(capi.EMX_SCHED_COST)
+    call()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(a.steps):
+        call()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    print(f"[bench] BDF stage: {ms:.3f} ms (index order {ms_first:.3f} ms)", file=sys.stderr, flush=True)
+    out, st = d_out.cpu().numpy(), status.cpu().numpy()
+    res = {"workload": "GRI-Mech 3.0 chemistry stage, BDF (rtol 1e-6, atol 1e-12), the "
+                       "4096 benchmark states",
+           "value": N / (ms * 1e-3), "unit": "point-integrations/s", "ms_per_step": ms,
+           "steps": a.steps, "first_stage_index_order_ms": ms_first,
+           "failed_points": int((st != 0).sum())}
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        info = cpu_info()
+        threads = a.cpu_threads or info["threads_used"]
+        om = O.Mech(mechanism.table_path("gri30"))
+        oo = O.options(integrator=1, atol_species=1e-12)
+        cpu_s, runs, (ref, rst, *_) = timed_cpu(lambda: om.integrate(states, dt, oo, nthreads=threads),
+                                                min(a.cpu_repeats, 3))
+        work = om.bdf_work(states, dt, oo, nthreads=threads)
+        F = bdf_flops(flops.rhs_flops(mechanism.load("gri30")), work, states.shape[1])
+        ok_T = bool((np.abs(out[:, 1] - ref[:, 1]) <= 1e-6 * np.abs(ref[:, 1])).all())
+        ok_Y = bool((np.abs(out[:, 2:] - ref[:, 2:]) <= 1e-6 * np.abs(ref[:, 2:]) + 5e-11).all())
+        res["parity"] = {"max_rel_T": float((np.abs(out[:, 1] - ref[:, 1]) / ref[:, 1]).max()),
+                         "max_abs_Y_excess": float((np.abs(out[:, 2:] - ref[:, 2:])
+                                                    - 1e-6 * np.abs(ref[:, 2:])).max()),
+                         "status_equal": bool((st == rst).all()), "y_floor": 5e-11,
+                         "within_rtol1e-6": ok_T and ok_Y}
+        res["cpu_baseline"] = {"value": N / cpu_s, "unit": "point-integrations/s", "cores": threads,
+                               "kind": "port", "sample": f"all {N} states, median of {len(runs)} "
+                                                         f"runs after 1 warm-up: {cpu_s:.3f} s"}
+        res["work_per_point"] = {"rhs": float(work[:, 0].mean()), "jacobians": float(work[:, 1].mean()),
+                                 "lu": float(work[:, 2].mean()), "solves": float(work[:, 3].mean())}
+        res["roofline"] = {"bound": "fp64-vector", "achieved": F / (ms * 1e-3) / 1e12,
+                           "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": F / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
+                           "flops_per_launch": F,
+                           "note": "RHS at the full count, LU 2/3 n^3, solves 2 n^2; work counts "
+                                   "from the oracle on the same states"}
+    return res
 
 
 def _pct(x):
@@ -323,15 +483,36 @@ def run_uneven(a, dev):
         "wave_per_point_launch": {"ms_per_step": ms_pp, "speedup": ms_pp / ms_graph,
                                   "results_equal": pp_equal},
         "rhs_evals_per_point": dict(_pct(counts), spread=float(counts.max() / counts.min())),
-        "roofline": {"bound": "mfma", "achieved": F / (ms_graph * 1e-3) / 1e12, "peak": 78.6,
-                     "unit": "TFLOP/s", "frac": F / (ms_graph * 1e-3) / 1e12 / 78.6,
+        "roofline": {"bound": "fp64-vector", "achieved": F / (ms_graph * 1e-3) / 1e12,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": F / (ms_graph * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
                      "traffic": None, "flops_per_launch": F,
-                     "note": "FLOPs from this run's evaluation counts; exp/log count 1 op"},
+                     "note": "FLOPs from this run's evaluation counts (equal to the oracle's "
+                             "when parity holds); exp/log count 1 op"},
     }
     if ms_plain:
         res["plain_gpu"] = {"ms_per_step": ms_plain, "value": N / (ms_plain * 1e-3),
                             "speedup": ms_plain / ms_graph}
-    print(json.dumps(res))
+    if not a.no_cpu_baseline:
+        # parity of the captured stage over all N points, and the CPU rate of the same run
+        from oracle import oracle as O
+        info = cpu_info()
+        threads = a.cpu_threads or info["threads_used"]
+        om = O.Mech(mechanism.table_path("gri30"))
+        print(f"[bench] uneven: CPU oracle on {N} points, {threads} threads", file=sys.stderr, flush=True)
+        c0 = time.perf_counter()
+        ref, rst, rg, rr, _ = om.integrate(st, dt, O.options(), nthreads=threads)
+        cpu_s = time.perf_counter() - c0
+        got = ref_out.cpu().numpy()
+        ok_T = bool((np.abs(got[:, 1] - ref[:, 1]) <= 1e-6 * np.abs(ref[:, 1])).all())
+        ok_Y = bool((np.abs(got[:, 2:] - ref[:, 2:]) <= 1e-6 * np.abs(ref[:, 2:]) + 1e-12).all())
+        res["parity"] = {"max_rel_T": float((np.abs(got[:, 1] - ref[:, 1]) / ref[:, 1]).max()),
+                         "status_equal": bool((ref_status.cpu().numpy() == rst).all()),
+                         "evaluation_counts_equal": bool((counts == rg).all()),
+                         "within_rtol1e-6_atol1e-12": ok_T and ok_Y}
+        res["cpu_baseline"] = {"value": N / cpu_s, "unit": "point-integrations/s", "cores": threads,
+                               "kind": "port", "sample": f"all {N} states, one run: {cpu_s:.2f} s"}
+    return res
 
 
 def run_diffusion(a, dev):
@@ -390,17 +571,18 @@ def run_diffusion(a, dev):
     }
     if not a.no_cpu_baseline:
         from oracle import oracle as O
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        c0 = time.perf_counter()
-        ref = O.diffusion_integrate(case.B, case.B_row, case.D, np.zeros_like(case.y0), case.x,
-                                    case.y0, 0.0, dt, h_per_var=h, nthreads=threads)
-        cpu_s = time.perf_counter() - c0
+        threads = a.cpu_threads or cpu_info()["threads_used"]
+        cpu_s, runs, ref = timed_cpu(
+            lambda: O.diffusion_integrate(case.B, case.B_row, case.D, np.zeros_like(case.y0), case.x,
+                                          case.y0, 0.0, dt, h_per_var=h, nthreads=threads),
+            a.cpu_repeats)
         err = np.abs(got - ref).max() / np.abs(ref).max()
         res["cpu_baseline"] = {"value": algo * solves / cpu_s / 1e9, "unit": "GB/s",
                                "cores": threads, "kind": "port",
-                               "sample": f"the full stage, Thomas per species in parallel, {cpu_s:.2f} s"}
+                               "sample": f"the full stage, Thomas per species in parallel; median "
+                                         f"of {len(runs)} runs after 1 warm-up: {cpu_s:.3f} s"}
         res["parity"] = {"max_abs_err_over_max": float(err)}
-    print(json.dumps(res))
+    return res
 
 
 def run_split(a, dev):
@@ -487,7 +669,7 @@ def run_split(a, dev):
         "host_driven_ms_per_step": ms_host, "speedup_vs_host_driven": ms_host / ms_dev,
         "max_abs_diff_vs_host_driven": float(np.abs(got - ref).max()),
     }
-    print(json.dumps(res))
+    return res
 
 
 def run_thermo(a, dev):
@@ -537,20 +719,18 @@ def run_thermo(a, dev):
     if not a.no_cpu_baseline:
         from oracle import oracle as O
         from ember_amd import mechanism
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = a.cpu_threads or cpu_info()["threads_used"]
         om = O.Mech(mechanism.table_path("gri30"))
-        om.thermo_properties(st[:64], nthreads=threads)
-        c0 = time.perf_counter()
-        ref = om.thermo_properties(st, nthreads=threads)
-        cpu_s = time.perf_counter() - c0
+        cpu_s, runs, ref = timed_cpu(lambda: om.thermo_properties(st, nthreads=threads), a.cpu_repeats)
         got = {"rho": rho, "Wmx": Wmx, "cp": cp, "cpSpec": cpk, "hk": hk}
         err = {k: float(np.abs(v.cpu().numpy() - ref[k]).max() / np.abs(ref[k]).max())
                for k, v in got.items()}
         res["cpu_baseline"] = {"value": algo / cpu_s / 1e9, "unit": "GB/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"all {N} points, OpenMP dynamic,1, {cpu_s * 1e3:.1f} ms"}
+                               "sample": f"all {N} points, OpenMP dynamic,1; median of {len(runs)} "
+                                         f"runs after 1 warm-up: {cpu_s * 1e3:.1f} ms"}
         res["parity"] = {"max_abs_err_over_max": err}
-    print(json.dumps(res))
+    return res
 
 
 def run_copy(a, dev):
@@ -572,11 +752,11 @@ def run_copy(a, dev):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / a.steps
     gbs = 2 * n * 8 / (ms * 1e-3) / 1e9
-    print(json.dumps({"metric": "device copy GB/s (512 MiB read + 512 MiB written)", "value": gbs,
+    return ({"metric": "device copy GB/s (512 MiB read + 512 MiB written)", "value": gbs,
                       "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
                       "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
                       "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-                      "config": {"workload": "torch copy_ of 512 MiB"}}))
+                      "config": {"workload": "torch copy_ of 512 MiB"}})
 
 
 def adjoint_cost(ctx, N, dt, dev, stream, steps):

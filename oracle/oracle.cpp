@@ -804,7 +804,7 @@ struct SourceBDF {
     std::vector<double> splitConst, Yv, hk, wdot, atol;
     double qLoss = 0, qDot = 0;
     double x = 0;  // SourceSystem::setPosition
-    int nfe = 0, nsteps = 0, njac = 0;
+    int nfe = 0, nsteps = 0, njac = 0, nlu = 0, nsolve = 0;
     bool failed_rates = false;
 
     SourceBDF(Gas& g, const orc_options& opt)
@@ -1013,6 +1013,7 @@ struct SourceBDF {
                         for (int i = 0; i < N; i++)
                             for (int j = 0; j < N; j++)
                                 LU[(size_t)i * N + j] = (i == j ? 1.0 : 0.0) - c * J[(size_t)j * N + i];
+                        nlu++;
                         if (!lu_factor(N, LU, piv)) return -5;
                         lu_valid = true;
                     }
@@ -1028,6 +1029,7 @@ struct SourceBDF {
                         for (int v = 0; v < N; v++) finite = finite && std::isfinite(fy[v]);
                         if (!finite) break;
                         for (int v = 0; v < N; v++) dy[v] = c * fy[v] - psi[v] - dvec[v];
+                        nsolve++;
                         lu_solve(N, LU, piv, dy.data());
                         double dy_norm = rms(dy.data(), scale.data());
                         double rate = (dy_norm_old < 0) ? -1.0 : dy_norm / dy_norm_old;
@@ -1418,6 +1420,41 @@ int orc_source_integrate(void* mech, const orc_options* o, int nPoints, const do
             if (err < 0) std::copy(yin, yin + N, yout);  // keep the old state
             if (status) status[j] = err;
             if (qint) qint[j] = qi;
+        }
+    }
+    return 0;
+}
+
+int orc_bdf_work(void* mech, const orc_options* o, int nPoints, const double* state_in, double dt,
+                 int* work, int nthreads) {
+    Mech& m = *static_cast<Mech*>(mech);
+    const int N = m.K + 2;
+    if (o->kinetics == 1) return -1;
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        Gas gas(m, o->kinetics, o->pressure);
+        SourceBDF bdf(gas, *o);
+        bdf.tStart = o->t_start;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int j = 0; j < nPoints; j++) {
+            std::fill(bdf.splitConst.begin(), bdf.splitConst.end(), 0.0);
+            bdf.qLoss = 0.0;
+            bdf.x = o->positions ? o->positions[j] : 0.0;
+            bdf.failed_rates = false;
+            bdf.nfe = bdf.nsteps = bdf.njac = bdf.nlu = bdf.nsolve = 0;
+            std::vector<double> y(state_in + (size_t)j * N, state_in + (size_t)(j + 1) * N);
+            double qi = 0.0;
+            int err = bdf.integrate(y.data(), dt, &qi);
+            int* w = work + (size_t)j * 4;
+            w[0] = err < 0 ? -1 : bdf.nfe;
+            w[1] = bdf.njac;
+            w[2] = bdf.nlu;
+            w[3] = bdf.nsolve;
         }
     }
     return 0;

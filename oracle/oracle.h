@@ -81,6 +81,13 @@ int orc_source_integrate(void* mech, const orc_options* o, int nPoints,
                          const double* qloss, double dt, double* state_out,
                          int* status, int* stat1, int* stat2, double* qint, int nthreads);
 
+/* Work counts of the BDF stage per point (zero split constants and heat loss), for the
+ * benchmark's FLOP count: work[4 j ..] = right-hand-side evaluations (including the
+ * finite-difference Jacobian columns; -1 for a failed point), Jacobians, LU
+ * factorisations, linear solves.  Standard kinetics only (-1 otherwise). */
+int orc_bdf_work(void* mech, const orc_options* o, int nPoints, const double* state_in, double dt,
+                 int* work, int nthreads);
+
 /* QssIntegrator on a test ODE.  problem 0: CHEMEQ2 cesium problem (n = 6);
  * problem 1: y' = qc - kc*y (n components).  y in/out.  Returns the integrator status. */
 int orc_qss_test_problem(int problem, int n, const double* qc, const double* kc, double epsmin,

@@ -61,6 +61,8 @@ def lib():
                                             _dp, _dp, C.c_int]
         L.orc_source_integrate.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, _dp,
                                            C.c_double, _dp, _ip, _ip, _ip, _dp, C.c_int]
+        L.orc_bdf_work.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, C.c_double, _ip,
+                                   C.c_int]
         L.orc_qss_test_problem.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double,
                                            C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
                                            C.c_int, C.c_double, _dp, _ip, _ip]
@@ -182,6 +184,18 @@ class Mech:
         lib().orc_source_integrate(self.h, C.byref(opts), n, _p(state), _p(sc), _p(ql), dt, _p(out),
                                    _p(status), _p(s1), _p(s2), _p(qint), nthreads)
         return out, status, s1, s2, qint
+
+    def bdf_work(self, state, dt, opts=None, nthreads=0):
+        """BDF work per point: (n_rhs, n_jac, n_lu, n_solve) columns (orc_bdf_work)."""
+        opts = opts or options(integrator=1)
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        n = state.shape[0]
+        w = np.zeros((n, 4), dtype=np.int32)
+        rc = lib().orc_bdf_work(self.h, C.byref(opts), n, _p(state), dt,
+                                w.ctypes.data_as(_ip), nthreads)
+        if rc:
+            raise RuntimeError(f"orc_bdf_work failed: {rc}")
+        return w
 
     def adjoint(self, state, dt, opts=None, Qbar=None, ybar=None, split_const=None, qloss=None,
                 nthreads=0):

@@ -1,8 +1,11 @@
 """Chemistry stages at the benchmark configurations' full sizes against the oracle.
 
 - QSS and BDF on all 4096 seeded tanh-progress GRI-Mech 3.0 states (seed 2, dt = 2e-5 s,
-  P = 1 atm): T and Y_k within |d| <= 1e-6 |ref| + floor (floor 1e-12 for QSS, 1e-11 for
-  the BDF whose species absolute tolerance is 1e-12), statuses equal.
+  P = 1 atm): T and Y_k within |d| <= 1e-6 |ref| + floor, statuses equal.  Floor 1e-12
+  for QSS.  For the BDF (species absolute tolerance 1e-12) the floor is 5e-11: its
+  accept/reject decisions make Y_k sensitive to rounding at that level; the oracle fed the
+  same states perturbed by one ulp moves Y_k by up to 1.2e-11
+  (tests/test_oracle_bdf.py::test_bdf_rounding_sensitivity).
 - The uneven-stiffness workload (60 % burned, 30 % flame front, 10 % pre-ignition, seed 3)
   on its first 8192 points, eager and replayed from a captured HIP graph, against the
   oracle; the graph replays equal the eager stage bit for bit.
@@ -60,7 +63,7 @@ def test_bdf_full_config(gpu, oracle_mod, gri_states, gri_oracle):
     ref, rstat, rnst, rnfe, _ = gri_oracle.integrate(
         gri_states, DT, oracle_mod.options(integrator=1, atol_species=1e-12), nthreads=NTHREADS)
     assert (status == 0).all() and (rstat == 0).all()
-    _assert_close(out, ref, "bdf4096", atol_y=1e-11)
+    _assert_close(out, ref, "bdf4096", atol_y=5e-11)
     assert np.mean(nst == rnst) > 0.8
 
 

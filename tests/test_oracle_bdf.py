@@ -74,3 +74,24 @@ def test_bdf_order_stays_positive_on_workload(oracle_mod):
     out, stat, nst, nfe, _ = m.integrate(st, 2e-5, oracle_mod.options(integrator=1), nthreads=0)
     assert (stat == 0).all()
     assert (nst > 0).all() and (nfe > nst).all()
+
+
+def test_bdf_rounding_sensitivity(oracle_mod):
+    """How far rounding alone moves the BDF's result on the GRI benchmark states: the
+    stage on 1024 of the seeded states, and again with every mass fraction perturbed by at
+    most one ulp.  T moves by < 1e-9 relative and Y_k by < 1e-6 |Y_k| + 5e-11, the bar of
+    the GPU-vs-oracle BDF comparison at 4096 states; about 1e-11 is reached, so a floor of
+    1e-11 would sit at the noise of the method itself."""
+    from ember_amd import workloads
+    st = workloads.tanh_progress_states(4096, seed=2, name="gri30")[2048:3072]
+    m = oracle_mod.Mech(mechanism.table_path("gri30"))
+    o = oracle_mod.options(integrator=1, atol_species=1e-12)
+    a, sa, na, *_ = m.integrate(st, 2e-5, o, nthreads=0)
+    pert = st.copy()
+    pert[:, 2:] *= 1 + np.random.default_rng(0).uniform(-1, 1, pert[:, 2:].shape) * 2.2e-16
+    b, sb, nb, *_ = m.integrate(pert, 2e-5, o, nthreads=0)
+    assert (sa == 0).all() and (sb == 0).all()
+    assert (np.abs(a[:, 1] - b[:, 1]) < 1e-9 * a[:, 1]).all()
+    excess = np.abs(a[:, 2:] - b[:, 2:]) - 1e-6 * np.abs(a[:, 2:])
+    assert excess.max() < 5e-11
+    assert excess.max() > 1e-12  # rounding does reach well above the species atol
