@@ -66,6 +66,9 @@ SIGNATURES = [
     ("emx_set_options", C.c_int, [_vp, C.POINTER(Options)]),
     ("emx_set_rate_multipliers", C.c_int, [_vp, _dp]),
     ("emx_set_rate_multiplier_function", C.c_int, [_vp, C.c_int, _dp]),
+    ("emx_set_interp_range", C.c_int, [_vp, C.c_int, C.c_double, C.c_double]),
+    ("emx_get_interp_range", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]),
     ("emx_set_point_positions", C.c_int, [_vp, _sz, _dp]),
     ("emx_set_stream", C.c_int, [_vp, _vp]),
     ("emx_set_launch", C.c_int, [_vp, C.c_int, C.c_int]),
@@ -202,6 +205,16 @@ class Context:
             return
         d = np.ascontiguousarray(data, dtype=np.float64)
         _check(lib().emx_set_rate_multiplier_function(self.h, len(d), _p(d)))
+
+    def set_interp_range(self, n: int, Tmin: float, Tmax: float):
+        """InterpKinetics table grid: n temperatures over [Tmin, Tmax] (emx_set_interp_range)."""
+        _check(lib().emx_set_interp_range(self.h, int(n), float(Tmin), float(Tmax)))
+
+    def interp_range(self):
+        n = C.c_int()
+        lo, hi = C.c_double(), C.c_double()
+        _check(lib().emx_get_interp_range(self.h, C.byref(n), C.byref(lo), C.byref(hi)))
+        return n.value, lo.value, hi.value
 
     def set_point_positions(self, x):
         """Grid positions of the stage's points (SourceSystem::setPosition), for the

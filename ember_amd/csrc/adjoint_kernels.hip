@@ -156,7 +156,7 @@ __device__ __forceinline__ bool rhs_vjp(const DevMech& m, int lane, double yy, b
     // ---- reaction slots: rates of progress -> base rate, rate constants, concentrations
     const double logT = emx_log(T), recipT = 1.0 / T;
     size_t nI = 0;
-    if (INTERP) nI = (size_t)floor((T - kInterpTmin) / (kInterpTmax - kInterpTmin) * (kInterpN - 1));
+    if (INTERP) nI = (size_t)floor((T - m.interpTmin) / (m.interpTmax - m.interpTmin) * (m.interpN - 1));
     double Trl = 0.0;  // this lane's T adjoint contributions
     const double* C = L.C;
 #pragma unroll

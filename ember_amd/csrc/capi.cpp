@@ -1,6 +1,7 @@
 // C-ABI of the ember_amd library (declared in include/ember_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -98,6 +99,10 @@ struct emx_context {
     std::vector<double> xpos;
     DevBuf qign;
     bool qign_valid = false;
+    // InterpKinetics table grid (Ember's defaults, grown by emx_set_interp_range or by the
+    // host-array stage entry point)
+    int interp_n = kInterpN;
+    double interp_tmin = kInterpTmin, interp_tmax = kInterpTmax;
 };
 
 namespace {
@@ -159,7 +164,11 @@ int upload_mech(emx_context* c) {
     d.Pref = m.Pref;
     d.lnRPref = std::log(kGasConstant / m.Pref);
     std::vector<double> kf_c, kf_s, kc_c, kc_s;
-    if (d.interp) build_interp_tables(m, L, c->P, kf_c, kf_s, kc_c, kc_s);
+    d.interpN = c->interp_n;
+    d.interpTmin = c->interp_tmin;
+    d.interpTmax = c->interp_tmax;
+    if (d.interp)
+        build_interp_tables(m, L, c->P, c->interp_n, c->interp_tmin, c->interp_tmax, kf_c, kf_s, kc_c, kc_s);
     std::vector<int32_t> aux;
     std::vector<double> auxd;
     build_pressure_tables(m, L, c->P, aux, auxd);
@@ -205,7 +214,11 @@ int upload_mech(emx_context* c) {
     pk.add(kf_s, &d.kf_s);
     pk.add(kc_c, &d.kc_c);
     pk.add(kc_s, &d.kc_s);
-    if (c->blob) (void)hipFree(c->blob);
+    if (c->blob) {
+        // stages queued on the context stream still read the old tables
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        (void)hipFree(c->blob);
+    }
     c->blob = nullptr;
     HIP_TRY(hipMalloc(&c->blob, pk.bytes.size()));
     HIP_TRY(hipMemcpy(c->blob, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice));
@@ -366,6 +379,24 @@ int emx_set_options(emx_context* c, const emx_options* o) {
     c->dopts = to_dev(*o);
     c->qign_valid = false;
     if (interp_changed) return upload_mech(c);
+    return 0;
+}
+
+int emx_set_interp_range(emx_context* c, int n, double Tmin, double Tmax) {
+    if (int rc = check_ctx(c)) return rc;
+    if (n < 2 || n > 100000 || !(Tmin > 0.0) || !(Tmax > Tmin))
+        return fail(-1, "interpolation grid needs n >= 2 and 0 < Tmin < Tmax");
+    c->interp_n = n;
+    c->interp_tmin = Tmin;
+    c->interp_tmax = Tmax;
+    return c->opts.kinetics_model == EMX_KIN_INTERP ? upload_mech(c) : 0;
+}
+
+int emx_get_interp_range(const emx_context* c, int* n, double* Tmin, double* Tmax) {
+    if (!c) return fail(-1, "null context");
+    if (n) *n = c->interp_n;
+    if (Tmin) *Tmin = c->interp_tmin;
+    if (Tmax) *Tmax = c->interp_tmax;
     return 0;
 }
 
@@ -607,6 +638,68 @@ int emx_source_adjoint_backward(emx_context* c, size_t nPoints, size_t nVars, co
     return 0;
 }
 
+// InterpKinetics::updateROP grows its tables when T leaves them (src/chemistry0d.cpp:
+// 279-291): 10 more nodes and 100 K more above, or 2 more nodes and a lower bound of
+// max(T - 20, 10) K below.  The host-array stage applies the same growth to the points
+// that failed with -2 (T outside the table) and integrates them again from their input
+// states, up to 40 growth steps.  The grown tables stay for later stages, as Ember's do.
+static int interp_range_retry(emx_context* c, size_t nPoints, size_t nVars, const double* state_in,
+                              const double* split_const, const double* qloss, double t0, double dt,
+                              double* state_out, std::vector<int32_t>& stat,
+                              std::vector<int32_t>& g1, std::vector<int32_t>& g2) {
+    hipStream_t st = c->stream;
+    for (int attempt = 0; attempt < 40; attempt++) {
+        std::vector<size_t> idx;
+        double Tlow = c->interp_tmin;
+        for (size_t j = 0; j < nPoints; j++)
+            if (stat[j] == -2) {
+                idx.push_back(j);
+                Tlow = std::min(Tlow, state_in[j * nVars + 1]);
+            }
+        if (idx.empty()) return 0;
+        if (Tlow < c->interp_tmin) {
+            c->interp_n += 2;
+            c->interp_tmin = std::max(Tlow - 20.0, 10.0);
+        } else {
+            c->interp_n += 10;
+            c->interp_tmax += 100.0;
+        }
+        if (int rc = upload_mech(c)) return rc;
+        const size_t n = idx.size();
+        std::vector<double> sin(n * nVars), sc(split_const ? n * nVars : 0), ql(qloss ? n : 0);
+        for (size_t i = 0; i < n; i++) {
+            memcpy(&sin[i * nVars], state_in + idx[i] * nVars, nVars * 8);
+            if (split_const) memcpy(&sc[i * nVars], split_const + idx[i] * nVars, nVars * 8);
+            if (qloss) ql[i] = qloss[idx[i]];
+        }
+        HIP_TRY(hipMemcpyAsync(c->b_in.p, sin.data(), n * nVars * 8, hipMemcpyHostToDevice, st));
+        if (split_const) HIP_TRY(hipMemcpyAsync(c->b_sc.p, sc.data(), n * nVars * 8, hipMemcpyHostToDevice, st));
+        if (qloss) HIP_TRY(hipMemcpyAsync(c->b_ql.p, ql.data(), n * 8, hipMemcpyHostToDevice, st));
+        c->cost_n = 0;  // the recorded costs are for the full point set
+        int rc = emx_source_integrate_device(c, n, nVars, c->b_in.as<double>(),
+                                             split_const ? c->b_sc.as<double>() : nullptr,
+                                             qloss ? c->b_ql.as<double>() : nullptr, t0, dt,
+                                             c->b_out.as<double>(), c->b_status.as<int32_t>(),
+                                             c->b_s1.as<int32_t>(), c->b_s2.as<int32_t>(), nullptr);
+        if (rc) return rc;
+        std::vector<double> sout(n * nVars);
+        std::vector<int32_t> s0(n), s1(n), s2(n);
+        HIP_TRY(hipMemcpyAsync(sout.data(), c->b_out.p, n * nVars * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(s0.data(), c->b_status.p, n * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(s1.data(), c->b_s1.p, n * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(s2.data(), c->b_s2.p, n * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        c->cost_n = 0;
+        for (size_t i = 0; i < n; i++) {
+            memcpy(state_out + idx[i] * nVars, &sout[i * nVars], nVars * 8);
+            stat[idx[i]] = s0[i];
+            g1[idx[i]] = s1[i];
+            g2[idx[i]] = s2[i];
+        }
+    }
+    return 0;
+}
+
 int emx_source_integrate(emx_context* c, size_t nPoints, size_t nVars, const double* state_in,
                          const double* split_const, const double* qloss, double t_stage_start,
                          double dt, double* state_out, int32_t* status, int32_t* stat1,
@@ -641,10 +734,18 @@ int emx_source_integrate(emx_context* c, size_t nPoints, size_t nVars, const dou
                                          c->b_s2.as<int32_t>(), nullptr);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(state_out, c->b_out.p, bytes, hipMemcpyDeviceToHost, st));
-    if (status) HIP_TRY(hipMemcpyAsync(status, c->b_status.p, nPoints * 4, hipMemcpyDeviceToHost, st));
-    if (stat1) HIP_TRY(hipMemcpyAsync(stat1, c->b_s1.p, nPoints * 4, hipMemcpyDeviceToHost, st));
-    if (stat2) HIP_TRY(hipMemcpyAsync(stat2, c->b_s2.p, nPoints * 4, hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> stat(nPoints), g1(nPoints), g2(nPoints);
+    HIP_TRY(hipMemcpyAsync(stat.data(), c->b_status.p, nPoints * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(g1.data(), c->b_s1.p, nPoints * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(g2.data(), c->b_s2.p, nPoints * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (c->opts.kinetics_model == EMX_KIN_INTERP)
+        if (int rc2 = interp_range_retry(c, nPoints, nVars, state_in, split_const, qloss, t_stage_start,
+                                         dt, state_out, stat, g1, g2))
+            return rc2;
+    if (status) memcpy(status, stat.data(), nPoints * 4);
+    if (stat1) memcpy(stat1, g1.data(), nPoints * 4);
+    if (stat2) memcpy(stat2, g2.data(), nPoints * 4);
     return 0;
 }
 

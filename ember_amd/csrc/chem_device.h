@@ -300,9 +300,13 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
         }
     }
     if (INTERP) {
-        if (!(T >= kInterpTmin && T < kInterpTmax)) return false;
-        size_t n = (size_t)floor((T - kInterpTmin) / (kInterpTmax - kInterpTmin) * (kInterpN - 1));
-        double dT = T - n * (kInterpTmax - kInterpTmin) / (kInterpN - 1) - kInterpTmin;
+        // InterpKinetics::updateROP's index and offset (src/chemistry0d.cpp:392-397); a T
+        // outside [Tmin, Tmax] would make Ember rebuild its tables (:279-291), here the
+        // point fails with -2 and the host grows the tables
+        const double Tlo = m.interpTmin, Thi = m.interpTmax;
+        if (!(T >= Tlo && T <= Thi)) return false;
+        size_t n = (size_t)floor((T - Tlo) / (Thi - Tlo) * (m.interpN - 1));
+        double dT = T - n * (Thi - Tlo) / (m.interpN - 1) - Tlo;
         const int stride = m.nRounds * 64;
 #pragma unroll
         for (int j = 0; j < NR; j++) {

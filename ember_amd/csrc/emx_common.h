@@ -13,7 +13,7 @@ constexpr int kMaxChunkRounds = 4; // species-sum chunks: <= 256 chunks
 constexpr int kChunkLen = 8;       // entries per species-sum chunk (padded with zero entries)
 constexpr int kPadSpecies = 63;    // stoichiometry padding index: C[63] = 1, mu[63] = 0
 constexpr int kInterpN = 200;      // InterpKinetics default table size (src/chemistry0d.h:70-72)
-constexpr double kInterpTmin = 273.0;
+constexpr double kInterpTmin = 273.0;  // and its default range; a context can grow it
 constexpr double kInterpTmax = 3500.0;
 
 constexpr double kGasConstant = 6.02214076e26 * 1.380649e-23;  // J/kmol/K
@@ -109,7 +109,9 @@ struct DevMech {
     const int32_t* aux;
     const double* auxd;
     int has_pdep;  // 1 if the mechanism has PLOG or Chebyshev reactions
-    // interp tables [kInterpN][nRounds*64]
+    // interp tables [interpN][nRounds*64] over [interpTmin, interpTmax]
+    int interpN;
+    double interpTmin, interpTmax;
     const double* kf_c;
     const double* kf_s;
     const double* kc_c;

@@ -333,17 +333,17 @@ void build_pressure_tables(const HostMech& m, const DevLayout& L, double P, std:
     if (auxd.empty()) auxd.push_back(0.0);
 }
 
-void build_interp_tables(const HostMech& m, const DevLayout& L, double P, std::vector<double>& kf_c,
-                         std::vector<double>& kf_s, std::vector<double>& kc_c,
-                         std::vector<double>& kc_s) {
+void build_interp_tables(const HostMech& m, const DevLayout& L, double P, int nT, double Tmin,
+                         double Tmax, std::vector<double>& kf_c, std::vector<double>& kf_s,
+                         std::vector<double>& kc_c, std::vector<double>& kc_s) {
     const int S = L.nRounds * 64;
-    kf_c.assign((size_t)kInterpN * S, 0.0);
-    kf_s.assign((size_t)kInterpN * S, 0.0);
-    kc_c.assign((size_t)kInterpN * S, 0.0);
-    kc_s.assign((size_t)kInterpN * S, 0.0);
+    kf_c.assign((size_t)nT * S, 0.0);
+    kf_s.assign((size_t)nT * S, 0.0);
+    kc_c.assign((size_t)nT * S, 0.0);
+    kc_s.assign((size_t)nT * S, 0.0);
     std::vector<double> mu(m.K);
-    for (int n = 0; n < kInterpN; n++) {
-        double T = kInterpTmin + ((double)n) / (kInterpN - 1) * (kInterpTmax - kInterpTmin);
+    for (int n = 0; n < nT; n++) {
+        double T = Tmin + ((double)n) / (nT - 1) * (Tmax - Tmin);
         double RT = T * kGasConstant;
         double tmp = std::log(P / m.Pref) * RT;
         for (int k = 0; k < m.K; k++) {
@@ -370,8 +370,9 @@ void build_interp_tables(const HostMech& m, const DevLayout& L, double P, std::v
             if (r.type <= 1) kf_c[(size_t)n * S + s] = r.A * std::exp(r.b * logT - r.EaR * recipT);
         }
     }
-    double dT = (kInterpTmax - kInterpTmin) / (kInterpN - 1);
-    for (int n = 0; n < kInterpN - 1; n++)
+    // the last node keeps slope 0, as InterpKinetics leaves it (src/chemistry0d.cpp:327-331)
+    double dT = (Tmax - Tmin) / (nT - 1);
+    for (int n = 0; n < nT - 1; n++)
         for (int s = 0; s < S; s++) {
             size_t a = (size_t)n * S + s, b = (size_t)(n + 1) * S + s;
             kf_s[a] = (kf_c[b] - kf_c[a]) / dT;

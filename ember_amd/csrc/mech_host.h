@@ -69,9 +69,10 @@ std::string build_layout(const HostMech& m, DevLayout& L);
 void build_pressure_tables(const HostMech& m, const DevLayout& L, double P, std::vector<int32_t>& aux,
                            std::vector<double>& auxd);
 
-// InterpKinetics / MultiArrheniusInterp tables at pressure P, [kInterpN][nRounds*64]
-void build_interp_tables(const HostMech& m, const DevLayout& L, double P, std::vector<double>& kf_c,
-                         std::vector<double>& kf_s, std::vector<double>& kc_c,
-                         std::vector<double>& kc_s);
+// InterpKinetics / MultiArrheniusInterp tables at pressure P, [nT][nRounds*64] over nT
+// equally spaced temperatures in [Tmin, Tmax] (src/chemistry0d.cpp:291-331, 415-436)
+void build_interp_tables(const HostMech& m, const DevLayout& L, double P, int nT, double Tmin,
+                         double Tmax, std::vector<double>& kf_c, std::vector<double>& kf_s,
+                         std::vector<double>& kc_c, std::vector<double>& kc_s);
 
 }  // namespace emx

@@ -109,6 +109,19 @@ int emx_set_rate_multipliers(emx_context* ctx, const double* multipliers);
  * set, and the plain kernel variant does not support it. */
 int emx_set_rate_multiplier_function(emx_context* ctx, int n, const double* data);
 
+/* InterpKinetics table grid (kinetics_model EMX_KIN_INTERP): n equally spaced
+ * temperatures over [Tmin, Tmax] for the interpolated forward rate constants and 1/Kc
+ * (InterpKinetics / MultiArrheniusInterp, src/chemistry0d.h:66-72, 147-149,
+ * src/chemistry0d.cpp:291-331, 392-397, 415-436); Ember's defaults 200 over [273, 3500] K.
+ * A right-hand side at a T outside the grid fails its point with status -2.  Ember instead
+ * rebuilds its tables (src/chemistry0d.cpp:279-291): emx_source_integrate does the same for
+ * its failed points (10 more nodes and 100 K more above; 2 more nodes and a lower bound of
+ * max(T - 20, 10) K below; then those points are integrated again from their input states),
+ * and the grown grid stays for later calls.  The device entry points report -2 and leave
+ * the growth to the caller through this function.  Waits for the context stream. */
+int emx_set_interp_range(emx_context* ctx, int n, double Tmin, double Tmax);
+int emx_get_interp_range(const emx_context* ctx, int* n, double* Tmin, double* Tmax);
+
 /* Grid positions x of the points of the production stages (SourceSystem::setPosition,
  * src/sourceSystem.cpp:73, called from src/flameSolver.cpp:501, :539), for the igniter.
  * Host array of nPoints; NULL clears.  The stages' nPoints must match while the igniter is
@@ -145,7 +158,14 @@ int emx_set_schedule(emx_context* ctx, int mode);
  *   state_in / state_out   nVars x nPoints (nVars = nSpecies + 2), host memory
  *   split_const            nVars x nPoints, SplitSolver::splitConstProd (NULL = zeros)
  *   qloss                  nPoints heat loss [W/m^3] evaluated at setState (NULL = zeros)
- *   status                 nPoints, 0 ok / <0 failed (may be NULL)
+ *   status                 nPoints, 0 ok / <0 failed (may be NULL): -1 QSS step below
+ *                          qss_dtmin, -2 T outside the interpolation tables (after the
+ *                          growth described at emx_set_interp_range), -3 step size below
+ *                          the minimum / step budget, -4 non-finite QSS error or BDF
+ *                          integrator_max_steps, -5 singular BDF Newton matrix, -6
+ *                          non-finite BDF error estimate, -7 / -8 too many BDF error-test
+ *                          / convergence failures in one step, -9 invalid input state
+ *                          (a NaN or infinite value, or T <= 0)
  *   stat1, stat2           nPoints: QSS gcount/rcount, BDF steps/RHS evaluations (NULL ok)
  * Synchronous. */
 int emx_source_integrate(emx_context* ctx, size_t nPoints, size_t nVars, const double* state_in,

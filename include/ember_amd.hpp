@@ -118,6 +118,17 @@ private:
     }
 };
 
+// Ember's BoundaryCondition::BC (src/grid.h:7-25: FixedValue 0, ZeroGradient 1,
+// ControlVolume 2, WallFlux 3, Floating 4) as the diffusion systems use it:
+// DiffusionSystem::get_A treats every left condition other than the first four, and every
+// right condition other than FixedValue, as ZeroGradient (src/diffusionSystem.cpp:24-58).
+inline int diffusion_left_bc(int emberBC) {
+    return (emberBC >= EMX_BC_FIXED_VALUE && emberBC <= EMX_BC_WALL_FLUX) ? emberBC : EMX_BC_ZERO_GRADIENT;
+}
+inline int diffusion_right_bc(int emberBC) {
+    return emberBC == EMX_BC_FIXED_VALUE ? EMX_BC_FIXED_VALUE : EMX_BC_ZERO_GRADIENT;
+}
+
 // Production and diffusion stages on the GPU (FlameSolver's two parallel_for loops).
 class FlameStages : public SplitSolver {
 public:
@@ -126,8 +137,9 @@ public:
         : ctx_(ctx), x_(x) {
         grid_.x = x_.data();
         grid_.alpha = alpha;
-        grid_.left_bc = leftBC;
-        grid_.right_bc = rightBC;
+        grid_.left_bc = diffusion_left_bc(leftBC);
+        grid_.right_bc = diffusion_right_bc(rightBC);
+        // DiffusionSystem's defaults (yInf = wallConst = 0) until setWallFlux
         grid_.y_inf = 0.0;
         grid_.wall_const = 0.0;
         grid_.wall_var = 1;  // kEnergy
@@ -135,6 +147,22 @@ public:
         // igniter's positions
         check(emx_set_point_positions(ctx_.get(), x_.size(), x_.data()));
     }
+
+    // OneDimGrid::leftBC / rightBC as FlameSolver::updateBC sets them (Ember's enum values).
+    void setBoundaryConditions(int leftBC, int rightBC) {
+        grid_.left_bc = diffusion_left_bc(leftBC);
+        grid_.right_bc = diffusion_right_bc(rightBC);
+    }
+
+    // ConfigOptions wallFlux / Tinf / Kwall (src/readConfig.h:130-132): FlameSolver gives the
+    // energy equation's DiffusionSystem yInf = Tinf and wallConst = Kwall when wallFlux is
+    // set (src/flameSolver.cpp:79-82); the other variables keep 0 and 0.
+    void setWallFlux(bool wallFlux, double Tinf, double Kwall) {
+        grid_.y_inf = wallFlux ? Tinf : 0.0;
+        grid_.wall_const = wallFlux ? Kwall : 0.0;
+        grid_.wall_var = 1;  // kEnergy
+    }
+    const emx_grid& grid() const { return grid_; }
 
     // Diffusion coefficients as FlameSolver::prepareIntegrators sets them
     // (flameSolver.cpp:160-172): B rows (1/rho, 1/(rho cp)) with B_row per variable,

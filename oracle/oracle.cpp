@@ -80,7 +80,7 @@ struct Mech {
     std::vector<Stoich> reac, prod, revprod;
     std::vector<int> thirdbody;  // reactions with a third body (three-body + falloff)
     std::vector<int> tb_slot;    // reaction -> slot in concm, or -1
-    // interp tables (InterpKinetics / MultiArrheniusInterp), frozen range
+    // interp tables (InterpKinetics / MultiArrheniusInterp) over [TminInt, TmaxInt]
     int nInterp = 200;
     double TminInt = 273.0, TmaxInt = 3500.0;
     std::vector<double> kf_const, kf_slope, kc_const, kc_slope;  // [n][I]
@@ -375,7 +375,8 @@ struct Gas {
         return r.type == 4 ? plog_kf(r, P, logT, recipT) : cheb_kf(r, P, recipT);
     }
 
-    // InterpKinetics / MultiArrheniusInterp tables over [TminInt, TmaxInt], frozen.
+    // InterpKinetics / MultiArrheniusInterp tables over [TminInt, TmaxInt]
+    // (src/chemistry0d.cpp:291-331, 415-436); the last node keeps slope 0.
     void build_interp() {
         if (m.interp_ready && m.interp_P == P) return;
         int n = m.nInterp;
@@ -413,7 +414,7 @@ struct Gas {
     bool updateRateConstants() {
         if (rateT == T) return true;
         if (kinetics == 1) {
-            if (T < m.TminInt || T >= m.TmaxInt) return false;
+            if (T < m.TminInt || T > m.TmaxInt) return false;
             size_t n = (size_t)std::floor((T - m.TminInt) / (m.TmaxInt - m.TminInt) * (m.nInterp - 1));
             double dT = T - n * (m.TmaxInt - m.TminInt) / (m.nInterp - 1) - m.TminInt;
             const double logT = std::log(T), recipT = 1.0 / T;
@@ -1274,6 +1275,16 @@ int orc_mech_nreactions(void* mech) { return static_cast<Mech*>(mech)->I; }
 void orc_mech_set_multipliers(void* mech, const double* m) {
     Mech* me = static_cast<Mech*>(mech);
     for (int i = 0; i < me->I; i++) me->perturb[i] = m ? m[i] : 1.0;
+}
+
+int orc_mech_set_interp_range(void* mech, int n, double Tmin, double Tmax) {
+    Mech& m = *static_cast<Mech*>(mech);
+    if (n < 2 || !(Tmin > 0.0) || !(Tmax > Tmin)) return -1;
+    m.nInterp = n;
+    m.TminInt = Tmin;
+    m.TmaxInt = Tmax;
+    m.interp_ready = false;
+    return 0;
 }
 
 int orc_mech_set_rate_function(void* mech, int n, const double* data) {

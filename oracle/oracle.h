@@ -50,6 +50,10 @@ int orc_mech_nspecies(void* mech);
 int orc_mech_nreactions(void* mech);
 /* Per-reaction rate multipliers (Cantera setMultiplier / m_perturb). */
 void orc_mech_set_multipliers(void* mech, const double* m);
+/* InterpKinetics table grid: n equally spaced temperatures over [Tmin, Tmax] (Ember's
+ * defaults 200 over [273, 3500], src/chemistry0d.h:70-72).  Rates at a T outside it fail
+ * (status -2).  Returns -1 for an invalid grid. */
+int orc_mech_set_interp_range(void* mech, int n, double Tmin, double Tmax);
 /* Ember's rate multiplier function, FlameSolver::rateMultiplierFunction: a ChebyshevFunction
  * whose coefficients ChebyshevFunction::setCoefficients(n, data) sets (data = [T0, T1,
  * c_0, ...], src/scalarFunction.cpp:33-51).  While set, every right-hand side multiplies

@@ -52,6 +52,7 @@ def lib():
         L.orc_mech_nspecies.argtypes = [C.c_void_p]
         L.orc_mech_nreactions.argtypes = [C.c_void_p]
         L.orc_mech_set_multipliers.argtypes = [C.c_void_p, _dp]
+        L.orc_mech_set_interp_range.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         L.orc_mech_set_rate_function.argtypes = [C.c_void_p, C.c_int, _dp]
         L.orc_rate_function_eval.restype = C.c_double
         L.orc_rate_function_eval.argtypes = [C.c_void_p, C.c_double]
@@ -112,6 +113,10 @@ class Mech:
     def set_multipliers(self, m):
         m = None if m is None else np.ascontiguousarray(m, dtype=np.float64)
         lib().orc_mech_set_multipliers(self.h, _p(m))
+
+    def set_interp_range(self, n, Tmin, Tmax):
+        if lib().orc_mech_set_interp_range(self.h, int(n), float(Tmin), float(Tmax)):
+            raise ValueError("invalid interpolation grid")
 
     def set_rate_function(self, data):
         """ChebyshevFunction::setCoefficients(len(data), data), data = [T0, T1, c0, ...];

@@ -4,7 +4,9 @@
 // step; and the device-resident split step (emx_split_step_device, state kept in device
 // memory) on the same inputs.
 //
-// usage: split_flame <mech_table> <input.bin> <output.bin> <nsteps> <dt>
+// usage: split_flame <mech_table> <input.bin> <output.bin> <nsteps> <dt> [leftBC Tinf Kwall]
+// (leftBC: Ember's BoundaryCondition value for the left end; with leftBC = 3 (WallFlux)
+// the energy equation carries yInf = Tinf and wallConst = Kwall, flameSolver.cpp:79-82)
 // input.bin: int64 nVars, nPoints, nB; doubles x[N], state[N*nV], B[nB*N], D[N*nV];
 //            int32 B_row[nV]
 // output.bin: doubles gpu[nsteps][N*nV], oracle[nsteps][N*nV], device[nsteps][N*nV];
@@ -23,11 +25,13 @@ namespace {
 
 class OracleStages : public ember_amd::SplitSolver {
 public:
-    OracleStages(void* mech, const orc_options& o, const std::vector<double>& x, int dmult_unused = 0)
-        : mech_(mech), o_(o), x_(x) { (void)dmult_unused; }
+    OracleStages(void* mech, const orc_options& o, const std::vector<double>& x)
+        : mech_(mech), o_(o), x_(x) {}
     ember_amd::Matrix B, D;
     std::vector<int32_t> B_row;
     int failures = 0;
+    int leftBC = 0;
+    double yInf = 0.0, wallConst = 0.0;
 
 protected:
     void integrateProductionTerms() override {
@@ -42,7 +46,8 @@ protected:
     void integrateDiffusionTerms() override {
         std::vector<int> brow(B_row.begin(), B_row.end());
         orc_diffusion_integrate((int)nVars, (int)nPoints, B.data(), (int)(B.size() / nPoints), brow.data(),
-                                D.data(), splitConstDiff.data(), x_.data(), 0, 0, 0, 0.0, 0.0, 1, 10.0, dt,
+                                D.data(), splitConstDiff.data(), x_.data(), 0, leftBC, 0, yInf, wallConst, 1,
+                                10.0, dt,
                                 nullptr, tStageStart, tStageEnd, state.data(), 0);
     }
 
@@ -58,12 +63,16 @@ bool rd(FILE* f, T* p, size_t n) { return fread(p, sizeof(T), n, f) == n; }
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 6) {
-        fprintf(stderr, "usage: %s table input output nsteps dt\n", argv[0]);
+    if (argc != 6 && argc != 9) {
+        fprintf(stderr, "usage: %s table input output nsteps dt [leftBC Tinf Kwall]\n", argv[0]);
         return 2;
     }
     const int nsteps = atoi(argv[4]);
     const double dt = atof(argv[5]);
+    const int leftBC = argc == 9 ? atoi(argv[6]) : EMX_BC_FIXED_VALUE;
+    const bool wallFlux = leftBC == EMX_BC_WALL_FLUX;
+    const double Tinf = argc == 9 ? atof(argv[7]) : 0.0;
+    const double Kwall = argc == 9 ? atof(argv[8]) : 0.0;
     FILE* f = fopen(argv[2], "rb");
     if (!f) return 3;
     int64_t hdr[3];
@@ -80,7 +89,8 @@ int main(int argc, char** argv) {
     emx_default_options(&eo);
     eo.global_timestep = dt;
     ember_amd::Context ctx(argv[1], 101325.0, eo);
-    ember_amd::FlameStages gpu(ctx, x);
+    ember_amd::FlameStages gpu(ctx, x, 0, leftBC, EMX_BC_FIXED_VALUE);
+    gpu.setWallFlux(wallFlux, Tinf, Kwall);
     gpu.resize(nV, N);
     gpu.state = st;
     gpu.B = B;
@@ -92,6 +102,9 @@ int main(int argc, char** argv) {
     void* mech = orc_mech_load(argv[1]);
     if (!mech) return 4;
     OracleStages cpu(mech, oo, x);
+    cpu.leftBC = leftBC;
+    cpu.yInf = wallFlux ? Tinf : 0.0;
+    cpu.wallConst = wallFlux ? Kwall : 0.0;
     cpu.resize(nV, N);
     cpu.state = st;
     cpu.B = B;
@@ -135,7 +148,8 @@ int main(int argc, char** argv) {
              hipMemcpy(d_Brow, B_row.data(), nV * 4, hipMemcpyHostToDevice) == hipSuccess;
         if (!ok) return 6;
         emx_split_arrays sa{d_state, d_ddt, d_ddt + n, d_ddt + 2 * n, nullptr, d_delta, d_delta + n, d_work};
-        emx_grid g{d_x, 0, EMX_BC_FIXED_VALUE, EMX_BC_FIXED_VALUE, 0.0, 0.0, 1};
+        emx_grid g = gpu.grid();
+        g.x = d_x;
         std::vector<double> host(n);
         std::vector<int32_t> hst(N);
         t = 0.0;

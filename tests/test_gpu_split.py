@@ -19,7 +19,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "tests", "cpp", "_build", "split_flame")
 
 
-def test_split_flame_100_steps(gpu, tmp_path):
+@pytest.mark.parametrize("wall", [None, (350.0, 200.0)], ids=["fixed", "wallflux"])
+def test_split_flame_100_steps(gpu, tmp_path, wall):
+    """wall = (Tinf, Kwall): the left end is a WallFlux boundary whose energy equation
+    carries yInf = Tinf and wallConst = Kwall (src/flameSolver.cpp:79-82)."""
     case = W.flame_profile(150, seed=1, name="h2o2")
     N, nV = case.state.shape
     inp = tmp_path / "in.bin"
@@ -30,9 +33,10 @@ def test_split_flame_100_steps(gpu, tmp_path):
         case.B_row.astype(np.int32).tofile(f)
     out = tmp_path / "out.bin"
     nsteps, dt = 100, 1e-5
-    print("[split] running 100 split steps", flush=True)
-    r = subprocess.run([EXE, mechanism.table_path("h2o2"), str(inp), str(out), str(nsteps), str(dt)],
-                       capture_output=True, text=True, timeout=300)
+    print(f"[split] running 100 split steps, wall={wall}", flush=True)
+    extra = [] if wall is None else ["3", repr(wall[0]), repr(wall[1])]
+    r = subprocess.run([EXE, mechanism.table_path("h2o2"), str(inp), str(out), str(nsteps), str(dt)]
+                       + extra, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     data = np.fromfile(out, dtype=np.float64, count=3 * nsteps * N * nV).reshape(3, nsteps, N, nV)
     fails = np.fromfile(out, dtype=np.int32, offset=3 * nsteps * N * nV * 8)
@@ -48,6 +52,10 @@ def test_split_flame_100_steps(gpu, tmp_path):
     assert dY.max() <= 1e-12
     # the flame evolved: chemistry and diffusion both acted
     assert np.abs(g[-1, :, 1] - case.state[:, 1]).max() > 1.0
+    if wall is not None:
+        # the wall pulls the left-end temperature towards Tinf
+        T0 = case.state[0, 1]
+        assert abs(g[-1, 0, 1] - wall[0]) < abs(T0 - wall[0]), (T0, g[-1, 0, 1])
 
 
 def test_correct_mass_fractions_device(gpu):
