@@ -78,6 +78,8 @@ SIGNATURES = [
      [_vp, _sz, _sz, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _ip, _ip, _ip]),
     ("emx_source_integrate_device", C.c_int,
      [_vp, _sz, _sz, _vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp]),
+    ("emx_source_integrate_device_noalloc", C.c_int,
+     [_vp, _sz, _sz, _vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp]),
     ("emx_set_adjoint_capacity", C.c_int, [_vp, C.c_int]),
     ("emx_source_adjoint_forward", C.c_int,
      [_vp, _sz, _sz, _vp, _vp, _vp, C.c_double, _vp, _vp, _vp]),

@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -105,7 +107,40 @@ struct emx_context {
     double interp_tmin = kInterpTmin, interp_tmax = kInterpTmax;
 };
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free production stage (emx_source_integrate_device_noalloc):
+// bytes [0, 256) hold the work-queue counter, then kNoAllocMaxPoints per-point costs and
+// kNoAllocMaxPoints work-queue positions (int32 each): 2 MiB + 256 B per GPU, held until
+// the process ends.  Host code never uses the array's name as an address: each device's
+// copy is reached through the pointer hipGetSymbolAddress returned for it.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocMaxPoints = 262144;
+constexpr size_t kScratchBytes = 256 + 2 * kNoAllocMaxPoints * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+struct DeviceScratch {
+    unsigned char* ptr = nullptr;  // this device's copy of `scratch`
+    size_t last_n = 0;             // points of the last allocation-free stage (0: none)
+};
+std::mutex g_scratch_mu;
+std::map<int, DeviceScratch> g_scratch;
+
+// Looks up the device copy of `scratch` once per device; the device must be current.
+int scratch_setup(int device, std::string& err) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device].ptr) return 0;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return -10;
+    }
+    g_scratch[device].ptr = static_cast<unsigned char*>(p);
+    return 0;
+}
 
 DevOpts to_dev(const emx_options& o) {
     DevOpts d;
@@ -340,6 +375,13 @@ int emx_create(const char* path, double pressure, const emx_options* opts, emx_c
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    {
+        std::string serr;
+        if (int src = scratch_setup(c->device, serr)) {
+            delete c;
+            return fail(src, serr);
+        }
+    }
     int rc = upload_mech(c);
     if (rc) { delete c; return rc; }
     std::vector<double> ones(c->lay.nRounds * 64, 1.0);
@@ -541,6 +583,59 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
                               wpb, c->stream);
     if (rc) return fail(-10, std::string("qss launch: ") + hipGetErrorString((hipError_t)rc));
     c->cost_n = nPoints;
+    return 0;
+}
+
+int emx_source_integrate_device_noalloc(emx_context* c, size_t nPoints, size_t nVars,
+                                        const double* state_in, const double* split_const,
+                                        const double* qloss, double t_stage_start, double dt,
+                                        double* state_out, int32_t* status, int32_t* stat1,
+                                        int32_t* stat2, double* qint) {
+    if (int rc = check_ctx(c)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (nPoints == 0) return 0;
+    if (nPoints > kNoAllocMaxPoints)
+        return fail(-1, "emx_source_integrate_device_noalloc takes at most 262144 points per call");
+    if (!state_in || !state_out) return fail(-1, "state pointers are NULL");
+    if (c->opts.chemistry_integrator != EMX_QSS)
+        return fail(-1, "emx_source_integrate_device_noalloc runs the QSS integrator only (the BDF "
+                        "needs a Jacobian workspace per wave)");
+    if (c->variant == EMX_VARIANT_PLAIN)
+        return fail(-1, "emx_source_integrate_device_noalloc does not run the plain baseline kernel");
+    if (c->opts.ignition_energy != 0.0 && !(c->qign_valid && c->xpos.size() == nPoints))
+        return fail(-1, "with the igniter on, run one emx_source_integrate_device stage over these "
+                        "points first (it prepares the igniter amplitudes)");
+    unsigned char* base;
+    bool cost_valid;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        DeviceScratch& d = g_scratch[c->device];
+        if (!d.ptr) return fail(-10, "scratch of this device was not set up");
+        base = d.ptr;
+        cost_valid = d.last_n == nPoints;
+        d.last_n = nPoints;
+    }
+    unsigned int* queue = reinterpret_cast<unsigned int*>(base);
+    int* cost = reinterpret_cast<int*>(base + 256);
+    int* order_buf = cost + kNoAllocMaxPoints;
+    // the costs left by the previous allocation-free stage on this device over the same
+    // number of points give the work-queue order (results do not depend on it)
+    const int* order = nullptr;
+    if (c->schedule == EMX_SCHED_COST && cost_valid && c->variant == EMX_VARIANT_WAVE) {
+        int rc = launch_cost_order(cost, (int)nPoints, order_buf, c->stream);
+        if (rc) return fail(-10, std::string("order launch: ") + hipGetErrorString((hipError_t)rc));
+        order = order_buf;
+    }
+    const bool per_point = c->variant == EMX_VARIANT_WAVE_PER_POINT;
+    const bool ext = c->ratefn_n > 0 || c->opts.ignition_energy != 0.0;
+    const int wpb = ext ? std::min(c->waves_per_block, 8) : c->waves_per_block;
+    const int qblocks = per_point ? (int)((nPoints + wpb - 1) / wpb)
+                                  : grid_blocks(c, (int)nPoints, false, wpb);
+    int rc = launch_qss_stage(c->dm, stage_opts(c, t_stage_start), state_in, split_const, qloss,
+                              stage_perturb(c), dt, (int)nPoints, per_point ? nullptr : order,
+                              per_point ? nullptr : queue, state_out, status, stat1, stat2, cost, qint,
+                              qblocks, wpb, c->stream);
+    if (rc) return fail(-10, std::string("qss launch: ") + hipGetErrorString((hipError_t)rc));
     return 0;
 }
 

@@ -181,6 +181,24 @@ int emx_source_integrate_device(emx_context* ctx, size_t nPoints, size_t nVars,
                                 double* state_out, int32_t* status, int32_t* stat1,
                                 int32_t* stat2, double* qint);
 
+/* The production stage of emx_source_integrate_device without any device allocation.
+ * No counterpart in the reference: for callers that cannot allocate device memory during
+ * a stage (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as emx_source_integrate_device (bit for bit),
+ * QSS integrator only; the work-queue counter, per-point costs and queue order live in a
+ * fixed library scratch of 2 MiB + 256 B per GPU (held until the process ends), so one
+ * call takes at most 262144 points (-1 beyond, and for the BDF or the plain variant).
+ * With the igniter on, one emx_source_integrate_device stage over the same points must
+ * come first.  Every call on one GPU shares that GPU's scratch: calls that overlap in
+ * time on one GPU (from two host threads, or on two streams) race; the costs left by the
+ * last call order the next one over the same number of points, which changes only the
+ * order in which points are integrated, never their results. */
+int emx_source_integrate_device_noalloc(emx_context* ctx, size_t nPoints, size_t nVars,
+                                        const double* state_in, const double* split_const,
+                                        const double* qloss, double t_stage_start, double dt,
+                                        double* state_out, int32_t* status, int32_t* stat1,
+                                        int32_t* stat2, double* qint);
+
 /* Discrete adjoint of the QSS production stage (QSS integrator with
  * qss_iteration_count = 1 and no stability check).  No counterpart in the reference: it
  * differentiates the stage that SourceSystemQSS::integrateToTime runs per point
