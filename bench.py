@@ -61,7 +61,8 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     p.add_argument("--headline-only", action="store_true",
                    help="skip the bdf / uneven / diffusion / thermo sub-objects")
-    p.add_argument("--workload", choices=["stage", "uneven", "diffusion", "split", "copy", "thermo"],
+    p.add_argument("--workload", choices=["stage", "uneven", "diffusion", "split", "copy", "thermo",
+                                          "transport"],
                    default="stage")
     p.add_argument("--mech", default="gri30", help="mechanism of the split workload")
     return p.parse_args()
@@ -111,7 +112,7 @@ def main():
     dev = torch.device("cuda:0")
     if a.workload != "stage":
         fn = {"uneven": run_uneven, "diffusion": run_diffusion, "copy": run_copy,
-              "split": run_split, "thermo": run_thermo}[a.workload]
+              "split": run_split, "thermo": run_thermo, "transport": run_transport}[a.workload]
         print(json.dumps(fn(a, dev)))
         return
     res = run_stage(a, dev)
@@ -125,6 +126,7 @@ def main():
         res["uneven"] = _subobject(run_uneven(sub, dev))
         res["diffusion"] = _subobject(run_diffusion(sub, dev))
         res["thermo"] = _subobject(run_thermo(sub, dev))
+        res["transport"] = _subobject(run_transport(sub, dev))
     print(json.dumps(res))
 
 
@@ -730,6 +732,81 @@ def run_thermo(a, dev):
                                "sample": f"all {N} points, OpenMP dynamic,1; median of {len(runs)} "
                                          f"runs after 1 warm-up: {cpu_s * 1e3:.1f} ms"}
         res["parity"] = {"max_abs_err_over_max": err}
+    return res
+
+
+def transport_flops(X, threshold):
+    """FP64 operations of the ApproxMixTransport evaluation per point, from its mole
+    fractions (N, K): each pair term of the mass-weighted diffusion sums 15 (the fitted
+    binary coefficient 10, two quotients and two sums), each Wilke term 12, each species'
+    own fits and sums 30; sqrt and division count 1."""
+    import numpy as np
+    K = X.shape[1]
+    nm = (X >= threshold).sum(axis=1).astype(np.float64)
+    pairs = nm * (K - 1) + (K - nm) * nm
+    return float((pairs * 15 + nm * nm * 12 + K * 30).sum())
+
+
+def run_transport(a, dev):
+    """Transport block of updateChemicalProperties (ApproxMixTransport: mu, lambda,
+    rho D_k) for GRI-Mech 3.0 on 65536 points."""
+    import numpy as np
+    import torch
+
+    from ember_amd import capi, mechanism, workloads
+    N = a.points if a.points != 4096 else 65536
+    st = workloads.tanh_progress_states(N, seed=2, name="gri30")
+    ctx = capi.Context("gri30")
+    s = torch.cuda.current_stream(dev)
+    ctx.set_stream(s.cuda_stream)
+    K = st.shape[1] - 2
+    f = dict(dtype=torch.float64, device=dev)
+    d = torch.from_numpy(st).to(dev)
+    mu, lam, rhoD = torch.empty(N, **f), torch.empty(N, **f), torch.empty((N, K), **f)
+    call = lambda: ctx.transport_properties_device(d, mu, lam, rhoD)
+    for _ in range(max(1, a.warmup)):
+        call()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(a.steps):
+        call()
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    m = mechanism.load("gri30")
+    Y = np.maximum(st[:, 2:], 0.0)
+    Y /= Y.sum(axis=1, keepdims=True)
+    X = Y / m.W
+    X = np.maximum(1e-20, X / X.sum(axis=1, keepdims=True))
+    F = transport_flops(X, ctx.options.transport_threshold)
+    print(f"[bench] transport properties gri30 x {N}: {ms * 1e3:.1f} us per call", file=sys.stderr, flush=True)
+    res = {
+        "metric": "transport properties points/s (GRI-Mech 3.0, 65536 points)", "value": N / (ms * 1e-3),
+        "unit": "points/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded tanh progress-variable states)",
+        "config": {"workload": "updateChemicalProperties transport block, ApproxMixTransport "
+                               "(threshold 1e-5): mu, lambda, rho D_k for GRI-Mech 3.0",
+                   "points": N, "seed": 2},
+        "roofline": {"bound": "fp64-vector", "achieved": F / (ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": F / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": F,
+                     "note": "pair and Wilke terms counted from the states' major-species sets"},
+    }
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = a.cpu_threads or cpu_info()["threads_used"]
+        om = O.Mech(mechanism.table_path("gri30"))
+        cpu_s, runs, ref = timed_cpu(lambda: om.transport_properties(st, nthreads=threads),
+                                     min(a.cpu_repeats, 3))
+        got = (mu.cpu().numpy(), lam.cpu().numpy(), rhoD.cpu().numpy())
+        err = {k: float((np.abs(g - r) / np.abs(r)).max()) for k, g, r in zip(("mu", "lambda", "rhoD"), got, ref)}
+        res["cpu_baseline"] = {"value": N / cpu_s, "unit": "points/s", "cores": threads, "kind": "port",
+                               "sample": f"all {N} points, OpenMP dynamic,1; median of {len(runs)} runs "
+                                         f"after 1 warm-up: {cpu_s * 1e3:.1f} ms"}
+        res["parity"] = {"max_rel_err": err}
     return res
 
 

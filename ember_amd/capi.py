@@ -40,7 +40,7 @@ class Options(C.Structure):
         ("diffusion_timestep_multiplier", C.c_double), ("global_timestep", C.c_double),
         ("ignition_t_start", C.c_double), ("ignition_duration", C.c_double),
         ("ignition_energy", C.c_double), ("ignition_center", C.c_double),
-        ("ignition_stddev", C.c_double),
+        ("ignition_stddev", C.c_double), ("transport_threshold", C.c_double),
     ]
 
 
@@ -98,6 +98,9 @@ SIGNATURES = [
     ("emx_correct_mass_fractions_device", C.c_int, [_vp, _sz, _sz, _vp]),
     ("emx_heat_release_device", C.c_int, [_vp, _sz, _sz, _vp, C.c_double, _vp, _vp]),
     ("emx_thermo_properties", C.c_int, [_vp, _sz, _sz, _dp, _dp, _dp, _dp, _dp, _dp]),
+    ("emx_transport_properties", C.c_int, [_vp, _sz, _sz, _dp, _dp, _dp, _dp]),
+    ("emx_transport_properties_device", C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp]),
+    ("emx_diffusion_coefficients_device", C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp]),
     ("emx_thermo_properties_device", C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]
 
@@ -363,6 +366,28 @@ class Context:
         _check(lib().emx_thermo_properties(self.h, nV, N, _p(st), _p(out["rho"]), _p(out["Wmx"]),
                                            _p(out["cp"]), _p(out["cpSpec"]), _p(out["hk"])))
         return out
+
+    def transport_properties(self, state):
+        """(mu, lambda, rhoD) at each point of a host state (emx_transport_properties)."""
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        n = state.shape[0]
+        mu, lam = np.zeros(n), np.zeros(n)
+        rhoD = np.zeros((n, self.n_species))
+        _check(lib().emx_transport_properties(self.h, self.n_vars, n, _p(state), _p(mu), _p(lam), _p(rhoD)))
+        return mu, lam, rhoD
+
+    def transport_properties_device(self, state, mu, lam, rhoD):
+        n = state.shape[0]
+        _check(lib().emx_transport_properties_device(
+            self.h, self.n_vars, n, _tptr(state, n * self.n_vars), _tptr(mu, n), _tptr(lam, n),
+            _tptr(rhoD, n * self.n_species)))
+
+    def diffusion_coefficients_device(self, state, B, D):
+        """B (2 x N: 1/rho, 1/(rho cp)) and D (N x nVars: mu, lambda, rho D_k) of the split
+        step from the device state (emx_diffusion_coefficients_device)."""
+        n = state.shape[0]
+        _check(lib().emx_diffusion_coefficients_device(
+            self.h, self.n_vars, n, _tptr(state, n * self.n_vars), _tptr(B, 2 * n), _tptr(D, n * self.n_vars)))
 
     def thermo_properties_device(self, state, rho, cp, Wmx=None, cpSpec=None, hk=None):
         """Density, mean molecular weight, cp and the species' partial molar cp and

@@ -245,6 +245,12 @@ int upload_mech(emx_context* c) {
     pk.add(L.chunk_sp, &d.chunk_sp);
     pk.add(aux, &d.aux);
     pk.add(auxd, &d.auxd);
+    d.tr_visc = d.tr_cond = d.tr_diff = nullptr;
+    if (m.has_transport()) {
+        pk.add(m.visc, &d.tr_visc);
+        pk.add(m.cond, &d.tr_cond);
+        pk.add(m.diff, &d.tr_diff);
+    }
     pk.add(kf_c, &d.kf_c);
     pk.add(kf_s, &d.kf_s);
     pk.add(kc_c, &d.kc_c);
@@ -351,6 +357,7 @@ void emx_default_options(emx_options* o) {
     o->ignition_energy = 0.0;
     o->ignition_center = 0.0;
     o->ignition_stddev = 1e-4;
+    o->transport_threshold = 1e-5;  // python/ember/input.py class Chemistry, threshold
 }
 
 const char* emx_last_error(void) { return g_err.c_str(); }
@@ -1127,6 +1134,62 @@ int emx_thermo_properties_device(emx_context* c, size_t nVars, size_t nPoints, c
     if (!state || !rho || !cp) return fail(-1, "state, rho or cp is NULL");
     int e = launch_thermo_props(c->dm, state, (int)nPoints, rho, Wmx, cp, cpSpec, hk, c->stream);
     return e ? fail(-10, std::string("thermo launch: ") + hipGetErrorString((hipError_t)e)) : 0;
+}
+
+static int check_transport(emx_context* c, size_t nVars, size_t nPoints) {
+    if (int rc = check_ctx(c)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
+    if (!c->hm.has_transport())
+        return fail(-1, "the mechanism table has no transport data (species `transport` blocks)");
+    if (!(c->opts.transport_threshold >= 0.0)) return fail(-1, "transport_threshold must be >= 0");
+    return 0;
+}
+
+int emx_transport_properties_device(emx_context* c, size_t nVars, size_t nPoints, const double* state,
+                                    double* mu, double* lambda, double* rhoD) {
+    if (int rc = check_transport(c, nVars, nPoints)) return rc;
+    if (nPoints == 0) return 0;
+    if (!state || !mu || !lambda || !rhoD) return fail(-1, "state, mu, lambda or rhoD is NULL");
+    int e = launch_transport_props(c->dm, state, (int)nPoints, c->opts.transport_threshold, mu, lambda, 1,
+                                   rhoD, (int)(nVars - 2), c->stream);
+    return e ? fail(-10, std::string("transport launch: ") + hipGetErrorString((hipError_t)e)) : 0;
+}
+
+int emx_transport_properties(emx_context* c, size_t nVars, size_t nPoints, const double* state, double* mu,
+                             double* lambda, double* rhoD) {
+    if (int rc = check_transport(c, nVars, nPoints)) return rc;
+    if (nPoints == 0) return 0;
+    if (!state || !mu || !lambda || !rhoD) return fail(-1, "state, mu, lambda or rhoD is NULL");
+    const size_t N = nPoints, K = nVars - 2;
+    hipStream_t st = c->stream;
+    HIP_TRY(c->b_in.ensure(N * nVars * sizeof(double)));
+    HIP_TRY(c->b_out.ensure((2 * N + N * K) * sizeof(double)));
+    double* o = c->b_out.as<double>();
+    HIP_TRY(hipMemcpyAsync(c->b_in.p, state, N * nVars * sizeof(double), hipMemcpyHostToDevice, st));
+    int e = launch_transport_props(c->dm, c->b_in.as<double>(), (int)N, c->opts.transport_threshold, o, o + N,
+                                   1, o + 2 * N, (int)K, st);
+    if (e) return fail(-10, std::string("transport launch: ") + hipGetErrorString((hipError_t)e));
+    HIP_TRY(hipMemcpyAsync(mu, o, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(lambda, o + N, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(rhoD, o + 2 * N, N * K * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int emx_diffusion_coefficients_device(emx_context* c, size_t nVars, size_t nPoints, const double* state,
+                                      double* B, double* D) {
+    if (int rc = check_transport(c, nVars, nPoints)) return rc;
+    if (nPoints == 0) return 0;
+    if (!state || !B || !D) return fail(-1, "state, B or D is NULL");
+    const int n = (int)nPoints, nv = (int)nVars;
+    hipStream_t st = c->stream;
+    // rho into B row 0, cp into B row 1, then B = (1/rho, 1/(rho cp)) in place
+    int e = launch_thermo_props(c->dm, state, n, B, nullptr, B + nPoints, nullptr, nullptr, st);
+    if (!e) e = launch_diffusion_B(n, B, st);
+    // D rows per point: mu, lambda, rho D_k (FlameSolver::prepareIntegrators)
+    if (!e) e = launch_transport_props(c->dm, state, n, c->opts.transport_threshold, D, D + 1, nv, D + 2, nv, st);
+    return e ? fail(-10, std::string("diffusion coefficient launch: ") + hipGetErrorString((hipError_t)e)) : 0;
 }
 
 int emx_tridiagonal_integrate(emx_context* c, size_t nSys, size_t N, const double* a, const double* b,

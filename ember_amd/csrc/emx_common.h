@@ -109,6 +109,12 @@ struct DevMech {
     const int32_t* aux;
     const double* auxd;
     int has_pdep;  // 1 if the mechanism has PLOG or Chebyshev reactions
+    // transport fits (polynomials in ln T, 5 coefficients each): sqrt(mu_k)/T^(1/4) and
+    // lambda_k/sqrt(T) per species, D_jk(T, 1 Pa)/T^(3/2) per pair j <= k (row by row);
+    // NULL without transport data
+    const double* tr_visc;
+    const double* tr_cond;
+    const double* tr_diff;
     // interp tables [interpN][nRounds*64] over [interpTmin, interpTmax]
     int interpN;
     double interpTmin, interpTmax;

@@ -31,6 +31,14 @@ int launch_time_derivatives(size_t n, double dt, const double* delta_diff, const
 int launch_correct_mass_fractions(int nVars, int nPoints, double* state, hipStream_t st);
 int launch_heat_release(const DevMech& m, const DevOpts& o, const double* state, const double* perturb, int n,
                         double* qdot, int* status, hipStream_t stream);
+// ApproxMixTransport properties (mu, lambda, rho D_k) of n points; needs m.tr_* fits.
+// mu and lambda of point p at [p * ml_stride], rho D_k at rhoD[p * rd_stride + k].
+int launch_transport_props(const DevMech& m, const double* state, int n, double threshold, double* mu,
+                           double* lambda, int ml_stride, double* rhoD, int rd_stride, hipStream_t stream);
+// B rows of the diffusion systems from rho (row 0) and cp (row 1) in place:
+// 1 / rho and 1 / (rho cp) (FlameSolver::prepareIntegrators)
+int launch_diffusion_B(int n, double* B, hipStream_t stream);
+
 int launch_thermo_props(const DevMech& m, const double* state, int n, double* rho, double* Wmx,
                         double* cp, double* cpSpec, double* hk, hipStream_t stream);
 

@@ -91,6 +91,26 @@ std::string read_table(const std::string& path, HostMech& m) {
             r->cheb.resize((size_t)r->chebT * r->chebP);
             for (auto& c : r->cheb)
                 if (err.empty() && fscanf(f, "%lf", &c) != 1) err = "bad CHEB data";
+        } else if (!strcmp(tag, "TRANSPORT")) {
+            if (fscanf(f, "%lf %lf", &m.trTmin, &m.trTmax) != 2 || m.K <= 0) {
+                err = "bad TRANSPORT record";
+                break;
+            }
+            m.visc.assign((size_t)m.K * 5, 0.0);
+            m.cond.assign((size_t)m.K * 5, 0.0);
+            m.diff.assign((size_t)m.K * (m.K + 1) / 2 * 5, 0.0);
+        } else if (!strcmp(tag, "TV") || !strcmp(tag, "TC") || !strcmp(tag, "TD")) {
+            int j = 0, k = 0;
+            const bool pair = tag[1] == 'D';
+            if (!m.has_transport() || fscanf(f, "%d", &j) != 1 || (pair && fscanf(f, "%d", &k) != 1) ||
+                j < 0 || j >= m.K || k < 0 || k >= m.K || (pair && k < j)) {
+                err = "bad transport fit record";
+                break;
+            }
+            double* c = tag[1] == 'V' ? &m.visc[(size_t)j * 5]
+                      : tag[1] == 'C' ? &m.cond[(size_t)j * 5] : &m.diff[(size_t)pair_index(j, k, m.K) * 5];
+            for (int i = 0; i < 5 && err.empty(); i++)
+                if (fscanf(f, "%lf", &c[i]) != 1) err = "bad transport fit data";
         } else {
             err = std::string("unknown record ") + tag;
         }

@@ -35,7 +35,19 @@ struct HostMech {
     std::vector<std::string> species;
     std::vector<double> W, Tmid, nasa;  // nasa: K x 14
     std::vector<HostRxn> rxn;
+    // transport fits (TRANSPORT records), five coefficients of a polynomial in ln T each:
+    // visc K x 5 (sqrt(mu_k)/T^(1/4)), cond K x 5 (lambda_k/sqrt(T)), diff K(K+1)/2 x 5
+    // (D_jk(T, 1 Pa)/T^(3/2), pairs j <= k row by row); empty when the table has none
+    double trTmin = 0, trTmax = 0;
+    std::vector<double> visc, cond, diff;
+    bool has_transport() const { return !visc.empty(); }
 };
+
+// index of the pair (j, k) in the upper triangle j <= k, row by row (MixTransport's ic)
+inline int pair_index(int j, int k, int K) {
+    if (k < j) { int t = j; j = k; k = t; }
+    return K * j - (j - 1) * j / 2 + k - j;
+}
 
 // Returns an error message, empty on success.
 std::string read_table(const std::string& path, HostMech& out);

@@ -48,6 +48,17 @@ __global__ void split_constants_kernel(size_t n, int balanced, double* ddt_conv,
     }
 }
 
+// FlameSolver::prepareIntegrators (src/flameSolver.cpp:160-172): B of the momentum and
+// species systems is rho.inverse(), of the energy system (rho * cp).inverse(); B holds rho
+// in row 0 and cp in row 1 on entry
+__global__ void diffusion_B_kernel(int n, double* B) {
+    const size_t i = gid();
+    if (i >= (size_t)n) return;
+    const double rho = B[i], cp = B[n + i];
+    B[i] = 1.0 / rho;
+    B[n + i] = 1.0 / (rho * cp);
+}
+
 __global__ void fill_kernel(size_t n, double v, double* a, double* b) {
     const size_t i = gid();
     if (i >= n) return;
@@ -101,6 +112,12 @@ int launch_split_constants(size_t n, int balanced, double* ddt_conv, const doubl
                            double* sc_prod, hipStream_t st) {
     hipLaunchKernelGGL(split_constants_kernel, grid_for(n), dim3(kSplitThreads), 0, st, n, balanced,
                        ddt_conv, ddt_diff, ddt_prod, ddt_cross, sc_diff, sc_prod);
+    return (int)hipGetLastError();
+}
+
+int launch_diffusion_B(int n, double* B, hipStream_t stream) {
+    const int blocks = (n + kSplitThreads - 1) / kSplitThreads;
+    hipLaunchKernelGGL(diffusion_B_kernel, dim3(blocks), dim3(kSplitThreads), 0, stream, n, B);
     return (int)hipGetLastError();
 }
 

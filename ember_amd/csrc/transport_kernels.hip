@@ -1,0 +1,146 @@
+// Transport properties of FlameSolver::updateChemicalProperties (src/flameSolver.cpp:678-692)
+// with Ember's default transport model, ApproxMixTransport (src/chemistry0d.cpp:12-261):
+// the mixture viscosity, thermal conductivity and mass-weighted mixture diffusion
+// coefficients rho D_k of every point of a point-major state.
+//
+// One wavefront per point, lane k = species k.  The pure-species viscosities and
+// conductivities and the binary diffusion coefficients come from the mechanism table's
+// temperature fits, evaluated as Cantera's MixTransport evaluates them (update_T: sqrt(T),
+// T^(1/4) and [1, ln T, ..., ln^4 T]; the products summed left to right as dot5):
+//   sqrt(mu_k) = T^(1/4) dot5(p, visc_k),  lambda_k = sqrt(T) dot5(p, cond_k),
+//   D_jk(T, 1 Pa) = T sqrt(T) dot5(p, diff_jk).
+// Mole fractions: Phase::setMassFractions (clip, normalise), then MixTransport::update_C's
+// floor X_k >= 1e-20; the species with X_k >= threshold are ApproxMixTransport's majors.
+//   mu      = sum_{k major} X_k mu_k / sum_{j major} X_j Phi_kj   (Wilke, with Phi_jk for
+//             j > k formed from Phi_kj as ApproxMixTransport::updateViscosity_T forms it)
+//   lambda  = (sum_k X_k lambda_k + 1 / sum_k X_k / lambda_k) / 2
+//   rho D_k = rho / (P sum_i X_i / D_ki + P X_k / (W - W_k X_k) sum_i X_i W_i / D_ki),
+//             i != k over all species for a major k, over the majors for a minor one
+//             (ApproxMixTransport::getMixDiffCoeffsMass, CanteraGas::getWeighted...Mass).
+#include <hip/hip_runtime.h>
+
+#include "chem_device.h"
+#include "kernels.h"
+
+namespace emx {
+
+namespace {
+
+__device__ __forceinline__ double dot5(const double (&p)[5], const double* c) {
+    return p[0] * c[0] + p[1] * c[1] + p[2] * c[2] + p[3] * c[3] + p[4] * c[4];
+}
+
+__device__ __forceinline__ int pair_slot(int j, int k, int K) {
+    if (k < j) {
+        const int t = j;
+        j = k;
+        k = t;
+    }
+    return K * j - (j - 1) * j / 2 + k - j;
+}
+
+// Wilke's Phi_kj for j <= k (ApproxMixTransport::updateViscosity_T's direct form):
+// (1 + (sqrt(mu_k) / sqrt(mu_j)) (W_j / W_k)^(1/4))^2 / (sqrt(8) sqrt(1 + W_k / W_j))
+__device__ __forceinline__ double phi_direct(double sqk, double sqj, double Wk, double Wj) {
+    const double wratjk = sqrt(sqrt(Wj / Wk));
+    const double wratkj1 = sqrt(1.0 + Wk / Wj);
+    const double f = 1.0 + (sqk / sqj) * wratjk;
+    return f * f / (sqrt(8.0) * wratkj1);
+}
+
+constexpr int kWavesPerBlock = 4;
+
+__global__ void __launch_bounds__(64 * kWavesPerBlock) transport_props_kernel(
+    DevMech m, const double* __restrict__ state, int n, double threshold, double* __restrict__ mu_out,
+    double* __restrict__ lambda_out, int ml_stride, double* __restrict__ rhoD_out, int rd_stride) {
+    __shared__ double sX[kWavesPerBlock][64], sSq[kWavesPerBlock][64], sVisc[kWavesPerBlock][64];
+    const int lane = lane_id();
+    const int wave = wave_in_block();
+    const int p = blockIdx.x * kWavesPerBlock + wave;
+    if (p >= n) return;  // whole wavefronts leave together
+    const int K = m.K, nv = m.nVars;
+    const bool sp = lane < K;
+    const double* s = state + (size_t)p * nv;
+    const double T = s[1];
+    const double y = sp ? s[2 + lane] : 0.0;
+    const double Wk = sp ? m.W[lane] : 1.0;
+    // Phase::setMassFractions: clip negatives, normalise; mean molecular weight
+    const double yc = fmax(y, 0.0);
+    const double norm = wave_sum(yc);
+    const double ym = sp ? (yc * (1.0 / norm)) * (1.0 / Wk) : 0.0;
+    const double mmw = 1.0 / wave_sum(ym);
+    const double dens = m.P * mmw / (T * kGasConstant);
+    const double X = sp ? fmax(1e-20, ym * mmw) : 0.0;
+    const bool major = sp && X >= threshold;
+    // MixTransport::update_T
+    const double logt = log(T), sqrt_t = sqrt(T), t14 = sqrt(sqrt_t);
+    const double pt[5] = {1.0, logt, logt * logt, logt * logt * logt, logt * logt * logt * logt};
+    double sqv = 0.0, visc = 0.0, cond = 1.0;
+    if (sp) {
+        sqv = t14 * dot5(pt, m.tr_visc + 5 * lane);
+        visc = sqv * sqv;
+        cond = sqrt_t * dot5(pt, m.tr_cond + 5 * lane);
+    }
+    // MixTransport::thermalConductivity
+    const double lam = 0.5 * (wave_sum(sp ? X * cond : 0.0) + 1.0 / wave_sum(sp ? X / cond : 0.0));
+    sX[wave][lane] = X;  // species lanes; 0 elsewhere
+    sSq[wave][lane] = sqv;
+    sVisc[wave][lane] = visc;
+    wave_lds_sync();
+    // ApproxMixTransport::viscosity over the major species
+    double contrib = 0.0;
+    if (major) {
+        double sum = 0.0;
+        for (int j = 0; j < K; j++) {
+            const double Xj = sX[wave][j];
+            if (!(Xj >= threshold)) continue;
+            const double Wj = m.W[j];
+            double phi;
+            if (j <= lane) {
+                phi = phi_direct(sqv, sSq[wave][j], Wk, Wj);
+            } else {
+                // the pair's Phi_jk, then Phi_kj = Phi_jk / (vratiokj wratiojk) with
+                // vratiokj = mu_j / mu_k and wratiojk = W_k / W_j (j > k)
+                const double vratio = sVisc[wave][j] / visc;
+                const double wratio = Wk / Wj;
+                phi = phi_direct(sSq[wave][j], sqv, Wj, Wk) / (vratio * wratio);
+            }
+            sum += Xj * phi;
+        }
+        contrib = X * visc / sum;
+    }
+    const double mu = wave_sum(contrib);
+    // ApproxMixTransport::getMixDiffCoeffsMass (pairs with a major member, as
+    // ApproxMixTransport::updateDiff_T fills them), times rho
+    if (sp) {
+        const double tts = T * sqrt_t;
+        double s1 = 0.0, s2 = 0.0;
+        for (int i = 0; i < K; i++) {
+            if (i == lane) continue;
+            const double Xi = sX[wave][i];
+            if (!major && !(Xi >= threshold)) continue;
+            const double b = tts * dot5(pt, m.tr_diff + 5 * pair_slot(lane, i, K));
+            s1 += Xi / b;
+            s2 += Xi * m.W[i] / b;
+        }
+        s1 *= m.P;
+        s2 *= m.P * X / (mmw - Wk * X);
+        rhoD_out[(size_t)p * rd_stride + lane] = dens * (1.0 / (s1 + s2));
+    }
+    if (lane == 0) {
+        mu_out[(size_t)p * ml_stride] = mu;
+        lambda_out[(size_t)p * ml_stride] = lam;
+    }
+}
+
+}  // namespace
+
+int launch_transport_props(const DevMech& m, const double* state, int n, double threshold, double* mu,
+                           double* lambda, int ml_stride, double* rhoD, int rd_stride, hipStream_t stream) {
+    const int blocks = (n + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL(transport_props_kernel, dim3(blocks), dim3(64 * kWavesPerBlock), 0, stream, m, state,
+                       n, threshold, mu, lambda, ml_stride, rhoD, rd_stride);
+    return (int)hipGetLastError();
+}
+
+}  // namespace emx

@@ -84,6 +84,20 @@ class Mechanism:
     nasa_high: np.ndarray       # [K, 7]
     reactions: List[Reaction]
     p_ref: float = ONE_ATM
+    # per-species transport parameters (Cantera YAML `transport` blocks); None when the
+    # file has none for some species
+    transport: Optional[list] = None
+
+    def transport_fits(self):
+        """Temperature fits of the pure-species viscosities and conductivities and the
+        binary diffusion coefficients (ember_amd.transport), over the temperature range
+        where every species' NASA7 polynomials hold."""
+        if self.transport is None:
+            return None
+        from . import transport as tr
+        Tmin = float(self.T_ranges[:, 0].max())
+        Tmax = float(self.T_ranges[:, 2].min())
+        return tr.fit(self.transport, self.W, self.cp_R, Tmin, Tmax)
 
     @property
     def n_species(self) -> int:
@@ -157,6 +171,19 @@ class Mechanism:
                 nT, nP = r.cheb.shape
                 out.append(f"CHEB {nT} {nP} " + " ".join(repr(float(v)) for v in r.cheb_range) + " " +
                            " ".join(repr(float(v)) for v in r.cheb.ravel()))
+        fits = self.transport_fits()
+        if fits is not None:
+            # TRANSPORT Tmin Tmax, then per species TV k (sqrt(mu)/T^(1/4)) and TC k
+            # (lambda/sqrt(T)), per pair j <= k TD j k (D_jk(P = 1 Pa)/T^(3/2)): five
+            # coefficients of a polynomial in ln T each
+            out.append(f"TRANSPORT {fits.Tmin!r} {fits.Tmax!r}")
+            for k in range(K):
+                out.append(f"TV {k} " + " ".join(repr(float(v)) for v in fits.visc[k]))
+                out.append(f"TC {k} " + " ".join(repr(float(v)) for v in fits.cond[k]))
+            from .transport import pair_index
+            for j in range(K):
+                for k in range(j, K):
+                    out.append(f"TD {j} {k} " + " ".join(repr(float(v)) for v in fits.diff[pair_index(j, k, K)]))
         out.append("END")
         tmp = path + ".tmp"
         with open(tmp, "w") as f:
@@ -370,7 +397,11 @@ def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
         r.duplicate = bool(rd.get("duplicate", False))
         rxns.append(r)
     name = os.path.splitext(os.path.basename(path))[0]
-    return Mechanism(name, elements, names, W, comp, Tr, lo, hi, rxns)
+    transport = None
+    if all("transport" in spdefs[n] for n in names):
+        from .transport import SpeciesTransport
+        transport = [SpeciesTransport.from_yaml(spdefs[n]["transport"]) for n in names]
+    return Mechanism(name, elements, names, W, comp, Tr, lo, hi, rxns, transport=transport)
 
 
 def builtin_yaml(name: str) -> str:

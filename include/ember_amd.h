@@ -62,6 +62,9 @@ typedef struct emx_options {
      * Off while energy = 0 (the default); on, it needs emx_set_point_positions. */
     double ignition_t_start, ignition_duration, ignition_energy, ignition_center,
         ignition_stddev;
+    /* ConfigOptions transportThreshold (python/ember/input.py class Chemistry, threshold):
+     * ApproxMixTransport's mole-fraction threshold of the major species. */
+    double transport_threshold;
 } emx_options;
 
 /* One-dimensional grid for the diffusion systems (OneDimGrid, src/grid.h:120-126). */
@@ -322,6 +325,34 @@ int emx_heat_release_device(emx_context* ctx, size_t nVars, size_t nPoints, cons
 int emx_thermo_properties_device(emx_context* ctx, size_t nVars, size_t nPoints,
                                  const double* state, double* rho, double* Wmx, double* cp,
                                  double* cpSpec, double* hk);
+
+/* FlameSolver::updateChemicalProperties, transport block (src/flameSolver.cpp:678-692)
+ * with Ember's default transport model "Approx" (ApproxMixTransport,
+ * src/chemistry0d.cpp:12-261; ConfigOptions transportThreshold = transport_threshold):
+ * setStateMass(Y_j, T_j), then getViscosity, getThermalConductivity and
+ * getWeightedDiffusionCoefficientsMass (chemistry0d.cpp:549-596) for every point of the
+ * device state matrix.  Device pointers: mu [Pa s], lambda [W/m/K] (nPoints each), rhoD
+ * [kg/m/s] (nPoints x nSpecies, point-major as Ember's rhoD matrix).  The pure-species
+ * and binary properties come from the mechanism table's temperature fits (the species'
+ * `transport` blocks; -1 without them).  Soret coefficients (soretDiffusion) are not
+ * computed. */
+int emx_transport_properties_device(emx_context* ctx, size_t nVars, size_t nPoints,
+                                    const double* state, double* mu, double* lambda, double* rhoD);
+
+/* The same on host arrays (synchronous). */
+int emx_transport_properties(emx_context* ctx, size_t nVars, size_t nPoints, const double* state,
+                             double* mu, double* lambda, double* rhoD);
+
+/* The diffusion coefficients of the split step from the device state: the property update
+ * (updateChemicalProperties, thermodynamic and transport blocks) followed by
+ * FlameSolver::prepareIntegrators' diffusion inputs (src/flameSolver.cpp:160-172):
+ *   B  2 x nPoints, row 0 = 1/rho (momentum and species), row 1 = 1/(rho cp) (energy),
+ *      for B_row = {0, 1, 0, 0, ...};
+ *   D  nVars x nPoints (point-major): mu, lambda, rho D_0 .. rho D_{K-1}.
+ * Device pointers, asynchronous on the context stream; the inputs of
+ * emx_split_step_device / emx_diffusion_integrate_device. */
+int emx_diffusion_coefficients_device(emx_context* ctx, size_t nVars, size_t nPoints,
+                                      const double* state, double* B, double* D);
 
 /* The same on host arrays, the drop-in for the TBB loop over points (copied through the
  * context's staging buffers; returns after the results are on the host). */

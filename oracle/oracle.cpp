@@ -86,6 +86,10 @@ struct Mech {
     std::vector<double> kf_const, kf_slope, kc_const, kc_slope;  // [n][I]
     bool interp_ready = false;
     double interp_P = -1;
+    // transport fits (TRANSPORT records): coefficients of polynomials in ln T for
+    // sqrt(mu_k)/T^(1/4) and lambda_k/sqrt(T) (K x 5) and D_jk(T, 1 Pa)/T^(3/2) (K x K x 5)
+    double trTmin = 0, trTmax = 0;
+    std::vector<double> visc, cond, diff;
 };
 
 // ChebyshevFunction::a (src/scalarFunction.cpp:53-72): t mapped onto [-1, 1] over
@@ -191,6 +195,26 @@ Mech* load_mech(const char* path) {
                         &r->chebPmin, &r->chebPmax) == 6 && r->chebT >= 1 && r->chebP >= 1;
             if (ok) r->cheb.resize((size_t)r->chebT * r->chebP);
             for (size_t j = 0; ok && j < r->cheb.size(); j++) ok = fscanf(f, "%lf", &r->cheb[j]) == 1;
+        } else if (!strcmp(tag, "TRANSPORT")) {
+            ok = fscanf(f, "%lf %lf", &m->trTmin, &m->trTmax) == 2 && m->K > 0;
+            m->visc.assign((size_t)m->K * 5, 0.0);
+            m->cond.assign((size_t)m->K * 5, 0.0);
+            m->diff.assign((size_t)m->K * m->K * 5, 0.0);  // full symmetric K x K
+        } else if (!strcmp(tag, "TV") || !strcmp(tag, "TC") || !strcmp(tag, "TD")) {
+            int j = 0, k = 0;
+            const bool pair = tag[1] == 'D';
+            ok = !m->visc.empty() && fscanf(f, "%d", &j) == 1 && (!pair || fscanf(f, "%d", &k) == 1) &&
+                 j >= 0 && j < m->K && k >= 0 && k < m->K;
+            double c[5];
+            for (int i = 0; i < 5 && ok; i++) ok = fscanf(f, "%lf", &c[i]) == 1;
+            if (ok) {
+                if (tag[1] == 'V') std::copy(c, c + 5, &m->visc[(size_t)j * 5]);
+                else if (tag[1] == 'C') std::copy(c, c + 5, &m->cond[(size_t)j * 5]);
+                else {
+                    std::copy(c, c + 5, &m->diff[((size_t)j * m->K + k) * 5]);
+                    std::copy(c, c + 5, &m->diff[((size_t)k * m->K + j) * 5]);
+                }
+            }
         } else {
             ok = false;
         }
@@ -1361,6 +1385,118 @@ int orc_thermo_properties(void* mech, double pressure, int nPoints, const double
             if (cpSpec)
                 for (int k = 0; k < m.K; k++) cpSpec[(size_t)j * m.K + k] = kGasConstant * g.cpR[k];
             if (hk) g.enthalpies(hk + (size_t)j * m.K);
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Transport properties: the transport block of FlameSolver::updateChemicalProperties
+// (src/flameSolver.cpp:678-692) with Ember's default model, ApproxMixTransport
+// (src/chemistry0d.cpp:12-261) over Cantera's MixTransport: mu (ApproxMixTransport::
+// viscosity, Wilke's rule over the species with X_k >= threshold), lambda
+// (MixTransport::thermalConductivity: 0.5 (sum X_k lambda_k + 1 / sum X_k / lambda_k)) and
+// rho D_k (CanteraGas::getWeightedDiffusionCoefficientsMass: rho times
+// ApproxMixTransport::getMixDiffCoeffsMass).  Pure-species and binary properties come
+// from the table's temperature fits as MixTransport evaluates them (update_T: sqrt(T),
+// T^(1/4), [1, ln T, ..., ln^4 T]; dot5).  Mole fractions carry MixTransport::update_C's
+// floor of 1e-20 (Cantera's Tiny).
+// ---------------------------------------------------------------------------
+namespace {
+double dot5(const double* x, const double* c) {
+    return x[0] * c[0] + x[1] * c[1] + x[2] * c[2] + x[3] * c[3] + x[4] * c[4];
+}
+}  // namespace
+
+int orc_transport_properties(void* mech, double pressure, double threshold, int nPoints,
+                             const double* state, double* mu, double* lambda, double* rhoD,
+                             int nthreads) {
+    Mech& m = *static_cast<Mech*>(mech);
+    if (m.visc.empty()) return -1;
+    const int K = m.K, nv = K + 2;
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+    {
+        Gas g(m, 0, pressure);
+        std::vector<double> X(K), visc(K), sqvisc(K), cond(K), phi((size_t)K * K), bdiff((size_t)K * K);
+        std::vector<int> major;
+#pragma omp for schedule(dynamic, 1)
+        for (int p = 0; p < nPoints; p++) {
+            const double* s = state + (size_t)p * nv;
+            g.setMassFractions(s + 2);
+            g.setState_TP(s[1]);
+            const double T = g.T;
+            // MixTransport::update_T
+            const double logt = std::log(T), sqrt_t = std::sqrt(T), t14 = std::sqrt(sqrt_t);
+            const double ptv[5] = {1.0, logt, logt * logt, logt * logt * logt, logt * logt * logt * logt};
+            // MixTransport::update_C and ApproxMixTransport::update_C
+            major.clear();
+            for (int k = 0; k < K; k++) {
+                X[k] = std::max(1e-20, g.ym[k] * g.mmw);
+                if (X[k] >= threshold) major.push_back(k);
+            }
+            // GasTransport::updateSpeciesViscosities, MixTransport::updateCond_T
+            for (int k = 0; k < K; k++) {
+                sqvisc[k] = t14 * dot5(ptv, &m.visc[(size_t)k * 5]);
+                visc[k] = sqvisc[k] * sqvisc[k];
+                cond[k] = sqrt_t * dot5(ptv, &m.cond[(size_t)k * 5]);
+            }
+            // ApproxMixTransport::updateViscosity_T (Wilke, Reid-Prausnitz-Poling 9-5.15)
+            for (size_t ij = 0; ij < major.size(); ij++) {
+                const int j = major[ij];
+                for (size_t ik = ij; ik < major.size(); ik++) {
+                    const int k = major[ik];
+                    const double vratiokj = visc[k] / visc[j];
+                    const double wratiojk = m.W[j] / m.W[k];
+                    const double wratjk_kj = std::sqrt(std::sqrt(m.W[j] / m.W[k]));
+                    const double wratkj1_jk = std::sqrt(1.0 + m.W[k] / m.W[j]);
+                    const double factor1 = 1.0 + (sqvisc[k] / sqvisc[j]) * wratjk_kj;
+                    phi[(size_t)k * K + j] = factor1 * factor1 / (std::sqrt(8.0) * wratkj1_jk);
+                    phi[(size_t)j * K + k] = phi[(size_t)k * K + j] / (vratiokj * wratiojk);
+                }
+            }
+            double vismix = 0.0;
+            for (int k : major) {
+                double sum = 0.0;
+                for (int j : major) sum += X[j] * phi[(size_t)k * K + j];
+                vismix += X[k] * visc[k] / sum;
+            }
+            mu[p] = vismix;
+            // MixTransport::thermalConductivity
+            double sum1 = 0.0, sum2 = 0.0;
+            for (int k = 0; k < K; k++) {
+                sum1 += X[k] * cond[k];
+                sum2 += X[k] / cond[k];
+            }
+            lambda[p] = 0.5 * (sum1 + 1.0 / sum2);
+            // ApproxMixTransport::updateDiff_T (rows and columns of the major species)
+            for (int j : major)
+                for (int k = 0; k < K; k++) {
+                    const double b = T * sqrt_t * dot5(ptv, &m.diff[((size_t)j * K + k) * 5]);
+                    bdiff[(size_t)j * K + k] = b;
+                    bdiff[(size_t)k * K + j] = b;
+                }
+            // ApproxMixTransport::getMixDiffCoeffsMass, times rho
+            const double mmw = g.mmw, P = pressure;
+            for (int k = 0; k < K; k++) {
+                double s1 = 0.0, s2 = 0.0;
+                if (X[k] >= threshold) {
+                    for (int i = 0; i < K; i++) {
+                        if (i == k) continue;
+                        s1 += X[i] / bdiff[(size_t)k * K + i];
+                        s2 += X[i] * m.W[i] / bdiff[(size_t)k * K + i];
+                    }
+                } else {
+                    for (int i : major) {
+                        if (i == k) continue;
+                        s1 += X[i] / bdiff[(size_t)k * K + i];
+                        s2 += X[i] * m.W[i] / bdiff[(size_t)k * K + i];
+                    }
+                }
+                s1 *= P;
+                s2 *= P * X[k] / (mmw - m.W[k] * X[k]);
+                rhoD[(size_t)p * K + k] = g.dens * (1.0 / (s1 + s2));
+            }
         }
     }
     return 0;

@@ -76,6 +76,14 @@ int orc_thermo_properties(void* mech, double pressure, int nPoints, const double
                           double* rho, double* Wmx, double* cp, double* cpSpec, double* hk,
                           int nthreads);
 
+/* Transport properties of FlameSolver::updateChemicalProperties with Ember's default
+ * transport model ("Approx": ApproxMixTransport with mole-fraction threshold `threshold`)
+ * for nPoints points of a point-major state: mu [Pa s], lambda [W/m/K] per point, rhoD
+ * [kg/m/s] nPoints x nSpecies.  -1 when the table has no transport fits. */
+int orc_transport_properties(void* mech, double pressure, double threshold, int nPoints,
+                             const double* state, double* mu, double* lambda, double* rhoD,
+                             int nthreads);
+
 /* One production stage for nPoints points (FlameSolver::integrateProductionTerms).
  * status: 0 ok, <0 failure (the point keeps its input state).
  * stat1/stat2: QSS gcount/rcount, BDF nsteps/nfe.  qint: integral of qdot over the

@@ -62,6 +62,8 @@ def lib():
                                             _dp, _dp, C.c_int]
         L.orc_source_integrate.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, _dp,
                                            C.c_double, _dp, _ip, _ip, _ip, _dp, C.c_int]
+        L.orc_transport_properties.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                               _dp, _dp, C.c_int]
         L.orc_bdf_work.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, C.c_double, _ip,
                                    C.c_int]
         L.orc_qss_test_problem.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double,
@@ -189,6 +191,18 @@ class Mech:
         lib().orc_source_integrate(self.h, C.byref(opts), n, _p(state), _p(sc), _p(ql), dt, _p(out),
                                    _p(status), _p(s1), _p(s2), _p(qint), nthreads)
         return out, status, s1, s2, qint
+
+    def transport_properties(self, state, pressure=101325.0, threshold=1e-5, nthreads=0):
+        """(mu, lambda, rhoD) of ApproxMixTransport at each point (orc_transport_properties)."""
+        state = np.ascontiguousarray(np.atleast_2d(state), dtype=np.float64)
+        n = state.shape[0]
+        mu, lam = np.zeros(n), np.zeros(n)
+        rhoD = np.zeros((n, self.K))
+        rc = lib().orc_transport_properties(self.h, pressure, threshold, n, _p(state), _p(mu), _p(lam),
+                                            _p(rhoD), nthreads)
+        if rc:
+            raise RuntimeError(f"orc_transport_properties failed: {rc}")
+        return mu, lam, rhoD
 
     def bdf_work(self, state, dt, opts=None, nthreads=0):
         """BDF work per point: (n_rhs, n_jac, n_lu, n_solve) columns (orc_bdf_work)."""
