@@ -50,6 +50,14 @@ class Grid(C.Structure):
                 ("wall_var", C.c_int32)]
 
 
+class ConvectionParams(C.Structure):
+    _fields_ = [("alpha", C.c_int32), ("beta", C.c_int32), ("left_bc", C.c_int32), ("right_bc", C.c_int32),
+                ("continuity_bc", C.c_int32), ("rVzero", C.c_double), ("Tleft", C.c_double),
+                ("Yleft", C.c_void_p), ("rhou", C.c_double), ("strain_initial", C.c_double),
+                ("strain_final", C.c_double), ("strain_t0", C.c_double), ("strain_dt", C.c_double),
+                ("max_steps", C.c_int32)]
+
+
 class SplitArrays(C.Structure):
     _fields_ = [("state", _vp), ("ddt_conv", _vp), ("ddt_diff", _vp), ("ddt_prod", _vp),
                 ("ddt_cross", _vp), ("delta_diff", _vp), ("delta_prod", _vp), ("work", _vp)]
@@ -95,6 +103,10 @@ SIGNATURES = [
     ("emx_split_step_device", C.c_int,
      [_vp, _sz, _sz, C.POINTER(SplitArrays), _vp, _sz, _vp, _vp, C.POINTER(Grid), _vp, _dp,
       C.c_double, C.c_double, C.c_int, _vp]),
+    ("emx_convection_integrate_device", C.c_int,
+     [_vp, _sz, _sz, C.c_void_p, _vp, _vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp]),
+    ("emx_convection_integrate", C.c_int,
+     [_vp, _sz, _sz, C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _ip]),
     ("emx_correct_mass_fractions_device", C.c_int, [_vp, _sz, _sz, _vp]),
     ("emx_heat_release_device", C.c_int, [_vp, _sz, _sz, _vp, C.c_double, _vp, _vp]),
     ("emx_thermo_properties", C.c_int, [_vp, _sz, _sz, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -346,6 +358,40 @@ class Context:
             self.h, nV, N, C.byref(sa), _tptr(B, B.shape[0] * N), B.shape[0], _tptr(B_row, nV, _I32),
             _tptr(D, N * nV), C.byref(g), _tptr(qloss, N), _p(hv), t, dt, 1 if balanced else 0,
             _tptr(status, N, _I32)))
+
+    def convection_params(self, Yleft=None, **kw):
+        """emx_convection_params with Ember's defaults (planar, FixedValue ends, fixedLeft
+        continuity, zero strain); Yleft is kept alive on the returned object."""
+        d = dict(alpha=0, beta=1, left_bc=0, right_bc=0, continuity_bc=0, rVzero=0.0, Tleft=300.0,
+                 rhou=1.0, strain_initial=0.0, strain_final=0.0, strain_t0=0.0, strain_dt=1.0, max_steps=0)
+        d.update(kw)
+        p = ConvectionParams(**d)
+        if Yleft is not None:
+            p._yleft = np.ascontiguousarray(Yleft, dtype=np.float64)
+            p.Yleft = p._yleft.ctypes.data
+        return p
+
+    def convection_integrate(self, state, x, t0, t1, params=None, split_const=None, drhodt=None):
+        """Convection stage on host arrays: (state_out, steps (1 + K))."""
+        params = params or self.convection_params()
+        st = np.ascontiguousarray(state, dtype=np.float64).copy()
+        n = st.shape[0]
+        steps = np.zeros(1 + self.n_species, dtype=np.int32)
+        sc = None if split_const is None else np.ascontiguousarray(split_const, dtype=np.float64)
+        dr = None if drhodt is None else np.ascontiguousarray(drhodt, dtype=np.float64)
+        _check(lib().emx_convection_integrate(self.h, self.n_vars, n, C.byref(params),
+                                              _p(np.ascontiguousarray(x, dtype=np.float64)), _p(sc), _p(dr),
+                                              t0, t1, _p(st), _p(steps, _ip)))
+        return st, steps
+
+    def convection_integrate_device(self, state_in, state_out, x, t0, t1, params=None, split_const=None,
+                                    drhodt=None, steps=None):
+        params = params or self.convection_params()
+        n = state_in.shape[0]
+        _check(lib().emx_convection_integrate_device(
+            self.h, self.n_vars, n, C.byref(params), _tptr(x, n), _tptr(state_in, n * self.n_vars),
+            _tptr(split_const, n * self.n_vars), _tptr(drhodt, n), t0, t1, _tptr(state_out, n * self.n_vars),
+            _tptr(steps, 1 + self.n_species, _I32)))
 
     def correct_mass_fractions_device(self, state):
         N, nV = state.shape

@@ -103,6 +103,8 @@ struct emx_context {
     bool qign_valid = false;
     // InterpKinetics table grid (Ember's defaults, grown by emx_set_interp_range or by the
     // host-array stage entry point)
+    // convection stage work arrays
+    DevBuf cv_work, cv_rec_t, cv_rec_v, cv_ints, cv_aux, cv_yleft;
     int interp_n = kInterpN;
     double interp_tmin = kInterpTmin, interp_tmax = kInterpTmax;
 };
@@ -412,7 +414,8 @@ void emx_destroy(emx_context* c) {
                       &c->d_B, &c->d_Brow, &c->d_D, &c->d_sc, &c->d_x, &c->d_y, &c->d_h, &c->d_ns,
                       &c->d_work, &c->d_hmin, &c->t_a, &c->t_b, &c->t_c, &c->t_k, &c->jac, &c->plain_ws,
                       &c->cost, &c->order, &c->ck_y, &c->ck_s, &c->nckpt, &c->record, &c->ratefn,
-                      &c->ones, &c->qign})
+                      &c->ones, &c->qign, &c->cv_work, &c->cv_rec_t, &c->cv_rec_v, &c->cv_ints,
+                      &c->cv_aux, &c->cv_yleft})
         b->release();
     delete c;
 }
@@ -1190,6 +1193,141 @@ int emx_diffusion_coefficients_device(emx_context* c, size_t nVars, size_t nPoin
     // D rows per point: mu, lambda, rho D_k (FlameSolver::prepareIntegrators)
     if (!e) e = launch_transport_props(c->dm, state, n, c->opts.transport_threshold, D, D + 1, nv, D + 2, nv, st);
     return e ? fail(-10, std::string("diffusion coefficient launch: ") + hipGetErrorString((hipError_t)e)) : 0;
+}
+
+// ---------------------------------------------------------------------------
+// convection stage
+// ---------------------------------------------------------------------------
+static constexpr int kConvMaxRecords = 2048;  // v records: UTW steps + 2
+
+int emx_convection_integrate_device(emx_context* c, size_t nVars, size_t nPoints,
+                                    const emx_convection_params* cp, const double* x, const double* state_in,
+                                    const double* split_const, const double* drhodt, double t0, double t1,
+                                    double* state_out, int32_t* steps) {
+    if (int rc = check_ctx(c)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (!cp || !x || !state_in || !state_out) return fail(-1, "params, x or a state pointer is NULL");
+    if (nPoints < 3 || nPoints > (size_t)kConvMaxPoints)
+        return fail(-1, "the convection stage takes 3 to 4096 grid points");
+    if (cp->continuity_bc != 0)
+        return fail(-1, "only the fixedLeft continuity boundary condition is supported");
+    if (cp->alpha < 0 || cp->alpha > 1 || (cp->beta != 1 && cp->beta != 2) || cp->left_bc < 0 ||
+        cp->left_bc > 3 || cp->right_bc < 0 || cp->right_bc > 1)
+        return fail(-1, "invalid grid geometry or boundary condition");
+    if (!(t1 > t0)) return fail(-1, "t1 must exceed t0");
+    const int N = (int)nPoints, K = c->hm.K;
+    hipStream_t st = c->stream;
+    // setLeftBC: Wleft = the mean molecular weight of the (normalised) left composition
+    double Wleft = 0.0;
+    if (cp->Yleft) {
+        double norm = 0.0, sum = 0.0;
+        for (int k = 0; k < K; k++) norm += std::max(cp->Yleft[k], 0.0);
+        if (!(norm > 0.0) || !std::isfinite(norm)) return fail(-1, "Yleft must hold a positive, finite composition");
+        for (int k = 0; k < K; k++) sum += std::max(cp->Yleft[k], 0.0) * (1.0 / norm) * (1.0 / c->hm.W[k]);
+        Wleft = 1.0 / sum;
+        HIP_TRY(c->cv_yleft.ensure(K * sizeof(double)));
+        HIP_TRY(hipMemcpyAsync(c->cv_yleft.p, cp->Yleft, K * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    HIP_TRY(c->cv_work.ensure(convection_work_doubles(N, K) * sizeof(double)));
+    HIP_TRY(c->cv_rec_t.ensure(kConvMaxRecords * sizeof(double)));
+    HIP_TRY(c->cv_rec_v.ensure((size_t)kConvMaxRecords * N * sizeof(double)));
+    HIP_TRY(c->cv_ints.ensure((2 + K) * sizeof(int)));
+    // aux: Wmx, rho, cp, scU, scT, scW, drhodt (zeros when NULL)
+    HIP_TRY(c->cv_aux.ensure(7 * (size_t)N * sizeof(double)));
+    double* aux = c->cv_aux.as<double>();
+    double *Wmx = aux, *rho = aux + N, *cpm = aux + 2 * N, *scU = aux + 3 * N, *scT = aux + 4 * N,
+           *scW = aux + 5 * N, *dr0 = aux + 6 * N;
+    int e = launch_thermo_props(c->dm, state_in, N, rho, Wmx, cpm, nullptr, nullptr, st);
+    if (!e) e = launch_conv_split_constants(c->dm, N, split_const, scU, scT, scW, st);
+    if (!e && !drhodt) e = launch_fill((size_t)N, 0.0, dr0, nullptr, st);
+    if (e) return fail(-10, std::string("convection set-up launch: ") + hipGetErrorString((hipError_t)e));
+    ConvParams p{};
+    p.N = N;
+    p.K = K;
+    p.nv = (int)nVars;
+    p.alpha = cp->alpha;
+    p.beta = cp->beta;
+    p.leftBC = cp->left_bc;
+    p.rightBC = cp->right_bc;
+    p.P = c->P;
+    p.rVzero = cp->rVzero;
+    p.Tleft = cp->Tleft;
+    p.Wleft = Wleft;
+    p.rhou = cp->rhou;
+    p.aInitial = cp->strain_initial;
+    p.aFinal = cp->strain_final;
+    p.T0 = cp->strain_t0;
+    p.Dt = cp->strain_dt;
+    // ConvectionSystemSplit::setTolerances (src/convectionSystem.cpp:448-455)
+    p.rtol = c->opts.integrator_rel_tol;
+    p.atolU = c->opts.integrator_momentum_abs_tol;
+    p.atolT = c->opts.integrator_energy_abs_tol;
+    p.atolW = c->opts.integrator_species_abs_tol * 20;
+    p.atolY = c->opts.integrator_species_abs_tol;
+    p.t0 = t0;
+    p.t1 = t1;
+    p.maxSteps = cp->max_steps > 0 ? cp->max_steps : 1000000;
+    p.maxRec = kConvMaxRecords;
+    ConvArrays a{};
+    a.state = state_in;
+    a.sc = split_const;
+    a.x = x;
+    a.drhodt = drhodt ? drhodt : dr0;
+    a.Wmx = Wmx;
+    a.scU = scU;
+    a.scT = scT;
+    a.scW = scW;
+    a.Yleft = cp->Yleft ? c->cv_yleft.as<double>() : nullptr;
+    a.state_out = state_out;
+    a.rec_t = c->cv_rec_t.as<double>();
+    a.rec_v = c->cv_rec_v.as<double>();
+    a.nrec = c->cv_ints.as<int>();
+    a.steps = steps ? steps : c->cv_ints.as<int>() + 1;
+    a.work = c->cv_work.as<double>();
+    e = launch_convection(p, a, st);
+    return e ? fail(-10, std::string("convection launch: ") + hipGetErrorString((hipError_t)e)) : 0;
+}
+
+int emx_convection_integrate(emx_context* c, size_t nVars, size_t nPoints, const emx_convection_params* cp,
+                             const double* x, const double* split_const, const double* drhodt, double t0,
+                             double t1, double* state, int32_t* steps) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!state || !x) return fail(-1, "state or x is NULL");
+    if (nPoints < 3 || nPoints > (size_t)kConvMaxPoints)
+        return fail(-1, "the convection stage takes 3 to 4096 grid points");
+    const size_t NV = nVars * nPoints * sizeof(double);
+    const int K = c->hm.K;
+    hipStream_t st = c->stream;
+    HIP_TRY(c->d_y.ensure(NV));
+    HIP_TRY(c->d_x.ensure(nPoints * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(c->d_y.p, state, NV, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_x.p, x, nPoints * sizeof(double), hipMemcpyHostToDevice, st));
+    const double* sc = nullptr;
+    if (split_const) {
+        HIP_TRY(c->d_sc.ensure(NV));
+        HIP_TRY(hipMemcpyAsync(c->d_sc.p, split_const, NV, hipMemcpyHostToDevice, st));
+        sc = c->d_sc.as<double>();
+    }
+    const double* dr = nullptr;
+    if (drhodt) {
+        HIP_TRY(c->d_hmin.ensure(nPoints * sizeof(double)));
+        HIP_TRY(hipMemcpyAsync(c->d_hmin.p, drhodt, nPoints * sizeof(double), hipMemcpyHostToDevice, st));
+        dr = c->d_hmin.as<double>();
+    }
+    HIP_TRY(c->b_status.ensure((1 + K) * sizeof(int32_t)));
+    int rc = emx_convection_integrate_device(c, nVars, nPoints, cp, c->d_x.as<double>(), c->d_y.as<double>(), sc,
+                                             dr, t0, t1, c->d_y.as<double>(), c->b_status.as<int32_t>());
+    if (rc) return rc;
+    std::vector<int32_t> s(1 + K);
+    HIP_TRY(hipMemcpyAsync(s.data(), c->b_status.p, (1 + K) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(state, c->d_y.p, NV, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (steps) memcpy(steps, s.data(), (1 + K) * sizeof(int32_t));
+    for (int i = 0; i <= K; i++)
+        if (s[i] < 0)
+            return fail(-3, std::string("convection integration failed (") + (i == 0 ? "UTW system" : "species") +
+                                ", code " + std::to_string(s[i]) + ")");
+    return 0;
 }
 
 int emx_tridiagonal_integrate(emx_context* c, size_t nSys, size_t N, const double* a, const double* b,

@@ -31,6 +31,40 @@ int launch_time_derivatives(size_t n, double dt, const double* delta_diff, const
 int launch_correct_mass_fractions(int nVars, int nPoints, double* state, hipStream_t st);
 int launch_heat_release(const DevMech& m, const DevOpts& o, const double* state, const double* perturb, int n,
                         double* qdot, int* status, hipStream_t stream);
+// Convection stage (convection_kernels.hip)
+constexpr int kConvMaxPoints = 4096;  // grid points of one convection system (LDS rows)
+struct ConvParams {
+    int N, K, nv, alpha, beta, leftBC, rightBC;
+    double P, rVzero, Tleft, Wleft, rhou;
+    double aInitial, aFinal, T0, Dt;  // Ember's LinearFunction strain rate
+    double rtol, atolU, atolT, atolW, atolY;
+    double t0, t1;
+    int maxSteps, maxRec;
+};
+struct ConvArrays {
+    const double* state;     // N x nv input
+    const double* sc;        // N x nv split constants (nullable)
+    const double* x;         // N
+    const double* drhodt;    // N
+    const double* Wmx;       // N mean molecular weights of the input state
+    const double* scU;       // N
+    const double* scT;       // N
+    const double* scW;       // N: sum_k splitConst_Yk / W_k
+    const double* Yleft;     // K (nullable)
+    double* state_out;       // N x nv
+    double* rec_t;           // maxRec
+    double* rec_v;           // maxRec x N
+    int* nrec;               // 1
+    int* steps;              // 1 + K
+    double* work;            // convection_work_doubles
+};
+size_t convection_work_doubles(int N, int K);
+int launch_convection(const ConvParams& p, const ConvArrays& a, hipStream_t stream);
+// split constants of the UTW system from the convection split constants (N x nv):
+// scU, scT rows and scW = sum_k sc_Yk / W_k
+int launch_conv_split_constants(const DevMech& m, int N, const double* sc, double* scU, double* scT,
+                                double* scW, hipStream_t stream);
+
 // ApproxMixTransport properties (mu, lambda, rho D_k) of n points; needs m.tr_* fits.
 // mu and lambda of point p at [p * ml_stride], rho D_k at rhoD[p * rd_stride + k].
 int launch_transport_props(const DevMech& m, const double* state, int n, double threshold, double* mu,

@@ -59,6 +59,24 @@ __global__ void diffusion_B_kernel(int n, double* B) {
     B[n + i] = 1.0 / (rho * cp);
 }
 
+// ConvectionSystemSplit::setSplitConstants (src/convectionSystem.cpp:573-586)
+__global__ void conv_split_constants_kernel(int N, int nv, const double* __restrict__ sc,
+                                            const double* __restrict__ W, double* scU, double* scT,
+                                            double* scW) {
+    const size_t j = gid();
+    if (j >= (size_t)N) return;
+    if (!sc) {
+        scU[j] = scT[j] = scW[j] = 0.0;
+        return;
+    }
+    const double* r = sc + j * nv;
+    scU[j] = r[0];
+    scT[j] = r[1];
+    double w = 0.0;
+    for (int k = 0; k < nv - 2; k++) w += r[2 + k] / W[k];
+    scW[j] = w;
+}
+
 __global__ void fill_kernel(size_t n, double v, double* a, double* b) {
     const size_t i = gid();
     if (i >= n) return;
@@ -118,6 +136,14 @@ int launch_split_constants(size_t n, int balanced, double* ddt_conv, const doubl
 int launch_diffusion_B(int n, double* B, hipStream_t stream) {
     const int blocks = (n + kSplitThreads - 1) / kSplitThreads;
     hipLaunchKernelGGL(diffusion_B_kernel, dim3(blocks), dim3(kSplitThreads), 0, stream, n, B);
+    return (int)hipGetLastError();
+}
+
+int launch_conv_split_constants(const DevMech& m, int N, const double* sc, double* scU, double* scT,
+                                double* scW, hipStream_t stream) {
+    const int blocks = (N + kSplitThreads - 1) / kSplitThreads;
+    hipLaunchKernelGGL(conv_split_constants_kernel, dim3(blocks), dim3(kSplitThreads), 0, stream, N, m.nVars,
+                       sc, m.W, scU, scT, scW);
     return (int)hipGetLastError();
 }
 

@@ -302,6 +302,45 @@ int emx_split_step_device(emx_context* ctx, size_t nVars, size_t nPoints,
                           const double* qloss, const double* h_per_var, double t, double dt,
                           int balanced, int32_t* status);
 
+/* Convection stage: ConvectionSystemSplit::integrateToTime (src/convectionSystem.cpp:588-627)
+ * as FlameSolver::integrateConvectionTerms runs it (src/flameSolver.cpp:720-750): the
+ * (U, T, Wmx) system of all points (ConvectionSystemUTW, continuity condition fixedLeft),
+ * then every species' advection by the recorded v = V / rho (ConvectionSystemY), with
+ * Ember's tolerances (integrator_rel_tol, momentum / energy abs tol, 20 x species abs tol
+ * for Wmx, species abs tol).  CVODE's Adams method is replaced by the explicit
+ * Dormand-Prince 5(4) pair with error control (no counterpart in the reference; the step
+ * sequence differs from CVODE's, the tolerances are Ember's).  3 to 4096 grid points. */
+typedef struct emx_convection_params {
+    int32_t alpha;          /* OneDimGrid::alpha: 0 planar, 1 cylindrical */
+    int32_t beta;           /* OneDimGrid::beta: 1, or 2 for disc (axisymmetric jet) flames */
+    int32_t left_bc, right_bc;  /* EMX_BC_* of grid.leftBC / rightBC */
+    int32_t continuity_bc;  /* 0: fixedLeft (ContinuityBoundaryCondition::Left); others refused */
+    double rVzero;          /* ConvectionSystemUTW::rVzero */
+    double Tleft;           /* setLeftBC: inflow temperature (ControlVolume / WallFlux left end) */
+    const double* Yleft;    /* setLeftBC: nSpecies inflow mass fractions, host (NULL: zeros) */
+    double rhou;            /* FlameSolver::rhou, unburned density of the strain term */
+    double strain_initial, strain_final, strain_t0, strain_dt;  /* LinearFunction strain rate */
+    int32_t max_steps;      /* per system (0: CVODE's 1000000 of ConvectionSystemSplit::setState) */
+} emx_convection_params;
+
+/* Device pointers: x (nPoints), state_in / state_out (nVars x nPoints; may be the same
+ * array), split_const (splitConstConv, nullable = zeros), drhodt (nPoints, nullable =
+ * zeros; FlameSolver::drhodt).  steps (device, 1 + nSpecies, nullable): accepted steps of
+ * the UTW system and of each species system, < 0 for a failed system (its variables are
+ * not written).  Asynchronous on the context stream. */
+int emx_convection_integrate_device(emx_context* ctx, size_t nVars, size_t nPoints,
+                                    const emx_convection_params* params, const double* x,
+                                    const double* state_in, const double* split_const,
+                                    const double* drhodt, double t0, double t1, double* state_out,
+                                    int32_t* steps);
+
+/* The same on host arrays, state in/out; synchronous; returns -3 when a system failed
+ * (steps, host, nullable, as above). */
+int emx_convection_integrate(emx_context* ctx, size_t nVars, size_t nPoints,
+                             const emx_convection_params* params, const double* x,
+                             const double* split_const, const double* drhodt, double t0, double t1,
+                             double* state, int32_t* steps);
+
 /* FlameSolver::correctMassFractions (src/flameSolver.cpp:942-949): per point, negative
  * mass fractions clipped to zero and the rest scaled to sum to one (Cantera
  * Phase::setMassFractions).  state: nVars x nPoints device matrix, in place, async. */

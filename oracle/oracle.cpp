@@ -1869,3 +1869,5 @@ int orc_qss_fd_gradient(void* mech, const orc_options* o, int nPoints, const dou
 }
 
 }  // extern "C"
+
+#include "convection.inc"

@@ -84,6 +84,28 @@ int orc_transport_properties(void* mech, double pressure, double threshold, int 
                              const double* state, double* mu, double* lambda, double* rhoD,
                              int nthreads);
 
+/* Convection stage (ConvectionSystemSplit::integrateToTime, src/convectionSystem.cpp)
+ * for continuity_bc 0 (fixedLeft).  Boundary codes as DiffusionSystem's; strain rate
+ * Ember's LinearFunction (strain_initial until strain_t0, linear over strain_dt, then
+ * strain_final); tolerances as ConvectionSystemSplit::setTolerances (the W atol is 20 x
+ * atol_species). */
+typedef struct orc_conv_params {
+    int alpha, beta, left_bc, right_bc, continuity_bc;
+    double pressure, rVzero, Tleft, rhou;
+    double strain_initial, strain_final, strain_t0, strain_dt;
+    double rtol, atol_momentum, atol_energy, atol_species;
+    int max_steps;
+} orc_conv_params;
+
+/* state, split_const: N x nVars point-major; drhodt: N (nullable = 0); Yleft: nSpecies
+ * (nullable = 0).  state_out receives U, T and Y_k after [t0, t1].  steps_out (nullable,
+ * 1 + nSpecies): accepted steps of the UTW system and of each species system.  Returns 0,
+ * or < 0 on an integration failure (-1 step limit, -3 step size, -4 non-finite). */
+int orc_convection_integrate(void* mech, const orc_conv_params* p, int N, const double* x,
+                             const double* state, const double* split_const, const double* drhodt,
+                             const double* Yleft, double t0, double t1, double* state_out,
+                             int* steps_out, int nthreads);
+
 /* One production stage for nPoints points (FlameSolver::integrateProductionTerms).
  * status: 0 ok, <0 failure (the point keeps its input state).
  * stat1/stat2: QSS gcount/rcount, BDF nsteps/nfe.  qint: integral of qdot over the

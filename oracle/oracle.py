@@ -31,6 +31,25 @@ class Options(C.Structure):
                 ("positions", C.c_void_p)]
 
 
+class ConvParams(C.Structure):
+    _fields_ = [("alpha", C.c_int), ("beta", C.c_int), ("left_bc", C.c_int), ("right_bc", C.c_int),
+                ("continuity_bc", C.c_int), ("pressure", C.c_double), ("rVzero", C.c_double),
+                ("Tleft", C.c_double), ("rhou", C.c_double), ("strain_initial", C.c_double),
+                ("strain_final", C.c_double), ("strain_t0", C.c_double), ("strain_dt", C.c_double),
+                ("rtol", C.c_double), ("atol_momentum", C.c_double), ("atol_energy", C.c_double),
+                ("atol_species", C.c_double), ("max_steps", C.c_int)]
+
+
+def conv_params(**kw) -> ConvParams:
+    """Convection-stage parameters with Ember's defaults (planar, FixedValue ends, fixedLeft
+    continuity, constant strain rate, input.py tolerances)."""
+    d = dict(alpha=0, beta=1, left_bc=0, right_bc=0, continuity_bc=0, pressure=101325.0, rVzero=0.0,
+             Tleft=300.0, rhou=1.0, strain_initial=0.0, strain_final=0.0, strain_t0=0.0, strain_dt=1.0,
+             rtol=1e-6, atol_momentum=1e-7, atol_energy=1e-8, atol_species=1e-13, max_steps=1000000)
+    d.update(kw)
+    return ConvParams(**d)
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB
@@ -64,6 +83,8 @@ def lib():
                                            C.c_double, _dp, _ip, _ip, _ip, _dp, C.c_int]
         L.orc_transport_properties.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                                _dp, _dp, C.c_int]
+        L.orc_convection_integrate.argtypes = [C.c_void_p, C.POINTER(ConvParams), C.c_int, _dp, _dp, _dp,
+                                               _dp, _dp, C.c_double, C.c_double, _dp, _ip, C.c_int]
         L.orc_bdf_work.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, C.c_double, _ip,
                                    C.c_int]
         L.orc_qss_test_problem.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double,
@@ -203,6 +224,24 @@ class Mech:
         if rc:
             raise RuntimeError(f"orc_transport_properties failed: {rc}")
         return mu, lam, rhoD
+
+    def convection_integrate(self, x, state, t0, t1, params=None, split_const=None, drhodt=None,
+                             Yleft=None, nthreads=0):
+        """Convection stage over [t0, t1]: (state_out, steps (1 + K: UTW, species))."""
+        params = params or conv_params()
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n = state.shape[0]
+        out = state.copy()
+        steps = np.zeros(1 + self.K, dtype=np.int32)
+        sc = None if split_const is None else np.ascontiguousarray(split_const, dtype=np.float64)
+        dr = None if drhodt is None else np.ascontiguousarray(drhodt, dtype=np.float64)
+        yl = None if Yleft is None else np.ascontiguousarray(Yleft, dtype=np.float64)
+        rc = lib().orc_convection_integrate(self.h, C.byref(params), n, _p(x), _p(state), _p(sc), _p(dr), _p(yl),
+                                            t0, t1, _p(out), steps.ctypes.data_as(_ip), nthreads)
+        if rc:
+            raise RuntimeError(f"orc_convection_integrate failed: {rc}")
+        return out, steps
 
     def bdf_work(self, state, dt, opts=None, nthreads=0):
         """BDF work per point: (n_rhs, n_jac, n_lu, n_solve) columns (orc_bdf_work)."""
