@@ -16,6 +16,9 @@ CPU rate:
   diffusion  53 species x 65536 points diffusion stage (seeded smooth profiles, seed 4,
              FixedValue ends, dt = 2e-5 s, h = dt: 8 BDF1 solves), HBM GB/s
   thermo     thermodynamic properties of 65536 GRI points, HBM GB/s
+  transport  mixture-averaged transport properties of 65536 GRI points
+  convection the convection stage (ConvectionSystemSplit) on a 400-point GRI flame profile,
+             half of a 2e-5 s split step
   adjoint    gradients of the integrated heat release on 4096 flame-front states
 --headline-only skips them; --workload NAME runs one of them alone.
 """
@@ -62,7 +65,7 @@ def parse():
     p.add_argument("--headline-only", action="store_true",
                    help="skip the bdf / uneven / diffusion / thermo sub-objects")
     p.add_argument("--workload", choices=["stage", "uneven", "diffusion", "split", "copy", "thermo",
-                                          "transport"],
+                                          "transport", "convection"],
                    default="stage")
     p.add_argument("--mech", default="gri30", help="mechanism of the split workload")
     return p.parse_args()
@@ -112,7 +115,8 @@ def main():
     dev = torch.device("cuda:0")
     if a.workload != "stage":
         fn = {"uneven": run_uneven, "diffusion": run_diffusion, "copy": run_copy,
-              "split": run_split, "thermo": run_thermo, "transport": run_transport}[a.workload]
+              "split": run_split, "thermo": run_thermo, "transport": run_transport,
+              "convection": run_convection}[a.workload]
         print(json.dumps(fn(a, dev)))
         return
     res = run_stage(a, dev)
@@ -127,6 +131,8 @@ def main():
         res["diffusion"] = _subobject(run_diffusion(sub, dev))
         res["thermo"] = _subobject(run_thermo(sub, dev))
         res["transport"] = _subobject(run_transport(sub, dev))
+        sub.points = 400
+        res["convection"] = _subobject(run_convection(sub, dev))
     print(json.dumps(res))
 
 
@@ -875,6 +881,104 @@ def adjoint_cost(ctx, N, dt, dev, stream, steps):
             "stage_ms": stage, "forward_ms": fwd, "backward_ms": bwd,
             "backward_over_stage": bwd / stage, "adjoint_total_over_stage": (fwd + bwd) / stage,
             "failed_points": int((status.cpu().numpy() != 0).sum())}
+
+
+
+def convection_flops(N, K, steps_utw, steps_species):
+    """Algorithmic FLOPs of one convection stage: per accepted Dormand-Prince step 6
+    right-hand sides and the stage combinations, error estimate and norm per component.
+    Per point and right-hand side: UTW 35 (density, continuity sum, three upwind
+    differences, the three equations), species 8 (velocity interpolation, one upwind
+    difference); per component and step 60 (21 FMA of the stage sums, 7 of the error
+    estimate, the norm).  Rejected steps are not counted."""
+    utw = steps_utw * N * (6 * 35 + 3 * 60)
+    species = sum(int(s) for s in steps_species) * N * (6 * 8 + 60)
+    return float(utw + species)
+
+
+def run_convection(a, dev):
+    """Convection stage (ConvectionSystemSplit::integrateToTime) on a GRI-Mech 3.0 flame
+    profile: U = a sqrt(rho_u / rho) with a = 150 1/s, inflow rV(0) = 0.5 rho_u, seeded
+    density derivatives and split constants, over half of a 2e-5 s split step."""
+    import numpy as np
+    import torch
+
+    from ember_amd import capi, mechanism, workloads
+    N = a.points if a.points != 4096 else 400
+    case = workloads.flame_profile(N, seed=3, name="gri30")
+    m = mechanism.load("gri30")
+    st = case.state.copy()
+    Y = st[:, 2:]
+    rho = 101325.0 / (mechanism.GAS_CONSTANT * st[:, 1] * (Y / m.W).sum(axis=1))
+    rng = np.random.default_rng(3)
+    strain = 150.0
+    st[:, 0] = strain * np.sqrt(rho[0] / rho)
+    sc = np.zeros_like(st)
+    sc[:, 0] = rng.normal(0, 50.0, N)
+    sc[:, 1] = rng.normal(0, 1e4, N)
+    sc[:, 2:] = rng.normal(0, 10.0, Y.shape) * Y
+    drhodt = rng.normal(0, 1.0, N) * rho
+    t1 = 1e-5
+    kw = dict(alpha=0, beta=1, left_bc=2, right_bc=1, rVzero=rho[0] * 0.5, Tleft=300.0, rhou=rho[0],
+              strain_initial=strain, strain_final=strain)
+    ctx = capi.Context("gri30")
+    s = torch.cuda.current_stream(dev)
+    ctx.set_stream(s.cuda_stream)
+    K = m.n_species
+    t = lambda arr: torch.from_numpy(np.ascontiguousarray(arr)).to(dev)
+    d_in, d_x, d_sc, d_dr = t(st), t(case.x), t(sc), t(drhodt)
+    d_out = torch.empty_like(d_in)
+    steps = torch.zeros(1 + K, dtype=torch.int32, device=dev)
+    params = ctx.convection_params(Yleft=st[0, 2:].copy(), **kw)
+    call = lambda: ctx.convection_integrate_device(d_in, d_out, d_x, 0.0, t1, params=params, split_const=d_sc,
+                                                   drhodt=d_dr, steps=steps)
+    for _ in range(max(1, a.warmup)):
+        call()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(a.steps):
+        call()
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    stp = steps.cpu().numpy()
+    F = convection_flops(N, K, int(stp[0]), stp[1:])
+    print(f"[bench] convection gri30 x {N}: {ms * 1e3:.1f} us per stage, steps UTW {stp[0]}, "
+          f"species {stp[1:].min()}-{stp[1:].max()}", file=sys.stderr, flush=True)
+    res = {
+        "metric": "convection stages/s (GRI-Mech 3.0 flame profile, 400 points)", "value": 1e3 / ms,
+        "unit": "stages/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded tanh flame profile, strained flow field)",
+        "config": {"workload": "ConvectionSystemSplit::integrateToTime: UTW system, then 53 species systems; "
+                               "continuity fixedLeft, ControlVolume inflow, strain rate 150 1/s, t = 1e-5 s",
+                   "points": N, "seed": 3},
+        "integrator_steps": {"utw": int(stp[0]), "species_min": int(stp[1:].min()),
+                             "species_max": int(stp[1:].max())},
+        "roofline": {"bound": "fp64-vector", "achieved": F / (ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": F / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": F,
+                     "note": "latency-bound by construction: one workgroup for the UTW system, one per "
+                             "species, each step a chain of dependent right-hand sides"},
+    }
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = a.cpu_threads or cpu_info()["threads_used"]
+        om = O.Mech(mechanism.table_path("gri30"))
+        op = O.conv_params(**kw)
+        cpu_s, runs, ref = timed_cpu(lambda: om.convection_integrate(case.x, st, 0.0, t1, params=op, split_const=sc,
+                                                                     drhodt=drhodt, Yleft=st[0, 2:], nthreads=threads),
+                                     a.cpu_repeats)
+        got = d_out.cpu().numpy()
+        rel = np.abs(got - ref[0]) / (np.abs(ref[0]) + 1e-12)
+        res["cpu_baseline"] = {"value": 1.0 / cpu_s, "unit": "stages/s", "cores": threads, "kind": "port",
+                               "sample": f"the same stage, species systems over OpenMP threads; median of "
+                                         f"{len(runs)} runs after 1 warm-up: {cpu_s * 1e3:.2f} ms"}
+        res["parity"] = {"max_rel_U_T": float(rel[:, :2].max()), "max_rel_Y_floor1e-12": float(rel[:, 2:].max()),
+                         "within_rtol1e-6": bool((np.abs(got - ref[0]) <= 1e-6 * np.abs(ref[0]) + 1e-12).all())}
+    return res
 
 
 if __name__ == "__main__":
