@@ -438,7 +438,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_forward_kernel(
     const int wave = wave_in_block();
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
     WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
-    init_wave_lds(L, lane, m.nRounds);
+    init_wave_lds(L, lane, m.nRounds, m.nChunkRounds);
     const int nv = m.nVars;
     const bool active = lane < nv;
     QssSystem S;
@@ -495,7 +495,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_backward_kernel(
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds) + adj_lds_doubles(m.nRounds);
     double* wbase = lds + mech_lds_doubles(m) + wave * per_wave;
     WaveLds L = carve_lds(wbase, m.nRounds, m.nChunkRounds);
-    init_wave_lds(L, lane, m.nRounds);
+    init_wave_lds(L, lane, m.nRounds, m.nChunkRounds);
     AdjLds A = carve_adj(wbase + wave_lds_doubles(m.nRounds, m.nChunkRounds), m.nRounds);
     init_adj_lds(A, lane);
     double* rows = record + (size_t)(blockIdx.x * (blockDim.x / 64) + wave) * kRecordDoubles;

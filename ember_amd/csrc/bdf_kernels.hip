@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(256) bdf_stage_kernel(
     const int per_wave = chem + nv * 64 + 64;
     double* base = lds + wave * per_wave;
     WaveLds L = carve_lds(base, m.nRounds, m.nChunkRounds);
-    init_wave_lds(L, lane, m.nRounds);
+    init_wave_lds(L, lane, m.nRounds, m.nChunkRounds);
     double* M = base + chem;
     double* ru = M + nv * 64;
     double* J = jac_scratch + (size_t)(blockIdx.x * wpb + wave) * 64 * 64;

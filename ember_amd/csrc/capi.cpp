@@ -196,6 +196,8 @@ int upload_mech(emx_context* c) {
     d.nChunkRounds = L.nChunkRounds;
     d.nEntries = (int)L.entry.size();
     d.nGrpEntries = (int)L.grp_sp.size();
+    d.grpMaxLen = 0;
+    for (int g = 0; g < L.nGroups; g++) d.grpMaxLen = std::max(d.grpMaxLen, L.grp_off[g + 1] - L.grp_off[g]);
     d.interp = c->opts.kinetics_model == EMX_KIN_INTERP ? 1 : 0;
     d.P = c->P;
     d.Pref = m.Pref;

@@ -34,6 +34,10 @@ __device__ __forceinline__ bool valid_input_state(int lane, bool active, double 
     return !wave_or(bad ? 1 : 0);
 }
 
+// Third-body cell of the slots without a third body (info_group 255): Mg[kNoGroup] = 1.
+// Mechanisms have at most kNoGroup efficiency groups, so no group writes it.
+constexpr int kNoGroup = 63;
+
 // per-wave LDS scratch (doubles)
 struct WaveLds {
     double* C;      // [64] concentrations
@@ -42,12 +46,12 @@ struct WaveLds {
     double* Mg;     // [64] third-body concentrations per efficiency group
     double* ropf;   // [nRounds*64 + 1]: the last entry is the zero slot
     double* ropr;   // [nRounds*64 + 1]
-    double* partq;  // [nChunkRounds*64]
-    double* partd;  // [nChunkRounds*64]
+    double* partq;  // [nChunkRounds*64 + 1]: the last entry is a zero cell
+    double* partd;  // [nChunkRounds*64 + 1]
 };
 
 __host__ __device__ inline int wave_lds_doubles(int nRounds, int nChunkRounds) {
-    return 4 * 64 + 2 * (nRounds * 64 + 1) + 2 * nChunkRounds * 64;
+    return 4 * 64 + 2 * (nRounds * 64 + 1) + 2 * (nChunkRounds * 64 + 1);
 }
 
 __device__ inline WaveLds carve_lds(double* base, int nRounds, int nChunkRounds) {
@@ -59,20 +63,23 @@ __device__ inline WaveLds carve_lds(double* base, int nRounds, int nChunkRounds)
     w.ropf = base + 256;
     w.ropr = w.ropf + nRounds * 64 + 1;
     w.partq = w.ropr + nRounds * 64 + 1;
-    w.partd = w.partq + nChunkRounds * 64;
+    w.partd = w.partq + nChunkRounds * 64 + 1;
     return w;
 }
 
 // Constant cells of the per-wave scratch: C[kPadSpecies] = ge = gi = 1 (stoichiometry
-// padding), ropf/ropr zero slot (species-sum padding).  Species lanes never write index
-// kPadSpecies since K <= 62.
-__device__ inline void init_wave_lds(const WaveLds& L, int lane, int nRounds) {
+// padding), ropf/ropr zero slot (species-sum padding), the chunk partials' zero cells.
+// Species lanes never write index kPadSpecies since K <= 62.
+__device__ inline void init_wave_lds(const WaveLds& L, int lane, int nRounds, int nChunkRounds) {
     if (lane == 0) {
         L.C[kPadSpecies] = 1.0;
         L.ge[kPadSpecies] = 1.0;
         L.gi[kPadSpecies] = 1.0;
         L.ropf[nRounds * 64] = 0.0;
         L.ropr[nRounds * 64] = 0.0;
+        L.partq[nChunkRounds * 64] = 0.0;
+        L.partd[nChunkRounds * 64] = 0.0;
+        L.Mg[kNoGroup] = 1.0;
     }
     wave_lds_sync();
 }
@@ -228,13 +235,13 @@ __device__ inline GasState set_state(const DevMech& m, int lane, double y, doubl
     if (!nonorm) {
         double yc = fmax(yk, 0.0);
         double norm = wave_sum(yc);
-        yk = yc * (1.0 / norm);
+        yk = yc * emx_div(1.0, norm);
     }
     g.ym = sp ? yk * rW : 0.0;
     double sum = wave_sum(g.ym);
-    g.mmw = 1.0 / sum;
-    g.dens = m.P * g.mmw / (T * kGasConstant);
-    g.ctot = g.dens / g.mmw;
+    g.mmw = emx_div(1.0, sum);
+    g.dens = emx_div(m.P * g.mmw, T * kGasConstant);
+    g.ctot = emx_div(g.dens, g.mmw);
     if (sp) L.C[lane - 2] = g.ym * g.dens;
     return g;
 }
@@ -278,9 +285,15 @@ __device__ __forceinline__ double inv_kc(uint32_t reac, uint32_t prod, const Wav
 // Thermo for this lane's species plus the T-dependent rate data, cached on T.
 // Returns false when T leaves the interpolation table range (INTERP only).
 // Outputs hRT/cpR for the lane's species.
+//
+// Every round's parameters are loaded before any is used (one memory latency instead of
+// one per round: the tables stay in global memory), the rate constants are formed without
+// branches on the slot kind (padding slots have A = 0 and padding stoichiometry, so they
+// give exact zeros), and a falloff slot's k_inf is the Arrhenius value of its round.
 template <int NR, bool INTERP>
 __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, RateRegs<NR>& R,
                                 const WaveLds& L, double& hRT, double& cpR) {
+    constexpr int NF = RateRegs<NR>::NF;
     const double T = g.T;
     const double logT = emx_log(T);
     const double recipT = 1.0 / T;
@@ -299,6 +312,9 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
             if (kind == kPlog || kind == kCheb) L.ropf[s] = pdep_kf(m, s, kind, logT, recipT);
         }
     }
+    // rounds past the mechanism's are not loaded (uniform branches: the loads keep one
+    // address per table and the round in the instruction's offset)
+    double khigh_of[NF];
     if (INTERP) {
         // InterpKinetics::updateROP's index and offset (src/chemistry0d.cpp:392-397); a T
         // outside [Tmin, Tmax] would make Ember rebuild its tables (:279-291), here the
@@ -308,13 +324,24 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
         size_t n = (size_t)floor((T - Tlo) / (Thi - Tlo) * (m.interpN - 1));
         double dT = T - n * (Thi - Tlo) / (m.interpN - 1) - Tlo;
         const int stride = m.nRounds * 64;
+        double kc0[NR], kc1[NR], kf0[NR], kf1[NR];
+        const size_t o0 = n * stride + lane;
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            kc0[j] = kc1[j] = kf0[j] = kf1[j] = 0.0;
+            if (j < m.nRounds) {
+                kc0[j] = m.kc_c[o0 + j * 64];
+                kc1[j] = m.kc_s[o0 + j * 64];
+                kf0[j] = m.kf_c[o0 + j * 64];
+                kf1[j] = m.kf_s[o0 + j * 64];
+            }
+        }
 #pragma unroll
         for (int j = 0; j < NR; j++) {
             if (j < m.nRounds) {
-                int s = j * 64 + lane;
-                size_t o = n * stride + s;
-                R.rkc[j] = m.kc_c[o] + m.kc_s[o] * dT;
-                R.kf[j] = m.kf_c[o] + m.kf_s[o] * dT;
+                const int s = j * 64 + lane;
+                R.rkc[j] = kc0[j] + kc1[j] * dT;
+                R.kf[j] = kf0[j] + kf1[j] * dT;
                 // PLOG and Chebyshev rates keep their own evaluation (only Arrhenius rates
                 // are interpolated, src/chemistry0d.cpp:264-269)
                 if (m.has_pdep) {
@@ -323,7 +350,22 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
                 }
             }
         }
+#pragma unroll
+        for (int j = 0; j < NF; j++) {
+            const int s = j * 64 + lane;
+            khigh_of[j] = (j < m.nFallRounds) ? arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT, m.exp2t) : 0.0;
+        }
     } else {
+        double pA[NR], pb[NR], pE[NR];
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            pA[j] = pb[j] = pE[j] = 0.0;
+            if (j < m.nRounds) {
+                pA[j] = m.A[j * 64 + lane];
+                pb[j] = m.b[j * 64 + lane];
+                pE[j] = m.EaR[j * 64 + lane];
+            }
+        }
         // IdealGasPhase::getStandardChemPotentials: mu_k = RT (h_k/RT - s_k/R) + RT ln(P/Pref);
         // with the standard concentration c0 = P / RT the pressure cancels from
         // mu_k / RT - ln c0 = h_k/RT - s_k/R + ln T + ln(R / Pref)
@@ -331,40 +373,47 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
             const double x = (hRT - sR) + (logT + m.lnRPref);
             const double e = emx_exp(x, m.exp2t);
             L.ge[lane - 2] = e;
-            L.gi[lane - 2] = 1.0 / e;
+            L.gi[lane - 2] = emx_div(1.0, e);
         }
         wave_lds_sync();
 #pragma unroll
         for (int j = 0; j < NR; j++) {
             if (j < m.nRounds) {
-                int s = j * 64 + lane;
-                int w = m.info[s];
-                int kind = info_kind(w);
-                double rkc = 0.0, kf = 0.0;
-                if (kind != kNone) {
-                    if (info_rev(w)) rkc = fmin(inv_kc(m.reac[s], m.prod[s], L), kBigNumber);
-                    if (kind <= kThreeBody) kf = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT, m.exp2t);
-                    else if (kind == kPlog || kind == kCheb) kf = L.ropf[s];
-                }
-                R.rkc[j] = rkc;
+                const int s = j * 64 + lane;
+                const int w = m.info[s];
+                const int kind = info_kind(w);
+                const uint32_t rs = m.reac[s], ps = m.prod[s];
+                const double g0 = L.ge[st_k(ps, 0)], g1 = L.gi[st_k(rs, 0)];
+                const double g2 = L.ge[st_k(ps, 1)], g3 = L.gi[st_k(rs, 1)];
+                const double g4 = L.ge[st_k(ps, 2)], g5 = L.gi[st_k(rs, 2)];
+                const double ka = arrhenius(pA[j], pb[j], pE[j], logT, recipT, m.exp2t);
+                const double ikc = (g0 * g1) * (g2 * g3) * (g4 * g5);  // inv_kc's pairing
+                R.rkc[j] = (kind != kNone && info_rev(w)) ? fmin(ikc, kBigNumber) : 0.0;
+                double kf = 0.0;
+                if (kind <= kThreeBody) kf = ka;
+                else if (kind == kPlog || kind == kCheb) kf = L.ropf[s];
                 R.kf[j] = kf;
+                if (j < NF) khigh_of[j] = ka;
             }
         }
     }
     // falloff: k_low, k_high and the T-dependent parts of F (FalloffRate, TroeRate and
     // SriRate::updateTemp)
 #pragma unroll
-    for (int j = 0; j < RateRegs<NR>::NF; j++) {
+    for (int j = 0; j < NF; j++) {
         if (j < m.nFallRounds) {
-            int s = j * 64 + lane;
-            int kind = info_kind(m.info[s]);
+            const int s = j * 64 + lane;
+            const int kind = info_kind(m.info[s]);
             double klow = 0.0, khigh = 0.0, work = 0.0, work2 = 0.0;
+            // the round's falloff parameters, loaded together
+            const double A0 = m.A0[s], b0 = m.b0[s], E0 = m.EaR0[s];
+            const double Ta = m.troeA[s], T3 = m.rT3[s], T1 = m.rT1[s], T2 = m.T2[s];
             if (kind == kLindemann || kind == kTroe || kind == kSri) {
-                klow = arrhenius(m.A0[s], m.b0[s], m.EaR0[s], logT, recipT, m.exp2t);
-                khigh = arrhenius(m.A[s], m.b[s], m.EaR[s], logT, recipT, m.exp2t);
+                klow = arrhenius(A0, b0, E0, logT, recipT, m.exp2t);
+                khigh = khigh_of[j];
                 if (kind == kTroe) {
-                    double Fcent = (1.0 - m.troeA[s]) * emx_exp(-T * m.rT3[s], m.exp2t) + m.troeA[s] * emx_exp(-T * m.rT1[s], m.exp2t);
-                    if (m.T2[s] != 0.0) Fcent += emx_exp(-m.T2[s] / T, m.exp2t);
+                    double Fcent = (1.0 - Ta) * emx_exp(-T * T3, m.exp2t) + Ta * emx_exp(-T * T1, m.exp2t);
+                    if (T2 != 0.0) Fcent += emx_exp(-T2 / T, m.exp2t);
                     work = emx_log(fmax(Fcent, kSmallNumber)) * kInvLn10;
                 } else if (kind == kSri) {
                     const double* q = m.auxd + m.aux[s];  // a, b, c, d, e
@@ -383,66 +432,115 @@ __device__ inline bool update_T(const DevMech& m, int lane, const GasState& g, R
     return true;
 }
 
-// ThirdBodyCalc::update per efficiency group: Mg = default * ctot + sum (eff - default) C
+// ThirdBodyCalc::update per efficiency group: Mg = default * ctot + sum (eff - default) C.
+// Lane g < nGroups sums group g's entries in order, kGrpBatch at a time with the species
+// indices, efficiencies and concentrations of a batch loaded together; entries past the
+// group's last add exact zeros (efficiency 0 on the unit padding concentration).
 __device__ inline void third_body_conc(const DevMech& m, int lane, const GasState& g,
                                        const WaveLds& L) {
+    constexpr int kGrpBatch = 8;
     wave_lds_sync();  // concentrations written by set_state
-    if (lane < m.nGroups) {
-        double sum = 0.0;
-        for (int e = m.grp_off[lane]; e < m.grp_off[lane + 1]; e++)
-            sum += m.grp_eff[e] * L.C[m.grp_sp[e]];
-        L.Mg[lane] = m.grp_def[lane] * g.ctot + sum;
+    const bool act = lane < m.nGroups;
+    const int e0 = act ? m.grp_off[lane] : 0;
+    const int ne = act ? m.grp_off[lane + 1] - e0 : 0;
+    const int elast = m.nGrpEntries > 0 ? m.nGrpEntries - 1 : 0;
+    double sum = 0.0;
+    for (int i = 0; i < m.grpMaxLen; i += kGrpBatch) {
+        int sp[kGrpBatch];
+        double ef[kGrpBatch];
+#pragma unroll
+        for (int u = 0; u < kGrpBatch; u++) {
+            const int e = min(e0 + i + u, elast);
+            const bool in = i + u < ne;
+            const int k = m.grp_sp[e];
+            const double f = m.grp_eff[e];
+            sp[u] = in ? k : kPadSpecies;
+            ef[u] = in ? f : 0.0;
+        }
+        double c[kGrpBatch];
+#pragma unroll
+        for (int u = 0; u < kGrpBatch; u++) c[u] = L.C[sp[u]];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kGrpBatch; u++) sum += ef[u] * c[u];
     }
+    if (act) L.Mg[lane] = m.grp_def[lane] * g.ctot + sum;
     wave_lds_sync();
 }
 
 // Rates of progress into LDS (composition-dependent part of updateROP).
 // Rate multipliers perturb[s] * fa (Cantera m_perturb; fa: the rate multiplier function).
+// Each round's table words and concentration gathers are issued before its arithmetic,
+// and the slot kinds are applied with selects rather than branches (a padding slot has
+// k_f = 0 and padding stoichiometry, and is written as exact zeros; a slot without third
+// body reads the unit cell Mg[kNoGroup]), so the loads of a round are in flight together.
+// Only the falloff rounds keep a branch around their pressure-dependent factor.
 template <int NR>
 __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasState& g,
                                          const RateRegs<NR>& R, const double* perturb, double fa,
                                          const WaveLds& L) {
     third_body_conc(m, lane, g, L);
+    // the table words of every round first (one LDS latency for all of them)
+    int wd[NR];
+    uint32_t rsw[NR], psw[NR];
+    double ptw[NR];
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+        wd[j] = 0;
+        rsw[j] = psw[j] = 0;
+        ptw[j] = 0.0;
+        if (j < m.nRounds) {
+            wd[j] = m.info[j * 64 + lane];
+            rsw[j] = m.reac[j * 64 + lane];
+            psw[j] = m.prod[j * 64 + lane];
+            ptw[j] = perturb[j * 64 + lane];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < NR; j++) {
         if (j < m.nRounds) {
-            int s = j * 64 + lane;
-            int w = m.info[s];
-            int kind = info_kind(w);
-            double ropf = 0.0, ropr = 0.0;
-            if (kind != kNone) {
-                double kf = R.kf[j];
-                if (j < RateRegs<NR>::NF && (kind == kLindemann || kind == kTroe || kind == kSri)) {
-                    double khigh = R.khigh[j];
-                    double pr = L.Mg[info_group(w)] * R.klow[j] / (khigh + kSmallNumber);
-                    double F = 1.0;
-                    if (kind == kTroe) {
-                        double work = R.work[j];
-                        double lpr = emx_log(fmax(pr, kSmallNumber)) * kInvLn10;
-                        double cc = -0.4 - 0.67 * work;
-                        double nn = 0.75 - 1.27 * work;
-                        double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
-                        double lf = work / (1.0 + f1 * f1);
-                        F = emx_exp(kLn10 * lf, m.exp2t);
-                    } else if (kind == kSri) {
-                        // SriRate: F = d T^e [a exp(-b/T) + exp(-T/c)]^X, X = 1/(1 + log10^2 Pr)
-                        const double lpr = emx_log(fmax(pr, kSmallNumber)) * kInvLn10;
-                        const double X = 1.0 / (1.0 + lpr * lpr);
-                        F = R.work2[j] * emx_exp(kLn10 * (X * R.work[j]), m.exp2t);
-                    }
-                    pr *= F / (1.0 + pr);
-                    kf = pr * khigh;
+            const int s = j * 64 + lane;
+            const int w = wd[j];
+            const int kind = info_kind(w);
+            const uint32_t rs = rsw[j], ps = psw[j];
+            const double pt = ptw[j];
+            // the round's gathers, all issued before its arithmetic
+            const double mg = L.Mg[min(info_group(w), kNoGroup)];
+            const double c0 = L.C[st_k(rs, 0)], c1 = L.C[st_k(rs, 1)], c2 = L.C[st_k(rs, 2)];
+            const double d0 = L.C[st_k(ps, 0)], d1 = L.C[st_k(ps, 1)], d2 = L.C[st_k(ps, 2)];
+            __builtin_amdgcn_sched_barrier(0);
+            const double cr = c0 * c1 * c2;
+            const double cp = d0 * d1 * d2;
+            double kf = R.kf[j];
+            if (j < RateRegs<NR>::NF && (kind == kLindemann || kind == kTroe || kind == kSri)) {
+                double khigh = R.khigh[j];
+                double pr = emx_div(mg * R.klow[j], khigh + kSmallNumber);
+                double F = 1.0;
+                if (kind == kTroe) {
+                    double work = R.work[j];
+                    double lpr = emx_log(fmax(pr, kSmallNumber)) * kInvLn10;
+                    double cc = -0.4 - 0.67 * work;
+                    double nn = 0.75 - 1.27 * work;
+                    double f1 = emx_div(lpr + cc, nn - 0.14 * (lpr + cc));
+                    double lf = emx_div(work, 1.0 + f1 * f1);
+                    F = emx_exp(kLn10 * lf, m.exp2t);
+                } else if (kind == kSri) {
+                    // SriRate: F = d T^e [a exp(-b/T) + exp(-T/c)]^X, X = 1/(1 + log10^2 Pr)
+                    const double lpr = emx_log(fmax(pr, kSmallNumber)) * kInvLn10;
+                    const double X = emx_div(1.0, 1.0 + lpr * lpr);
+                    F = R.work2[j] * emx_exp(kLn10 * (X * R.work[j]), m.exp2t);
                 }
-                double rfn = kf * perturb[s] * fa;
-                ropf = rfn;
-                if (kind == kThreeBody) ropf *= L.Mg[info_group(w)];
-                ropr = ropf;
-                ropf *= conc_product(m.reac[s], L.C);
-                ropr *= R.rkc[j];
-                if (info_rev(w)) ropr *= conc_product(m.prod[s], L.C);
+                pr *= emx_div(F, 1.0 + pr);
+                kf = pr * khigh;
             }
-            L.ropf[s] = ropf;
-            L.ropr[s] = ropr;
+            const double rfn = kf * pt * fa;
+            double ropf = (kind == kThreeBody) ? rfn * mg : rfn;
+            double ropr = ropf * R.rkc[j];
+            ropf *= cr;
+            if (info_rev(w)) ropr *= cp;
+            const bool none = kind == kNone;
+            L.ropf[s] = none ? 0.0 : ropf;
+            L.ropr[s] = none ? 0.0 : ropr;
         }
     }
     wave_lds_sync();
@@ -451,6 +549,9 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
 // Species creation / destruction rates: fixed-length chunk sums (chunk c = cr*64 + lane,
 // kChunkLen entries, padding entries add exact zeros), then per-species totals over the
 // species' consecutive chunks.  Returns (wQ, wD) for this lane's species (0 elsewhere).
+// The gathers of a chunk are all issued before its sums (the scheduling barrier keeps the
+// compiler from interleaving each load with its use, which serialised the LDS latencies);
+// the sums keep their sequential order.
 __device__ inline void species_rates(const DevMech& m, int lane, const WaveLds& L, double& wQ,
                                      double& wD) {
 #pragma unroll
@@ -461,12 +562,19 @@ __device__ inline void species_rates(const DevMech& m, int lane, const WaveLds& 
             uint32_t en[kChunkLen];
 #pragma unroll
             for (int e = 0; e < kChunkLen; e++) en[e] = ep[e];
+            double vq[kChunkLen], vd[kChunkLen];
+#pragma unroll
+            for (int e = 0; e < kChunkLen; e++) {
+                vq[e] = L.ropf[en_q(en[e])];  // ropf, zero slot and ropr are one array
+                vd[e] = L.ropf[en_d(en[e])];
+            }
+            __builtin_amdgcn_sched_barrier(0);
             double q = 0.0, d = 0.0;
 #pragma unroll
             for (int e = 0; e < kChunkLen; e++) {
                 const double nu = (double)en_nu(en[e]);
-                q += nu * L.ropf[en_q(en[e])];  // ropf, zero slot and ropr are one array
-                d += nu * L.ropf[en_d(en[e])];
+                q += nu * vq[e];
+                d += nu * vd[e];
             }
             L.partq[c] = q;
             L.partd[c] = d;
@@ -475,14 +583,33 @@ __device__ inline void species_rates(const DevMech& m, int lane, const WaveLds& 
     wave_lds_sync();
     wQ = 0.0;
     wD = 0.0;
-    if (is_species_lane(lane, m)) {
-        const int k = lane - 2;
-        const int c0 = m.sp_chunk0[k], nc = m.sp_nchunk[k];
-#pragma unroll 4
-        for (int i = 0; i < nc; i++) {
-            wQ += L.partq[c0 + i];
-            wD += L.partd[c0 + i];
+    // per-species totals, kSumBatch chunks per batch with their loads issued together; a
+    // batch's chunks past the species' last read the zero cell (x + 0 = x exactly, so the
+    // totals equal the one-by-one sums)
+    constexpr int kSumBatch = 4;
+    const bool sp = is_species_lane(lane, m);
+    const int k = sp ? lane - 2 : 0;
+    const int c0 = sp ? m.sp_chunk0[k] : 0, nc = sp ? m.sp_nchunk[k] : 0;
+    const int zero = m.nChunkRounds * 64;
+    const int nmax = (int)wave_max0((double)nc);
+    for (int i = 0; i < nmax; i += kSumBatch) {
+        double aq[kSumBatch], ad[kSumBatch];
+#pragma unroll
+        for (int u = 0; u < kSumBatch; u++) {
+            const int c = (i + u < nc) ? c0 + i + u : zero;
+            aq[u] = L.partq[c];
+            ad[u] = L.partd[c];
         }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kSumBatch; u++) {
+            wQ += aq[u];
+            wD += ad[u];
+        }
+    }
+    if (!sp) {
+        wQ = 0.0;
+        wD = 0.0;
     }
     wave_lds_sync();
 }

@@ -62,6 +62,7 @@ struct DevMech {
     int nChunkRounds;
     int nEntries;          // species-sum entries, nChunkRounds * 64 * kChunkLen
     int nGrpEntries;       // third-body efficiency entries
+    int grpMaxLen;         // entries of the largest efficiency group
     int interp;            // 1: InterpKinetics tables
     double P, Pref;
     double lnRPref;  // ln(R / Pref): standard-state term of the 1/Kc factors

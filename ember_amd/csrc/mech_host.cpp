@@ -188,7 +188,7 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
                  : r.type == 4 ? kPlog : r.type == 5 ? kCheb : r.type == 6 ? kSri : kElementary;
         L.info[s] = kind | (r.rev ? 8 : 0) | (r.hasT2 ? 16 : 0) | (group << 8);
     }
-    if ((int)glist.size() > 64) return "more than 64 distinct third-body efficiency sets";
+    if ((int)glist.size() > 63) return "more than 63 distinct third-body efficiency sets";
     L.nGroups = (int)glist.size();
     L.grp_off.assign(1, 0);
     for (auto& g : glist) {

@@ -45,6 +45,7 @@ struct QssSystem {
     double qign;       // igniter amplitude at this point's position (0: none)
     double T;          // temperature used by the rates (frozen in correctors)
     double rho, cp;    // from the last non-corrector call (SourceSystem::updateThermo)
+    double rrho, rrc;  // 1 / rho and 1 / (rho cp) of the same call
     double hk;         // J/kmol, species lanes
     double qdot;
     double wQ, wD;     // creation / destruction rate of this lane's species (last evaluation)
@@ -67,7 +68,9 @@ __device__ inline bool qss_odefun(const DevMech& m, int lane, double y, bool cor
         S.rho = g.dens;
         double s = wave_sum(g.ym * cpR);
         double cp_mole = kGasConstant * (g.mmw * s);
-        S.cp = cp_mole / g.mmw;
+        S.cp = emx_div(cp_mole, g.mmw);
+        S.rrho = emx_div(1.0, S.rho);
+        S.rrc = emx_div(1.0, S.rho * S.cp);
     }
     rates_of_progress<NR>(m, lane, g, R, perturb, fa, L);
     double wQ, wD;
@@ -76,20 +79,15 @@ __device__ inline bool qss_odefun(const DevMech& m, int lane, double y, bool cor
     S.wD = wD;
     S.qdot = -wave_sum((wQ - wD) * S.hk);
     if (EXT) S.qdot += qi;  // + getQdotIgniter(t) (src/sourceSystem.cpp:465)
-    const double rho = S.rho, cp = S.cp;
-    if (lane == 0) {
-        q = S.sc;
-        d = 0.0;
-    } else if (lane == 1) {
-        q = S.qdot / (rho * cp) + S.sc;
-        d = S.qloss / (rho * cp);
-    } else if (is_species_lane(lane, m)) {
-        q = wQ * S.W / rho + S.sc;
-        d = wD * S.W / rho;
-    } else {
-        q = 0.0;
-        d = 0.0;
-    }
+    // q and d of every lane without divergent branches: the energy lane scales by
+    // 1 / (rho cp), the species lanes by W_k / rho
+    const bool sp = is_species_lane(lane, m);
+    const bool en = lane == 1;
+    const double f = en ? S.rrc : S.rrho;
+    const double nq = en ? S.qdot : wQ * S.W;
+    const double nd = en ? S.qloss : wD * S.W;
+    q = (lane == 0) ? S.sc : (en || sp) ? nq * f + S.sc : 0.0;
+    d = (en || sp) ? nd * f : 0.0;
     return ok;
 }
 
@@ -168,7 +166,7 @@ __device__ __forceinline__ int qss_integrate(const DevMech& m, const DevOpts& o,
                 dt = fmin(dt, o.qss_dtmax);
             }
             ts = tn;
-            Z.rtau = Z.enforce ? dt * Z.d / Z.y : 0.0;
+            Z.rtau = Z.enforce ? emx_div(dt * Z.d, Z.y) : 0.0;
             Z.qs = Z.q;
             Z.ys = Z.y;
             Z.rtaus = Z.rtau;
@@ -176,11 +174,11 @@ __device__ __forceinline__ int qss_integrate(const DevMech& m, const DevOpts& o,
             start_attempt = true;
         } else {
             // ---- after a corrector evaluation
-            double rtaub = Z.enforce ? 0.5 * (Z.rtaus + dt * Z.d / Z.y) : 0.0;
-            double alpha = (180. + rtaub * (60. + rtaub * (11. + rtaub))) /
-                           (360. + rtaub * (60. + rtaub * (12. + rtaub)));
-            Z.scratch = (Z.qs * (1.0 - alpha) + Z.q * alpha - Z.ys * rtaub / dt) /
-                        (1.0 + alpha * rtaub);
+            double rtaub = Z.enforce ? 0.5 * (Z.rtaus + emx_div(dt * Z.d, Z.y)) : 0.0;
+            double alpha = emx_div(180. + rtaub * (60. + rtaub * (11. + rtaub)),
+                                   360. + rtaub * (60. + rtaub * (12. + rtaub)));
+            Z.scratch = emx_div(Z.qs * (1.0 - alpha) + Z.q * alpha - emx_div(Z.ys * rtaub, dt),
+                                1.0 + alpha * rtaub);
             iter++;
             if (iter < o.qss_itermax) {
                 if (stab_on) {
@@ -199,8 +197,8 @@ __device__ __forceinline__ int qss_integrate(const DevMech& m, const DevOpts& o,
                 double scr1 = fabs(scr2 - Z.y1);
                 Z.y = fmax(scr2, Z.ymin);
                 if (active && fabs(Z.y) > o.qss_abstol && 0.25 * (Z.ys + Z.y) > Z.ymin) {
-                    scr1 = scr1 / Z.y;
-                    c = .5 * (scr1 + fmin(fabs(Z.q - Z.d) / (Z.q + Z.d + 1e-30), scr1));
+                    scr1 = emx_div(scr1, Z.y);
+                    c = .5 * (scr1 + fmin(emx_div(fabs(Z.q - Z.d), Z.q + Z.d + 1e-30), scr1));
                 }
             }
             eps = fmax(eps, wave_max0(c));
@@ -226,10 +224,10 @@ __device__ __forceinline__ int qss_integrate(const DevMech& m, const DevOpts& o,
                 tn = ts;
             }
             double rteps = 0.5 * (eps + 1.0);
-            rteps = 0.5 * (rteps + eps / rteps);
-            rteps = 0.5 * (rteps + eps / rteps);
+            rteps = 0.5 * (rteps + emx_div(eps, rteps));
+            rteps = 0.5 * (rteps + emx_div(eps, rteps));
             double dto = dt;
-            dt = fmin(dt * (1.0 / rteps + 0.005), tfd * (tf - tn));
+            dt = fmin(dt * (emx_div(1.0, rteps) + 0.005), tfd * (tf - tn));
             dt = fmin(dt, o.qss_dtmax);
             if (stab_on) dt = fmin(dt, dto / (stab + .001));
             if (!ok) {
@@ -246,8 +244,8 @@ __device__ __forceinline__ int qss_integrate(const DevMech& m, const DevOpts& o,
         if (start_attempt) {
             // predictor terms, then the first corrector iterate
             double r = Z.rtau;
-            Z.scratch = (Z.q - Z.d) /
-                        (1.0 + r * (180 + r * (60 + r * (11 + r))) / (360 + r * (60 + r * (12 + r))));
+            Z.scratch = emx_div(Z.q - Z.d,
+                                1.0 + emx_div(r * (180 + r * (60 + r * (11 + r))), 360 + r * (60 + r * (12 + r))));
             iter = 0;
             if (stab_on) {
                 Z.ym2 = Z.ym1;

@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
     const int wave = wave_in_block();
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
     WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
-    init_wave_lds(L, lane, m.nRounds);
+    init_wave_lds(L, lane, m.nRounds, m.nChunkRounds);
     const int nv = m.nVars;
     const bool active = lane < nv;
     QssSystem S;
@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(256) rates_kernel(DevMech mg, const double* __
     const int wave = wave_in_block();
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
     WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
-    init_wave_lds(L, lane, m.nRounds);
+    init_wave_lds(L, lane, m.nRounds, m.nChunkRounds);
     const int p = blockIdx.x * (blockDim.x / 64) + wave;
     if (p >= n) return;
     const int K = m.K;
@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(256) heat_release_kernel(DevMech mg, const dou
     const int wave = wave_in_block();
     const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds);
     WaveLds L = carve_lds(lds + mech_lds_doubles(m) + wave * per_wave, m.nRounds, m.nChunkRounds);
-    init_wave_lds(L, lane, m.nRounds);
+    init_wave_lds(L, lane, m.nRounds, m.nChunkRounds);
     const int p = blockIdx.x * (blockDim.x / 64) + wave;
     if (p >= n) return;
     const int nv = m.nVars;
