@@ -428,6 +428,7 @@ __global__ void __launch_bounds__(256) bdf_stage_kernel(
         S.qign = o.qign ? o.qign[p] : 0.0;
         S.nfe = 0;
         R.T = -1.0;
+        const unsigned long long t_point = point_clock();
         double y = y0;
         int nsteps = 0;
         int err = kInvalidState;
@@ -439,7 +440,7 @@ __global__ void __launch_bounds__(256) bdf_stage_kernel(
             if (status) status[p] = err;
             if (stat1) stat1[p] = nsteps;
             if (stat2) stat2[p] = S.nfe;
-            if (cost) cost[p] = S.nfe;
+            if (cost) cost[p] = point_cost(t_point);
         }
     }
 }

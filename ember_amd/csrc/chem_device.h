@@ -27,6 +27,17 @@ namespace emx {
 // (Phase::setTemperature needs T > 0; a NaN or infinite value anywhere in the state).
 constexpr int kInvalidState = -9;
 
+// A point's cost for the next stage's queue order, from the real-time counter (100 MHz)
+// read at its start: the integration time in 640 ns units (1/16-octave cost bins resolve
+// times up to 42 ms).  Time rather than a count of right-hand sides: a predictor
+// evaluation (rate constants, thermo) costs about twice a corrector's, a Jacobian column
+// less than a Newton iteration with its solves, and the mix differs between points.
+__device__ __forceinline__ unsigned long long point_clock() { return __builtin_amdgcn_s_memrealtime(); }
+__device__ __forceinline__ int point_cost(unsigned long long t0) {
+    const unsigned long long d = (point_clock() - t0) >> 6;
+    return (int)(d >= (1ull << 30) ? (1ull << 30) : (d | 1ull));
+}
+
 // The whole wave agrees whether this point's input state can be integrated: every state
 // variable finite and T (lane 1) positive.
 __device__ __forceinline__ bool valid_input_state(int lane, bool active, double y0) {

@@ -145,6 +145,12 @@ __device__ __forceinline__ int qss_integrate(const DevMech& m, const DevOpts& o,
         if (!qss_odefun<NR, INTERP, EXT>(m, lane, Z.y, !predict, S, R, perturb, fa, qi, L, Z.q, Z.d))
             return -2;
         gcount += 1;
+        // Alternate this wave's issue priority with its evaluation count: the two waves
+        // sharing a SIMD then take turns at priority instead of the older one always
+        // issuing first, so points progress at the same rate whichever wave holds them
+        // (the cost-ordered queue assumes they do)
+        if (gcount & 1) __builtin_amdgcn_s_setprio(1);
+        else __builtin_amdgcn_s_setprio(0);
         if (predict) H.predicted(lane, Z, S);
         else if (!H.corrected(lane, Z, S, dt)) return -5;
         bool start_attempt = false;

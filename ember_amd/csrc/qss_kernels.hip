@@ -70,6 +70,7 @@ __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
         Z.ym1 = Z.ym2 = 0.0;
         int g = 0, r = 0;
         double qi = 0.0;
+        const unsigned long long t_point = point_clock();
         NoHooks H;
         int err = kInvalidState;
         if (valid_input_state(lane, active, y0))
@@ -81,7 +82,7 @@ __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
             if (status) status[p] = err;
             if (stat1) stat1[p] = g;
             if (stat2) stat2[p] = r;
-            if (cost) cost[p] = g;
+            if (cost) cost[p] = point_cost(t_point);
             if (qint_out) qint_out[p] = qi;
         }
     }
