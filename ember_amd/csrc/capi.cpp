@@ -1239,9 +1239,13 @@ int emx_convection_integrate_device(emx_context* c, size_t nVars, size_t nPoints
     double* aux = c->cv_aux.as<double>();
     double *Wmx = aux, *rho = aux + N, *cpm = aux + 2 * N, *scU = aux + 3 * N, *scT = aux + 4 * N,
            *scW = aux + 5 * N, *dr0 = aux + 6 * N;
-    int e = launch_thermo_props(c->dm, state_in, N, rho, Wmx, cpm, nullptr, nullptr, st);
-    if (!e) e = launch_conv_split_constants(c->dm, N, split_const, scU, scT, scW, st);
-    if (!e && !drhodt) e = launch_fill((size_t)N, 0.0, dr0, nullptr, st);
+    // the register path (N <= kConvRegMax) forms Wmx and the UTW split constants itself
+    int e = 0;
+    if (N > kConvRegMax) {
+        e = launch_thermo_props(c->dm, state_in, N, rho, Wmx, cpm, nullptr, nullptr, st);
+        if (!e) e = launch_conv_split_constants(c->dm, N, split_const, scU, scT, scW, st);
+        if (!e && !drhodt) e = launch_fill((size_t)N, 0.0, dr0, nullptr, st);
+    }
     if (e) return fail(-10, std::string("convection set-up launch: ") + hipGetErrorString((hipError_t)e));
     ConvParams p{};
     p.N = N;
@@ -1274,7 +1278,8 @@ int emx_convection_integrate_device(emx_context* c, size_t nVars, size_t nPoints
     a.state = state_in;
     a.sc = split_const;
     a.x = x;
-    a.drhodt = drhodt ? drhodt : dr0;
+    a.drhodt = drhodt ? drhodt : (N > kConvRegMax ? dr0 : nullptr);
+    a.rW = c->dm.rW;
     a.Wmx = Wmx;
     a.scU = scU;
     a.scT = scT;

@@ -51,6 +51,7 @@ struct ConvArrays {
     const double* scT;       // N
     const double* scW;       // N: sum_k splitConst_Yk / W_k
     const double* Yleft;     // K (nullable)
+    const double* rW;        // K: 1 / W_k (register path: Wmx and scW formed in the kernel)
     double* state_out;       // N x nv
     double* rec_t;           // maxRec
     double* rec_v;           // maxRec x N
@@ -59,6 +60,9 @@ struct ConvArrays {
     double* work;            // convection_work_doubles
 };
 size_t convection_work_doubles(int N, int K);
+// grids of at most kConvRegMax points take the register path (one point per thread; the
+// kernels form Wmx and the UTW split constants themselves: Wmx, scU, scT, scW unused)
+constexpr int kConvRegMax = 1024;
 int launch_convection(const ConvParams& p, const ConvArrays& a, hipStream_t stream);
 // split constants of the UTW system from the convection split constants (N x nv):
 // scU, scT rows and scW = sum_k sc_Yk / W_k

@@ -36,6 +36,29 @@ __device__ __forceinline__ double dpp_move0(double v) {
     return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+// A DPP move that writes only the rows in ROWMASK; lanes without a source, and the rows
+// left out, read +0.0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_move_rows0(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWMASK, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Inclusive prefix sum over the 64 lanes in lane order: shifts by 1, 2, 4, 8 within each
+// row of 16 lanes, then row 0's total into row 1 and row 2's into row 3 (row_bcast:15),
+// then rows 0-1's total into rows 2 and 3 (row_bcast:31).  Fixed combination order.
+__device__ __forceinline__ double wave_inclusive_scan(double v) {
+    v += dpp_move_rows0<0x111, 0xF>(v);  // row_shr:1
+    v += dpp_move_rows0<0x112, 0xF>(v);  // row_shr:2
+    v += dpp_move_rows0<0x114, 0xF>(v);  // row_shr:4
+    v += dpp_move_rows0<0x118, 0xF>(v);  // row_shr:8
+    v += dpp_move_rows0<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_move_rows0<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // value of `v` held by lane `src` (src uniform across the wave): v_readlane
 __device__ __forceinline__ double bcast(double v, int src) {
     const uint64_t b = __builtin_bit_cast(uint64_t, v);
