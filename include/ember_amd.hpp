@@ -129,7 +129,8 @@ inline int diffusion_right_bc(int emberBC) {
     return emberBC == EMX_BC_FIXED_VALUE ? EMX_BC_FIXED_VALUE : EMX_BC_ZERO_GRADIENT;
 }
 
-// Production and diffusion stages on the GPU (FlameSolver's two parallel_for loops).
+// Production, diffusion and (once set up) convection stages on the GPU (FlameSolver's
+// parallel_for loops and its convection system).
 class FlameStages : public SplitSolver {
 public:
     FlameStages(Context& ctx, const std::vector<double>& x, int alpha = 0,
@@ -164,6 +165,16 @@ public:
     }
     const emx_grid& grid() const { return grid_; }
 
+    // The convection stages (FlameSolver::integrateConvectionTerms, flameSolver.cpp:736-751):
+    // ConvectionSystemSplit's parameters (rVzero, the inflow state, the strain function,
+    // the grid's boundary conditions) and drhodt (FlameSolver::drhodt, nPoints; empty:
+    // zeros).  Until setConvection the convection stages change nothing.
+    void setConvection(const emx_convection_params& p) {
+        conv_ = p;
+        convOn_ = true;
+    }
+    std::vector<double> drhodt;
+
     // Diffusion coefficients as FlameSolver::prepareIntegrators sets them
     // (flameSolver.cpp:160-172): B rows (1/rho, 1/(rho cp)) with B_row per variable,
     // D nVars x nPoints (mu, lambda, rhoD_k).
@@ -174,6 +185,12 @@ public:
     std::vector<int32_t> stat1, stat2;
 
 protected:
+    void integrateConvectionTerms() override {
+        if (!convOn_) return;
+        check(emx_convection_integrate(ctx_.get(), nVars, nPoints, &conv_, x_.data(), splitConstConv.data(),
+                                       drhodt.empty() ? nullptr : drhodt.data(), tStageStart, tStageEnd,
+                                       state.data(), nullptr));
+    }
     void integrateProductionTerms() override {
         status.assign(nPoints, 0);
         stat1.assign(nPoints, 0);
@@ -196,6 +213,8 @@ private:
     Context& ctx_;
     std::vector<double> x_;
     emx_grid grid_;
+    emx_convection_params conv_{};
+    bool convOn_ = false;
 };
 
 }  // namespace ember_amd

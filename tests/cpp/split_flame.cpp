@@ -4,9 +4,13 @@
 // step; and the device-resident split step (emx_split_step_device, state kept in device
 // memory) on the same inputs.
 //
-// usage: split_flame <mech_table> <input.bin> <output.bin> <nsteps> <dt> [leftBC Tinf Kwall]
+// usage: split_flame <mech_table> <input.bin> <output.bin> <nsteps> <dt>
+//                    [leftBC Tinf Kwall [rVzero strain rhou]]
 // (leftBC: Ember's BoundaryCondition value for the left end; with leftBC = 3 (WallFlux)
-// the energy equation carries yInf = Tinf and wallConst = Kwall, flameSolver.cpp:79-82)
+// the energy equation carries yInf = Tinf and wallConst = Kwall, flameSolver.cpp:79-82.
+// rVzero, strain, rhou: the convection stages run too, planar, fixedLeft continuity, a
+// constant strain rate, drhodt = 0; the device-resident step, which has no convection
+// stages, is then left out and its trajectory written as zeros)
 // input.bin: int64 nVars, nPoints, nB; doubles x[N], state[N*nV], B[nB*N], D[N*nV];
 //            int32 B_row[nV]
 // output.bin: doubles gpu[nsteps][N*nV], oracle[nsteps][N*nV], device[nsteps][N*nV];
@@ -32,6 +36,8 @@ public:
     int failures = 0;
     int leftBC = 0;
     double yInf = 0.0, wallConst = 0.0;
+    orc_conv_params conv{};
+    bool convOn = false;
 
 protected:
     void integrateProductionTerms() override {
@@ -41,6 +47,16 @@ protected:
                              tStageEnd - tStageStart, out.data(), status.data(), nullptr, nullptr,
                              nullptr, 0);
         for (int s : status) failures += (s != 0);
+        state.swap(out);
+    }
+    void integrateConvectionTerms() override {
+        if (!convOn) return;
+        std::vector<double> out(state.size());
+        std::vector<int> steps(nVars - 1);
+        const int rc = orc_convection_integrate(mech_, &conv, (int)nPoints, x_.data(), state.data(),
+                                                splitConstConv.data(), nullptr, nullptr, tStageStart,
+                                                tStageEnd, out.data(), steps.data(), 0);
+        if (rc != 0) failures++;
         state.swap(out);
     }
     void integrateDiffusionTerms() override {
@@ -63,10 +79,12 @@ bool rd(FILE* f, T* p, size_t n) { return fread(p, sizeof(T), n, f) == n; }
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 6 && argc != 9) {
-        fprintf(stderr, "usage: %s table input output nsteps dt [leftBC Tinf Kwall]\n", argv[0]);
+    if (argc != 6 && argc != 9 && argc != 12) {
+        fprintf(stderr, "usage: %s table input output nsteps dt [leftBC Tinf Kwall [rVzero strain rhou]]\n",
+                argv[0]);
         return 2;
     }
+    const bool conv = argc == 12;
     const int nsteps = atoi(argv[4]);
     const double dt = atof(argv[5]);
     const int leftBC = argc == 9 ? atoi(argv[6]) : EMX_BC_FIXED_VALUE;
@@ -110,6 +128,42 @@ int main(int argc, char** argv) {
     cpu.B = B;
     cpu.D = D;
     cpu.B_row = B_row;
+    if (conv) {
+        const double rVzero = atof(argv[9]), strain = atof(argv[10]), rhou = atof(argv[11]);
+        emx_convection_params ep{};
+        ep.alpha = 0;
+        ep.beta = 1;
+        ep.left_bc = leftBC;
+        ep.right_bc = EMX_BC_FIXED_VALUE;
+        ep.continuity_bc = 0;
+        ep.rVzero = rVzero;
+        ep.Tleft = st[1];
+        ep.Yleft = &st[2];
+        ep.rhou = rhou;
+        ep.strain_initial = ep.strain_final = strain;
+        ep.strain_t0 = 0.0;
+        ep.strain_dt = 1.0;
+        gpu.setConvection(ep);
+        orc_conv_params& cp = cpu.conv;
+        cp.alpha = 0;
+        cp.beta = 1;
+        cp.left_bc = leftBC;
+        cp.right_bc = EMX_BC_FIXED_VALUE;
+        cp.continuity_bc = 0;
+        cp.pressure = 101325.0;
+        cp.rVzero = rVzero;
+        cp.Tleft = st[1];
+        cp.rhou = rhou;
+        cp.strain_initial = cp.strain_final = strain;
+        cp.strain_t0 = 0.0;
+        cp.strain_dt = 1.0;
+        cp.rtol = eo.integrator_rel_tol;
+        cp.atol_momentum = eo.integrator_momentum_abs_tol;
+        cp.atol_energy = eo.integrator_energy_abs_tol;
+        cp.atol_species = eo.integrator_species_abs_tol;
+        cp.max_steps = 1000000;
+        cpu.convOn = true;
+    }
 
     FILE* o = fopen(argv[3], "wb");
     if (!o) return 5;
@@ -131,7 +185,9 @@ int main(int argc, char** argv) {
     // device-resident split step on the same inputs
     std::vector<double> traj_d;
     int dfail = 0;
-    {
+    if (conv) {
+        traj_d.assign((size_t)nsteps * N * nV, 0.0);
+    } else {
         const size_t n = N * nV;
         double *d_state, *d_ddt, *d_delta, *d_work, *d_B, *d_D, *d_x;
         int32_t *d_Brow, *d_status;

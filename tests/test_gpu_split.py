@@ -2,8 +2,9 @@
 150-point H2/air flame-like profile for 100 steps of dt = 1e-5 s: the C++ SplitSolver
 mirror over the GPU library against the same driver over the CPU oracle, T and Y
 compared after every step (rtol 1e-6); and the device-resident split step
-(emx_split_step_device) against the host-driven one, bit for bit.  Convection is out of
-scope (no-op stage)."""
+(emx_split_step_device) against the host-driven one, bit for bit.  With convection: the
+full Strang sequence D C D P D C D, the convection stages through
+emx_convection_integrate (GPU) and orc_convection_integrate (oracle)."""
 import os
 import subprocess
 
@@ -19,10 +20,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "tests", "cpp", "_build", "split_flame")
 
 
-@pytest.mark.parametrize("wall", [None, (350.0, 200.0)], ids=["fixed", "wallflux"])
-def test_split_flame_100_steps(gpu, tmp_path, wall):
+@pytest.mark.parametrize("wall,conv", [(None, False), ((350.0, 200.0), False), (None, True)],
+                         ids=["fixed", "wallflux", "convection"])
+def test_split_flame_100_steps(gpu, tmp_path, wall, conv):
     """wall = (Tinf, Kwall): the left end is a WallFlux boundary whose energy equation
-    carries yInf = Tinf and wallConst = Kwall (src/flameSolver.cpp:79-82)."""
+    carries yInf = Tinf and wallConst = Kwall (src/flameSolver.cpp:79-82).  conv: the
+    convection stages run too, with inflow rV(0) = 0.2 rho_left (kg/m^2/s) from the left
+    and a strain rate of 100 1/s (rhou = rho_left)."""
     case = W.flame_profile(150, seed=1, name="h2o2")
     N, nV = case.state.shape
     inp = tmp_path / "in.bin"
@@ -33,8 +37,13 @@ def test_split_flame_100_steps(gpu, tmp_path, wall):
         case.B_row.astype(np.int32).tofile(f)
     out = tmp_path / "out.bin"
     nsteps, dt = 100, 1e-5
-    print(f"[split] running 100 split steps, wall={wall}", flush=True)
+    print(f"[split] running 100 split steps, wall={wall}, convection={conv}", flush=True)
     extra = [] if wall is None else ["3", repr(wall[0]), repr(wall[1])]
+    if conv:
+        m = mechanism.load("h2o2")
+        Y0 = case.state[0, 2:]
+        rho0 = 101325.0 / (mechanism.GAS_CONSTANT * case.state[0, 1] * (Y0 / m.W).sum())
+        extra = ["0", "0", "0", repr(float(0.2 * rho0)), "100.0", repr(float(rho0))]
     r = subprocess.run([EXE, mechanism.table_path("h2o2"), str(inp), str(out), str(nsteps), str(dt)]
                        + extra, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -42,9 +51,16 @@ def test_split_flame_100_steps(gpu, tmp_path, wall):
     fails = np.fromfile(out, dtype=np.int32, offset=3 * nsteps * N * nV * 8)
     assert fails.tolist() == [0, 0, 0]
     g, c, d = data
-    # the device-resident step runs the same kernels with the same arithmetic
-    print(f"[split] device-resident vs host-driven max |diff| {np.abs(d - g).max():.3e}")
-    np.testing.assert_array_equal(d, g)
+    if not conv:
+        # the device-resident step runs the same kernels with the same arithmetic
+        print(f"[split] device-resident vs host-driven max |diff| {np.abs(d - g).max():.3e}")
+        np.testing.assert_array_equal(d, g)
+    else:
+        # the strain term drove U up from 0 and the inflow advected the profile
+        assert g[-1, :, 0].max() > 1.0
+        dU = np.abs(g[:, :, 0] - c[:, :, 0]) / (np.abs(c[:, :, 0]) + 1e-3)
+        print(f"[split] max rel dU {dU.max():.2e}")
+        assert dU.max() <= 1e-6
     dT = np.abs(g[:, :, 1] - c[:, :, 1]) / c[:, :, 1]
     dY = np.abs(g[:, :, 2:] - c[:, :, 2:]) - 1e-6 * np.abs(c[:, :, 2:])
     print(f"[split] max rel dT {dT.max():.2e}, max Y excess {dY.max():.2e}")
