@@ -442,6 +442,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_forward_kernel(
     const int nv = m.nVars;
     const bool active = lane < nv;
     QssSystem S;
+    S.prio = 0;
     S.W = is_species_lane(lane, m) ? m.W[lane - 2] : 0.0;
     S.rW = is_species_lane(lane, m) ? m.rW[lane - 2] : 0.0;
     RateRegs<NR> R;
@@ -503,6 +504,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_backward_kernel(
     const bool active = lane < nv;
     const bool sp = is_species_lane(lane, m);
     QssSystem S;
+    S.prio = 0;
     S.W = sp ? m.W[lane - 2] : 0.0;
     S.rW = sp ? m.rW[lane - 2] : 0.0;
     RateRegs<NR> R;

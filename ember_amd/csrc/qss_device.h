@@ -49,7 +49,19 @@ struct QssSystem {
     double hk;         // J/kmol, species lanes
     double qdot;
     double wQ, wD;     // creation / destruction rate of this lane's species (last evaluation)
+    int prio;          // issue-priority toggle (alternate_priority)
 };
+
+// Alternates this wave's issue priority between 0 and 1 at each call (once per right-hand
+// side).  The SIMD otherwise favours the older of its two waves, so whichever point a
+// later-dispatched wave holds would progress more slowly; with the toggle the two waves
+// take turns, and points progress at about the same rate (which the cost-ordered queue
+// assumes).
+__device__ __forceinline__ void alternate_priority(int& prio) {
+    prio ^= 1;
+    if (prio) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
 
 // EXT: the call carries the time-dependent external terms (rate multiplier factor fa,
 // igniter heat release qi); without them fa = 1 and qi = 0 compile away.
@@ -57,6 +69,7 @@ template <int NR, bool INTERP, bool EXT = false>
 __device__ inline bool qss_odefun(const DevMech& m, int lane, double y, bool corrector,
                                   QssSystem& S, RateRegs<NR>& R, const double* perturb,
                                   double fa, double qi, const WaveLds& L, double& q, double& d) {
+    alternate_priority(S.prio);
     if (!corrector) S.T = bcast(y, 1);  // unroll_y: T not updated in correctors
     GasState g = set_state(m, lane, y, S.rW, S.T, false, L);
     bool ok = true;
@@ -145,12 +158,6 @@ __device__ __forceinline__ int qss_integrate(const DevMech& m, const DevOpts& o,
         if (!qss_odefun<NR, INTERP, EXT>(m, lane, Z.y, !predict, S, R, perturb, fa, qi, L, Z.q, Z.d))
             return -2;
         gcount += 1;
-        // Alternate this wave's issue priority with its evaluation count: the two waves
-        // sharing a SIMD then take turns at priority instead of the older one always
-        // issuing first, so points progress at the same rate whichever wave holds them
-        // (the cost-ordered queue assumes they do)
-        if (gcount & 1) __builtin_amdgcn_s_setprio(1);
-        else __builtin_amdgcn_s_setprio(0);
         if (predict) H.predicted(lane, Z, S);
         else if (!H.corrected(lane, Z, S, dt)) return -5;
         bool start_attempt = false;

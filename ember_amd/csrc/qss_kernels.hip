@@ -39,6 +39,7 @@ __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
     const int nv = m.nVars;
     const bool active = lane < nv;
     QssSystem S;
+    S.prio = 0;
     S.W = is_species_lane(lane, m) ? m.W[lane - 2] : 0.0;
     S.rW = is_species_lane(lane, m) ? m.rW[lane - 2] : 0.0;
     RateRegs<NR> R;
