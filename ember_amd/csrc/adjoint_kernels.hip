@@ -165,6 +165,9 @@ __device__ __forceinline__ bool rhs_vjp(const DevMech& m, int lane, double yy, b
             const int s = j * 64 + lane;
             const int w = m.info[s];
             const int kind = info_kind(w);
+            // the slot's Arrhenius parameters (global memory), issued before the gathers
+            // below so that their latency overlaps them
+            const double sb = m.b[s], sE = m.EaR[s], sdn = m.dn[s];
             double pf = 0.0, pr = 0.0, mb = 0.0;
             if (kind != kNone) {
                 const bool rev = info_rev(w);
@@ -216,7 +219,7 @@ __device__ __forceinline__ bool rhs_vjp(const DevMech& m, int lane, double yy, b
                     if (INTERP)
                         Trl += kfb * m.kf_s[nI * S + s];
                     else
-                        Trl += kfb * kf * (m.b[s] * recipT + m.EaR[s] * recipT * recipT);
+                        Trl += kfb * kf * (sb * recipT + sE * recipT * recipT);
                 } else {
                     const double prf = pr0 * (F / (1.0 + pr0));
                     double khighb = kfb * prf;
@@ -252,14 +255,14 @@ __device__ __forceinline__ bool rhs_vjp(const DevMech& m, int lane, double yy, b
                     const double klowb = pr0b * Mgv / kd;
                     khighb += -pr0b * Mgv * klow / (kd * kd);
                     Trl += klowb * klow * (m.b0[s] * recipT + m.EaR0[s] * recipT * recipT);
-                    Trl += khighb * khigh * (m.b[s] * recipT + m.EaR[s] * recipT * recipT);
+                    Trl += khighb * khigh * (sb * recipT + sE * recipT * recipT);
                 }
                 if (rev) {
                     const double rkcb = roprb * base * PCp;
                     if (INTERP) {
                         Trl += rkcb * m.kc_s[nI * S + s];
                     } else if (rk < kBigNumber) {
-                        const double sh = m.dn[s] + A.hrt[r0] + A.hrt[r1] + A.hrt[r2] - A.hrt[p0] -
+                        const double sh = sdn + A.hrt[r0] + A.hrt[r1] + A.hrt[r2] - A.hrt[p0] -
                                           A.hrt[p1] - A.hrt[p2];
                         Trl += rkcb * rk * sh * recipT;
                     }
@@ -465,6 +468,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_forward_kernel(
         CkptHooks H{ck_y + (size_t)p * cap * nv, ck_s + (size_t)p * cap * 2, cap, nv, 0, 0};
         int g = 0, r = 0;
         double qi = 0.0;
+        const unsigned long long t_point = point_clock();
         int err = qss_integrate<NR, INTERP>(m, o, lane, active, Z, S, R, perturb, L, dt,
                                             QssResume{0.0, 0.0, true}, H, g, r, qi);
         if (err == 1) err = -5;  // checkpoint capacity
@@ -472,7 +476,7 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_forward_kernel(
         if (lane == 0) {
             if (status) status[p] = err;
             if (Q_out) Q_out[p] = qi;
-            if (cost) cost[p] = g;
+            if (cost) cost[p] = point_cost(t_point);
             nckpt[p] = err >= 0 ? H.nckpt : 0;
         }
     }

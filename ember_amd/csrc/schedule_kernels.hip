@@ -1,8 +1,7 @@
 // Cost-ordered work queue for the chemistry stage kernels.
 //
-// The stage kernels record a cost per point: the integration time of its last stage
-// (QSS and BDF stages, in 640 ns units of the real-time counter) or its right-hand-side
-// evaluations (the adjoint's forward stage).  Before the next stage over the same points,
+// The stage kernels record a cost per point: the integration time of its last stage (the
+// QSS, BDF and adjoint forward stages, in 640 ns units of the real-time counter).  Before the next stage over the same points,
 // one workgroup bins the points by cost (1/16-octave bins, costs up to 2^16 resolved) and
 // writes them in descending cost order, so the
 // persistent waves take the most expensive points first (longest-processing-time
