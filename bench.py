@@ -185,8 +185,8 @@ def run_stage(a, dev):
     torch.cuda.synchronize()
     ms_fifo = timed(a.steps)
     print(f"[bench] index-order queue: {ms_fifo:.3f} ms/stage", file=sys.stderr, flush=True)
-    # cost-ordered queue: each stage takes the points in descending order of the previous
-    # stage's right-hand-side evaluation counts
+    # cost-ordered queue: each stage takes the points in descending order of their measured
+    # integration times in the previous stage
     ms = ms_fifo
     if a.schedule == "cost":
         ctx.set_schedule(capi.EMX_SCHED_COST)
@@ -246,7 +246,7 @@ def run_stage(a, dev):
         "first_stage_index_order": {"ms_per_step": ms_fifo, "value": N / (ms_fifo * 1e-3),
                                     "note": "work queue in index order, as a stage over new "
                                             "points runs; the headline is the cost-ordered queue "
-                                            "(previous stage's evaluation counts)"},
+                                            "(the points' measured times in the previous stage)"},
         "host_buffer_ms_per_step": ms_e2e,
     }
     if ms_plain:

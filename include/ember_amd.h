@@ -148,7 +148,7 @@ int emx_set_kernel_variant(emx_context* ctx, int variant);
 
 /* Work-queue order of the chemistry kernels.  EMX_SCHED_COST (default): when the
  * previous stage ran over the same number of points, the points are taken in
- * descending order of their previous right-hand-side evaluation counts (most expensive
+ * descending order of their measured integration times in that stage (most expensive
  * first); otherwise, and with EMX_SCHED_FIFO, in index order.  Results do not depend on
  * the order.  No counterpart in the reference: TBB's dynamic grain-1 scheduling over
  * points (src/flameSolver.cpp:758-759) is the analogue. */
