@@ -686,7 +686,13 @@ __global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const dou
                                                         const int* nsteps, int q_solve, int q_next,
                                                         const double* X, double* own, double* nxt) {
     const int v = lane_id();
-    const int chunk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+    // XCD-aware block order: workgroups are dispatched round-robin over the 8 XCDs, so
+    // block b runs on XCD b % 8; give the blocks of one XCD consecutive chunk groups, so
+    // that the chunks whose summaries share a cache line (consecutive chunks of a plane)
+    // are written through one L2 and leave it as whole lines
+    const int nb = gridDim.x;
+    const int lb = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
+    const int chunk = __builtin_amdgcn_readfirstlane(lb * (blockDim.x / 64) + threadIdx.x / 64);
     const int N = R.N;
     const int C = (N + P - 1) / P;
     if (chunk >= C || v >= R.nV) return;
