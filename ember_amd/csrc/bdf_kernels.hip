@@ -125,55 +125,205 @@ __device__ __forceinline__ void change_D(double (&D)[ND], int order, double fact
     wave_lds_sync();
 }
 
-// LU of M = I - c J in LDS (M[j*64 + lane] = row `lane`, column j), row-per-lane,
-// partial pivoting: at step k the pivot is the not-yet-pivoted lane with the largest
-// |M[lane][k]| (lowest lane on ties).  step_of = k for the pivot lane of step k.
-__device__ __forceinline__ bool lu_factor(double* M, const double* J, double c, int n, int lane, int& step_of) {
+// Factors of the Newton matrix M = I - c J, kept between right-hand sides.
+//   M (LDS): row i, column j at M[(j / 2) * 128 + 2 i + (j % 2)]: a lane's two entries of a
+//            column pair are adjacent, so one 16-byte access moves both.  Factored in place
+//            without row exchanges (L multipliers in the rows not yet pivoted at their step,
+//            U in the pivot rows).
+//   src:     lane k holds the row that was the pivot of step k (lanes >= n: k itself)
+//   u, r:    lane k holds U of step k's pivot, U[src_k][k], and 1 / U[src_k][k]
+struct LuFactors {
+    int src;
+    double u, r;
+};
+
+__device__ __forceinline__ int lu_index(int row, int col) { return (col >> 1) * 128 + 2 * row + (col & 1); }
+
+// Pivot of one elimination step over column values `a` (this lane's entry): the
+// not-yet-pivoted row with the largest |a| (lowest lane on ties).  Returns the pivot lane
+// and 1 / a[pivot] (IEEE division), or -1 if every candidate is zero (singular).
+__device__ __forceinline__ int lu_pivot(double a, bool used, double& inv) {
+    const double cand = used ? -1.0 : fabs(a);
+    const double best = wave_max(cand);
+    if (!(best > 0.0)) return -1;
+    const int p = __ffsll(__ballot(!used && cand == best)) - 1;
+    inv = 1.0 / bcast(a, p);
+    return p;
+}
+
+// LU with partial pivoting, two elimination steps per pass over the trailing columns.
+// Step k: pivot p1 on column k, l1 = M[i][k] * (1 / M[p1][k]); step k + 1: pivot p2 on
+// column k + 1 after step k, l2 likewise.  A trailing entry becomes
+// v1 = M[i][j] - l1 M[p1][j], then v2 = v1 - l2 v1(p2) (one FMA each, in that order: the
+// same operations as two single steps), written once: v2 for rows pivoted after k + 1,
+// v1 for p2, nothing for p1 and earlier pivots.  Column pairs are read and written 16
+// bytes at a time (the pivot rows' pairs as broadcast reads; v_readlane costs ~10-15
+// cycles of issue on gfx950, an LDS read of 16 bytes a few), and a block's loads are
+// issued before the previous block's stores.
+__device__ __forceinline__ bool lu_factor(double* M, const double* J, double c, int n, int lane,
+                                          LuFactors& lu) {
     const bool row = lane < n;
-    for (int j = 0; j < n; j++)
-        M[j * 64 + lane] = row ? ((lane == j ? 1.0 : 0.0) - c * J[j * 64 + lane]) : 0.0;
-    wave_lds_sync();
-    step_of = row ? 1 << 30 : -1;
-    bool used = !row;
-    for (int k = 0; k < n; k++) {
-        const double akk = M[k * 64 + lane];
-        double cand = used ? -1.0 : fabs(akk);
-        double best = wave_max(cand);
-        if (!(best > 0.0)) return false;
-        unsigned long long ball = __ballot(!used && cand == best);
-        const int p = __ffsll(ball) - 1;
-        if (lane == p) {
-            used = true;
-            step_of = k;
+    const int npair = (n + 1) >> 1;
+    double2* Mp = reinterpret_cast<double2*>(M) + lane;  // pair q of this row: Mp[q * 64]
+    // the per-wave Jacobian slot holds 64 columns (those >= n unused): 16 loads in flight
+    for (int q0 = 0; q0 < npair; q0 += 8) {
+        double jv[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) jv[t] = J[(2 * q0 + t) * 64 + lane];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int j = 2 * (q0 + t);
+            double2 v;
+            v.x = (row && j < n) ? ((lane == j ? 1.0 : 0.0) - c * jv[2 * t]) : 0.0;
+            v.y = (row && j + 1 < n) ? ((lane == j + 1 ? 1.0 : 0.0) - c * jv[2 * t + 1]) : 0.0;
+            if (q0 + t < npair) Mp[(q0 + t) * 64] = v;
         }
-        const double inv = 1.0 / bcast(akk, p);
-        const double l = akk * inv;
-        if (!used) M[k * 64 + lane] = l;
-        for (int j = k + 1; j < n; j++) {
-            const double pj = M[j * 64 + p];
-            if (!used) M[j * 64 + lane] -= l * pj;
+    }
+    wave_lds_sync();
+    bool used = !row;
+    int step = lane;
+    double rd = 0.0;
+    for (int k = 0; k < n; k += 2) {
+        const int q = k >> 1;
+        const double2 col = Mp[q * 64];
+        double inv1;
+        const int p1 = lu_pivot(col.x, used, inv1);
+        if (p1 < 0) return false;
+        const bool before = used;  // pivoted at an earlier pass
+        if (lane == p1) {
+            used = true;
+            step = k;
+            rd = inv1;
+        }
+        const double l1 = col.x * inv1;
+        if (k + 1 == n) {  // odd n: the last step has no column after it
+            if (!used) M[lu_index(lane, k)] = l1;
+            break;
+        }
+        // column k + 1 after step k, then step k + 1
+        const double a1 = col.y - l1 * bcast(col.y, p1);
+        double inv2;
+        const int p2 = lu_pivot(a1, used, inv2);
+        if (p2 < 0) return false;
+        const bool is_p2 = lane == p2;
+        if (is_p2) {
+            used = true;
+            step = k + 1;
+            rd = inv2;
+        }
+        const double l2 = a1 * inv2;
+        // rows written this pass: p2 and those still not pivoted
+        const bool write = !before && lane != p1;
+        if (write) Mp[q * 64] = make_double2(l1, is_p2 ? a1 : l2);
+        // Trailing pairs q + 1 .. npair - 1, 2 pairs per block (loads past the last pair
+        // clamp to it).  The pivot rows' pairs are broadcast LDS reads; row p2 after step k
+        // is formed in every lane as M[p2][j] - l1(p2) M[p1][j], the FMA lane p2 itself does.
+        const double l1p2 = bcast(l1, p2);
+        const double2* R1 = reinterpret_cast<const double2*>(M) + p1;
+        const double2* R2 = reinterpret_cast<const double2*>(M) + p2;
+        const int qlast = npair - 1;
+        double2 own[2], r1[2], r2[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int qq = min(q + 1 + t, qlast) * 64;
+            own[t] = Mp[qq];
+            r1[t] = R1[qq];
+            r2[t] = R2[qq];
+        }
+        for (int q0 = q + 1; q0 <= qlast; q0 += 2) {
+            double2 nown[2], nr1[2], nr2[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int qq = min(q0 + 2 + t, qlast) * 64;
+                nown[t] = Mp[qq];
+                nr1[t] = R1[qq];
+                nr2[t] = R2[qq];
+            }
+            double2 res[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const double x1 = own[t].x - l1 * r1[t].x;
+                const double y1 = own[t].y - l1 * r1[t].y;
+                const double ux = r2[t].x - l1p2 * r1[t].x;
+                const double uy = r2[t].y - l1p2 * r1[t].y;
+                const double x2 = x1 - l2 * ux;
+                const double y2 = y1 - l2 * uy;
+                res[t] = is_p2 ? make_double2(x1, y1) : make_double2(x2, y2);
+            }
+            if (write) {
+                Mp[q0 * 64] = res[0];
+                if (q0 + 1 <= qlast) Mp[(q0 + 1) * 64] = res[1];
+            }
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                own[t] = nown[t];
+                r1[t] = nr1[t];
+                r2[t] = nr2[t];
+            }
         }
         wave_lds_sync();
     }
+    wave_lds_sync();
+    // src: lane step_i receives i (ds_permute writes lane i's value into lane addr / 4)
+    lu.src = __builtin_amdgcn_ds_permute(step * 4, lane);
+    const uint64_t rb = __builtin_bit_cast(uint64_t, rd);
+    const int rlo = __builtin_amdgcn_ds_permute(step * 4, (int)(uint32_t)rb);
+    const int rhi = __builtin_amdgcn_ds_permute(step * 4, (int)(uint32_t)(rb >> 32));
+    lu.r = __builtin_bit_cast(double, ((uint64_t)(uint32_t)rhi << 32) | (uint32_t)rlo);
+    lu.u = 1.0;
+    if (row) lu.u = M[lu_index(lu.src, lane)];
     return true;
 }
 
+__device__ __forceinline__ double lane_gather(double v, int src) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_ds_bpermute(src * 4, (int)(uint32_t)b);
+    const int hi = __builtin_amdgcn_ds_bpermute(src * 4, (int)(uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
 // Solve M x = b; b is this lane's row value (lane = row); returns x for lane = variable.
-__device__ __forceinline__ double lu_solve(const double* M, int n, int lane, int step_of, double b) {
-    // forward: rows in pivot order
-    for (int k = 0; k < n; k++) {
-        const int p = __ffsll(__ballot(step_of == k)) - 1;
-        const double zk = bcast(b, p);
-        if (step_of > k) b -= M[k * 64 + lane] * zk;
+// Lane k works on the pivot row of step k (the right-hand side gathered into that
+// order): forward substitution, then back substitution, in which the division by the
+// pivot is the reciprocal times the value with one correction (r = 1/u rounded to
+// nearest, q0 = z r, q = q0 + (z - q0 u) r: the correctly rounded quotient by Markstein's
+// theorem).  The LDS reads do not depend on the running values and are issued ahead.
+__device__ __forceinline__ double lu_solve(const double* M, int n, int lane, const LuFactors& lu, double b) {
+    // this lane's row in pivot order, column pair q at Mr[q * 64] (16-byte reads)
+    const double2* Mr = reinterpret_cast<const double2*>(M) + lu.src;
+    const int qlast = (n - 1) >> 1;
+    double z = lane_gather(b, lu.src);
+    // 8 steps (4 column pairs) per block, unrolled by hand (the loop holds v_readlane,
+    // which the compiler does not unroll at run-time trip counts); steps past the end read
+    // a clamped pair and change nothing
+    for (int q0 = 0; q0 <= qlast; q0 += 4) {
+        double2 m[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) m[t] = Mr[min(q0 + t, qlast) * 64];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int k = 2 * q0 + t;
+            const double mk = (t & 1) ? m[t >> 1].y : m[t >> 1].x;
+            const double zk = bcast(z, min(k, 63));
+            z -= (lane > k && k < n ? mk : 0.0) * zk;
+        }
     }
     double x = 0.0;
-    for (int k = n - 1; k >= 0; k--) {
-        const int p = __ffsll(__ballot(step_of == k)) - 1;
-        double xk = 0.0;
-        if (lane == p) xk = b / M[k * 64 + lane];
-        xk = bcast(xk, p);
-        if (lane == k) x = xk;
-        if (step_of < k && step_of >= 0) b -= M[k * 64 + lane] * xk;
+    for (int q0 = qlast; q0 >= 0; q0 -= 4) {
+        double2 m[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) m[t] = Mr[max(q0 - t, 0) * 64];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int k = 2 * q0 + 1 - t;  // columns 2 q0 + 1, 2 q0, 2 q0 - 1, ...
+            const double mk = (t & 1) ? m[t >> 1].x : m[t >> 1].y;
+            const bool live = k >= 0 && k < n;
+            const double q0v = z * lu.r;
+            const double q = fma(fma(-q0v, lu.u, z), lu.r, q0v);
+            const double xk = bcast(q, live ? k : 0);
+            if (live && lane == k) x = xk;
+            z -= (live && lane < k ? mk : 0.0) * xk;
+        }
     }
     return x;
 }
@@ -232,7 +382,7 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
     D[0] = y;
     D[1] = f0 * h_abs;
     bool lu_valid = false;
-    int step_of = -1;
+    LuFactors lu;
     ok = fd_jacobian<NR, INTERP>(m, lane, n, y, f0, S, R, perturb, fn_mult(o, 0.0), igniter(o, S.qign, 0.0),
                                  L, J) && ok;
     if (!ok) return -2;
@@ -275,7 +425,7 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
             bool converged = false;
             while (!converged) {
                 if (!lu_valid) {
-                    if (!lu_factor(M, J, c, n, lane, step_of)) return -5;
+                    if (!lu_factor(M, J, c, n, lane, lu)) return -5;
                     lu_valid = true;
                 }
                 dvec = 0.0;
@@ -288,7 +438,7 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
                                            igniter(o, S.qign, t_new), L, fy) && ok;
                     const int bad = wave_or(active && !isfinite(fy) ? 1 : 0);
                     if (bad) break;
-                    double dy = lu_solve(M, n, lane, step_of, c * fy - psi - dvec);
+                    double dy = lu_solve(M, n, lane, lu, c * fy - psi - dvec);
                     const double dy_norm = rms(dy, scale, active, n);
                     const double rate = (dy_norm_old < 0) ? -1.0 : dy_norm / dy_norm_old;
                     if (rate >= 0 && (rate >= 1 || pow(rate, kNewtonMaxIter - k) / (1 - rate) * dy_norm > newton_tol))
@@ -403,12 +553,12 @@ __global__ void __launch_bounds__(256) bdf_stage_kernel(
     const int wpb = blockDim.x / 64;
     const int nv = m.nVars;
     const int chem = wave_lds_doubles(m.nRounds, m.nChunkRounds);
-    const int per_wave = chem + nv * 64 + 64;
+    const int per_wave = chem + ((nv + 1) / 2) * 128 + 64;
     double* base = lds + wave * per_wave;
     WaveLds L = carve_lds(base, m.nRounds, m.nChunkRounds);
     init_wave_lds(L, lane, m.nRounds, m.nChunkRounds);
     double* M = base + chem;
-    double* ru = M + nv * 64;
+    double* ru = M + ((nv + 1) / 2) * 128;
     double* J = jac_scratch + (size_t)(blockIdx.x * wpb + wave) * 64 * 64;
     const bool active = lane < nv;
     BdfSystem S;
@@ -453,7 +603,7 @@ __global__ void reset_queue_kernel(unsigned int* q) {
 
 size_t bdf_lds_bytes(const DevMech& m, int waves_per_block) {
     return (size_t)waves_per_block *
-           (wave_lds_doubles(m.nRounds, m.nChunkRounds) + m.nVars * 64 + 64) * sizeof(double);
+           (wave_lds_doubles(m.nRounds, m.nChunkRounds) + ((m.nVars + 1) / 2) * 128 + 64) * sizeof(double);
 }
 
 int launch_bdf_stage(const DevMech& m, const DevOpts& o, const double* state_in,
@@ -468,7 +618,7 @@ int launch_bdf_stage(const DevMech& m, const DevOpts& o, const double* state_in,
         hipLaunchKernelGGL((bdf_stage_kernel<NRV, true>), dim3(grid_blocks),                    \
                            dim3(64 * waves_per_block), lds, stream, m, o, state_in, split_const, \
                            qloss, perturb, dt, nPoints, order, queue, state_out, status, stat1,  \
-                           stat2, cost, qint, jac_scratch);                                           \
+                           stat2, cost, qint, jac_scratch);                                      \
     else                                                                                        \
         hipLaunchKernelGGL((bdf_stage_kernel<NRV, false>), dim3(grid_blocks),                   \
                            dim3(64 * waves_per_block), lds, stream, m, o, state_in, split_const, \
