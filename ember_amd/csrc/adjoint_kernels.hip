@@ -37,25 +37,26 @@ struct AdjLds {
     double* wDb;    // [64] adjoint of destruction rates
     double* hrt;    // [64] h/RT of each species (index 63: zero)
     double* Mgb;    // [64] adjoint of third-body concentrations per group
-    double* MB;     // [nRounds*64] per-slot third-body adjoint contributions
+    double* MB;     // [nTbRounds*64] per-slot third-body adjoint contributions
     double* att_dt; // [kTapeAttempts] attempt step sizes of the recorded interval
     double* att_qd; // [kTapeAttempts] heat release rate of each attempt's corrector evaluation
     double* stp_qd; // [kCkptInterval] heat release rate of each step's predictor evaluation
     int* att0;      // [kCkptInterval + 1] first attempt of each recorded step
 };
 
-__host__ __device__ inline int adj_lds_doubles(int nRounds) {
-    return 4 * 64 + nRounds * 64 + 2 * kTapeAttempts + kCkptInterval + (kCkptInterval + 2) / 2 + 1;
+// MB covers the rounds holding third-body slots (they come first in device order)
+__host__ __device__ inline int adj_lds_doubles(const DevMech& m) {
+    return 4 * 64 + m.nTbRounds * 64 + 2 * kTapeAttempts + kCkptInterval + (kCkptInterval + 2) / 2 + 1;
 }
 
-__device__ inline AdjLds carve_adj(double* base, int nRounds) {
+__device__ inline AdjLds carve_adj(double* base, const DevMech& m) {
     AdjLds a;
     a.wQb = base;
     a.wDb = base + 64;
     a.hrt = base + 128;
     a.Mgb = base + 192;
     a.MB = base + 256;
-    a.att_dt = a.MB + nRounds * 64;
+    a.att_dt = a.MB + m.nTbRounds * 64;
     a.att_qd = a.att_dt + kTapeAttempts;
     a.stp_qd = a.att_qd + kTapeAttempts;
     a.att0 = reinterpret_cast<int*>(a.stp_qd + kCkptInterval);
@@ -69,6 +70,32 @@ __device__ inline void init_adj_lds(const AdjLds& A, int lane) {
         A.hrt[kPadSpecies] = 0.0;
     }
     wave_lds_sync();
+}
+
+// Adjoint tables (group -> slots, species -> groups, chunk -> species) staged in LDS by the
+// backward kernel after the per-wave regions: every VJP walks them.
+__host__ __device__ inline int adj_tables_doubles(const DevMech& m) {
+    const int ints = (m.nGroups + 1) + m.nGrpSlots + (m.K + 1) + m.nSpGrp + m.nChunkRounds * 64;
+    return m.nSpGrp + (ints + 1) / 2;
+}
+
+// Block-cooperative copy; the caller synchronises the block.
+__device__ inline void stage_adj_tables(DevMech& v, const DevMech& m, double* base) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < m.nSpGrp; i += nt) base[i] = m.sp_grp_eff[i];
+    v.sp_grp_eff = base;
+    int32_t* q = reinterpret_cast<int32_t*>(base + m.nSpGrp);
+    auto cpi = [&](const int32_t* src, int n) {
+        for (int i = tid; i < n; i += nt) q[i] = src[i];
+        const int32_t* r = q;
+        q += n;
+        return r;
+    };
+    v.grp_slot_off = cpi(m.grp_slot_off, m.nGroups + 1);
+    v.grp_slot = cpi(m.grp_slot, m.nGrpSlots);
+    v.sp_grp_off = cpi(m.sp_grp_off, m.K + 1);
+    v.sp_grp = cpi(m.sp_grp, m.nSpGrp);
+    v.chunk_sp = cpi(m.chunk_sp, m.nChunkRounds * 64);
 }
 
 __global__ void reset_queue_kernel_adj(unsigned int* q) {
@@ -270,7 +297,7 @@ __device__ __forceinline__ bool rhs_vjp(const DevMech& m, int lane, double yy, b
             }
             L.ropf[s] = pf;  // reactant-side product adjoint
             L.ropr[s] = pr;  // product-side product adjoint
-            A.MB[s] = mb;
+            if (j < m.nTbRounds) A.MB[s] = mb;  // zero in the later rounds
         }
     }
     wave_lds_sync();
@@ -493,15 +520,20 @@ __global__ void __launch_bounds__(512, 2) qss_adjoint_backward_kernel(
     double* __restrict__ grad_state, double* __restrict__ grad_lnk, int* __restrict__ status) {
     extern __shared__ double lds[];
     const double* perturb;
-    const DevMech m = stage_mech(mg, perturb_g, lds, perturb);
+    DevMech ms = stage_mech(mg, perturb_g, lds, perturb);
+    {
+        const int pw = wave_lds_doubles(mg.nRounds, mg.nChunkRounds) + adj_lds_doubles(mg);
+        stage_adj_tables(ms, mg, lds + mech_lds_doubles(mg) + (int)(blockDim.x / 64) * pw);
+    }
+    const DevMech m = ms;
     __syncthreads();
     const int lane = lane_id();
     const int wave = wave_in_block();
-    const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds) + adj_lds_doubles(m.nRounds);
+    const int per_wave = wave_lds_doubles(m.nRounds, m.nChunkRounds) + adj_lds_doubles(m);
     double* wbase = lds + mech_lds_doubles(m) + wave * per_wave;
     WaveLds L = carve_lds(wbase, m.nRounds, m.nChunkRounds);
     init_wave_lds(L, lane, m.nRounds, m.nChunkRounds);
-    AdjLds A = carve_adj(wbase + wave_lds_doubles(m.nRounds, m.nChunkRounds), m.nRounds);
+    AdjLds A = carve_adj(wbase + wave_lds_doubles(m.nRounds, m.nChunkRounds), m);
     init_adj_lds(A, lane);
     double* rows = record + (size_t)(blockIdx.x * (blockDim.x / 64) + wave) * kRecordDoubles;
     const int nv = m.nVars;
@@ -681,8 +713,9 @@ size_t adjoint_record_doubles_per_wave() { return (size_t)kRecordDoubles; }
 int adjoint_checkpoint_interval() { return kCkptInterval; }
 
 size_t adjoint_lds_bytes(const DevMech& m, int waves_per_block, bool backward) {
-    size_t per = wave_lds_doubles(m.nRounds, m.nChunkRounds) + (backward ? adj_lds_doubles(m.nRounds) : 0);
-    return ((size_t)mech_lds_doubles(m) + (size_t)waves_per_block * per) * sizeof(double);
+    size_t per = wave_lds_doubles(m.nRounds, m.nChunkRounds) + (backward ? adj_lds_doubles(m) : 0);
+    return ((size_t)mech_lds_doubles(m) + (size_t)waves_per_block * per + (backward ? adj_tables_doubles(m) : 0)) *
+           sizeof(double);
 }
 
 int launch_qss_adjoint_forward(const DevMech& m, const DevOpts& o, const double* state_in,

@@ -198,6 +198,16 @@ int upload_mech(emx_context* c) {
     d.nGrpEntries = (int)L.grp_sp.size();
     d.grpMaxLen = 0;
     for (int g = 0; g < L.nGroups; g++) d.grpMaxLen = std::max(d.grpMaxLen, L.grp_off[g + 1] - L.grp_off[g]);
+    d.nGrpSlots = (int)L.grp_slot.size();
+    d.nSpGrp = (int)L.sp_grp.size();
+    d.grpSlotMax = d.spGrpMax = 0;
+    for (int g = 0; g < L.nGroups; g++) d.grpSlotMax = std::max(d.grpSlotMax, L.grp_slot_off[g + 1] - L.grp_slot_off[g]);
+    for (int k = 0; k < m.K; k++) d.spGrpMax = std::max(d.spGrpMax, L.sp_grp_off[k + 1] - L.sp_grp_off[k]);
+    // slots with a third-body group come first in device order (falloff, three-body)
+    int last_tb = 0;
+    for (int s = 0; s < m.I; s++)  // padding slots (kNone) read as group 0: not counted
+        if (info_group(L.info[s]) < L.nGroups) last_tb = s;
+    d.nTbRounds = last_tb / 64 + 1;
     d.interp = c->opts.kinetics_model == EMX_KIN_INTERP ? 1 : 0;
     d.P = c->P;
     d.Pref = m.Pref;

@@ -105,6 +105,10 @@ struct DevMech {
     const int32_t* sp_grp;
     const double* sp_grp_eff;
     const int32_t* chunk_sp;      // [nChunkRounds * 64]
+    int nGrpSlots;                // entries of grp_slot (>= 1)
+    int nSpGrp;                   // entries of sp_grp and sp_grp_eff (>= 1)
+    int grpSlotMax, spGrpMax;     // longest group slot list, longest species group list
+    int nTbRounds;                // rounds holding slots with a third-body group (>= 1)
     // pressure-dependent rates (PLOG, Chebyshev) at the context pressure: per slot offset
     // into auxd (-1: none); layouts in build_pressure_tables (mech_host.h)
     const int32_t* aux;
