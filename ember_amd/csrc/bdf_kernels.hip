@@ -51,7 +51,7 @@ struct BdfSystem {
 // SourceSystemCVODE::f for this lane's variable.  Returns false if rates failed.
 // Every device function below is forced inline: outlined, its LDS pointers lose their
 // address space (flat loads) and the rate-constant registers go to the stack.
-template <int NR, bool INTERP>
+template <int NR, bool INTERP, bool ORD>
 __device__ __forceinline__ bool bdf_f(const DevMech& m, int lane, double yv, BdfSystem& S, RateRegs<NR>& R,
                              const double* perturb, double fa, double qi, const WaveLds& L,
                              double& ydot) {
@@ -60,7 +60,7 @@ __device__ __forceinline__ bool bdf_f(const DevMech& m, int lane, double yv, Bdf
     GasState g = set_state(m, lane, yv, S.rW, T, true, L);
     double hRT, cpR;
     bool ok = update_T<NR, INTERP>(m, lane, g, R, L, hRT, cpR);
-    rates_of_progress<NR>(m, lane, g, R, perturb, fa, L);
+    rates_of_progress<NR, ORD>(m, lane, g, R, perturb, fa, L);
     double wQ, wD;
     species_rates(m, lane, L, wQ, wD);
     const bool sp = is_species_lane(lane, m);
@@ -329,7 +329,7 @@ __device__ __forceinline__ double lu_solve(const double* M, int n, int lane, con
 }
 
 // SourceSystemCVODE::fdJacobian: column i written to J[i*64 + lane]
-template <int NR, bool INTERP>
+template <int NR, bool INTERP, bool ORD>
 __device__ __forceinline__ bool fd_jacobian(const DevMech& m, int lane, int n, double yv, double ydot, BdfSystem& S,
                             RateRegs<NR>& R, const double* perturb, double fa, double qi,
                             const WaveLds& L, double* J) {
@@ -342,13 +342,13 @@ __device__ __forceinline__ bool fd_jacobian(const DevMech& m, int lane, int n, d
         double yp = yv;
         if (lane == i) yp += dy;
         double yd2;
-        ok = bdf_f<NR, INTERP>(m, lane, yp, S, R, perturb, fa, qi, L, yd2) && ok;
+        ok = bdf_f<NR, INTERP, ORD>(m, lane, yp, S, R, perturb, fa, qi, L, yd2) && ok;
         J[i * 64 + lane] = (yd2 - ydot) / dy;
     }
     return ok;
 }
 
-template <int NR, bool INTERP>
+template <int NR, bool INTERP, bool ORD>
 __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o, int lane, bool active, double& y,
                              BdfSystem& S, RateRegs<NR>& R, const double* perturb, const WaveLds& L,
                              double* M, double* ru, double* J, double tf, int& nsteps,
@@ -361,14 +361,14 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
     bool ok = true;
     double f0;
     // the right-hand sides see t_start + t (CVODE's t0 = tInitial, src/sourceSystem.cpp:298)
-    ok = bdf_f<NR, INTERP>(m, lane, y, S, R, perturb, fn_mult(o, 0.0), igniter(o, S.qign, 0.0), L, f0) && ok;
+    ok = bdf_f<NR, INTERP, ORD>(m, lane, y, S, R, perturb, fn_mult(o, 0.0), igniter(o, S.qign, 0.0), L, f0) && ok;
     double qprev = S.qdot, qint = 0.0;
     double scale = S.atol + fabs(y) * rtol;
     const double d0 = rms(y, scale, active, n), d1 = rms(f0, scale, active, n);
     double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
     h0 = fmin(h0, tf);
     double f1;
-    ok = bdf_f<NR, INTERP>(m, lane, y + h0 * f0, S, R, perturb, fn_mult(o, h0), igniter(o, S.qign, h0), L,
+    ok = bdf_f<NR, INTERP, ORD>(m, lane, y + h0 * f0, S, R, perturb, fn_mult(o, h0), igniter(o, S.qign, h0), L,
                            f1) && ok;
     const double d2 = rms(f1 - f0, scale, active, n) / h0;
     const double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? fmax(1e-6, h0 * 1e-3)
@@ -383,7 +383,7 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
     D[1] = f0 * h_abs;
     bool lu_valid = false;
     LuFactors lu;
-    ok = fd_jacobian<NR, INTERP>(m, lane, n, y, f0, S, R, perturb, fn_mult(o, 0.0), igniter(o, S.qign, 0.0),
+    ok = fd_jacobian<NR, INTERP, ORD>(m, lane, n, y, f0, S, R, perturb, fn_mult(o, 0.0), igniter(o, S.qign, 0.0),
                                  L, J) && ok;
     if (!ok) return -2;
     double t = 0.0;
@@ -434,7 +434,7 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
                 int k;
                 for (k = 0; k < kNewtonMaxIter; k++) {
                     double fy;
-                    ok = bdf_f<NR, INTERP>(m, lane, ynew, S, R, perturb, fn_mult(o, t_new),
+                    ok = bdf_f<NR, INTERP, ORD>(m, lane, ynew, S, R, perturb, fn_mult(o, t_new),
                                            igniter(o, S.qign, t_new), L, fy) && ok;
                     const int bad = wave_or(active && !isfinite(fy) ? 1 : 0);
                     if (bad) break;
@@ -456,8 +456,8 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
                     if (current_jac) break;
                     double fp;
                     const double fa = fn_mult(o, t_new), qi = igniter(o, S.qign, t_new);
-                    ok = bdf_f<NR, INTERP>(m, lane, ypred, S, R, perturb, fa, qi, L, fp) && ok;
-                    ok = fd_jacobian<NR, INTERP>(m, lane, n, ypred, fp, S, R, perturb, fa, qi, L, J) && ok;
+                    ok = bdf_f<NR, INTERP, ORD>(m, lane, ypred, S, R, perturb, fa, qi, L, fp) && ok;
+                    ok = fd_jacobian<NR, INTERP, ORD>(m, lane, n, ypred, fp, S, R, perturb, fa, qi, L, J) && ok;
                     lu_valid = false;
                     current_jac = true;
                 }
@@ -494,7 +494,7 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
         y = ynew;
         if (qint_out) {
             double ftmp;
-            ok = bdf_f<NR, INTERP>(m, lane, y, S, R, perturb, fn_mult(o, t), igniter(o, S.qign, t), L, ftmp) && ok;
+            ok = bdf_f<NR, INTERP, ORD>(m, lane, y, S, R, perturb, fn_mult(o, t), igniter(o, S.qign, t), L, ftmp) && ok;
             S.nfe--;
             qint += 0.5 * hstep * (qprev + S.qdot);
             qprev = S.qdot;
@@ -540,7 +540,7 @@ __device__ __forceinline__ int bdf_integrate(const DevMech& m, const DevOpts& o,
     return 0;
 }
 
-template <int NR, bool INTERP>
+template <int NR, bool INTERP, bool ORD>
 __global__ void __launch_bounds__(256) bdf_stage_kernel(
     DevMech m, DevOpts o, const double* __restrict__ state_in, const double* __restrict__ split_const,
     const double* __restrict__ qloss, const double* __restrict__ perturb, double dt, int nPoints,
@@ -583,7 +583,7 @@ __global__ void __launch_bounds__(256) bdf_stage_kernel(
         int nsteps = 0;
         int err = kInvalidState;
         if (valid_input_state(lane, active, y0))
-            err = bdf_integrate<NR, INTERP>(m, o, lane, active, y, S, R, perturb, L, M, ru, J, dt, nsteps,
+            err = bdf_integrate<NR, INTERP, ORD>(m, o, lane, active, y, S, R, perturb, L, M, ru, J, dt, nsteps,
                                             qint_out ? qint_out + p : nullptr);
         if (active) state_out[(size_t)p * nv + lane] = (err >= 0) ? y : y0;
         if (lane == 0) {
@@ -613,22 +613,26 @@ int launch_bdf_stage(const DevMech& m, const DevOpts& o, const double* state_in,
                      int grid_blocks, int waves_per_block, double* jac_scratch, hipStream_t stream) {
     const size_t lds = bdf_lds_bytes(m, waves_per_block);
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, stream, queue);
+// ORD: the mechanism has reactions with explicit forward orders
+#define EMX_BDF_LAUNCH_O(NRV, INTERP, ORD)                                                       \
+    hipLaunchKernelGGL((bdf_stage_kernel<NRV, INTERP, ORD>), dim3(grid_blocks),                  \
+                       dim3(64 * waves_per_block), lds, stream, m, o, state_in, split_const,    \
+                       qloss, perturb, dt, nPoints, order, queue, state_out, status, stat1,     \
+                       stat2, cost, qint, jac_scratch)
 #define EMX_BDF_LAUNCH(NRV)                                                                     \
-    if (m.interp)                                                                               \
-        hipLaunchKernelGGL((bdf_stage_kernel<NRV, true>), dim3(grid_blocks),                    \
-                           dim3(64 * waves_per_block), lds, stream, m, o, state_in, split_const, \
-                           qloss, perturb, dt, nPoints, order, queue, state_out, status, stat1,  \
-                           stat2, cost, qint, jac_scratch);                                      \
-    else                                                                                        \
-        hipLaunchKernelGGL((bdf_stage_kernel<NRV, false>), dim3(grid_blocks),                   \
-                           dim3(64 * waves_per_block), lds, stream, m, o, state_in, split_const, \
-                           qloss, perturb, dt, nPoints, order, queue, state_out, status, stat1,  \
-                           stat2, cost, qint, jac_scratch);
+    if (m.has_orders) {                                                                         \
+        if (m.interp) EMX_BDF_LAUNCH_O(NRV, true, true);                                        \
+        else EMX_BDF_LAUNCH_O(NRV, false, true);                                                \
+    } else {                                                                                    \
+        if (m.interp) EMX_BDF_LAUNCH_O(NRV, true, false);                                       \
+        else EMX_BDF_LAUNCH_O(NRV, false, false);                                               \
+    }
     if (m.nRounds <= 1) { EMX_BDF_LAUNCH(1) }
     else if (m.nRounds <= 2) { EMX_BDF_LAUNCH(2) }
     else if (m.nRounds <= 4) { EMX_BDF_LAUNCH(4) }
     else if (m.nRounds <= 6) { EMX_BDF_LAUNCH(6) }
     else { EMX_BDF_LAUNCH(8) }
+#undef EMX_BDF_LAUNCH_O
 #undef EMX_BDF_LAUNCH
     return (int)hipGetLastError();
 }

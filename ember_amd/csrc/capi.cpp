@@ -223,6 +223,8 @@ int upload_mech(emx_context* c) {
     build_pressure_tables(m, L, c->P, aux, auxd);
     d.has_pdep = 0;
     for (const auto& r : m.rxn) d.has_pdep |= (r.type == 4 || r.type == 5);
+    d.has_orders = 0;
+    for (const auto& r : m.rxn) d.has_orders |= r.hasOrd;
     Packer pk;
     pk.add(m.W, &d.W);
     pk.add(L.rW, &d.rW);
@@ -243,6 +245,7 @@ int upload_mech(emx_context* c) {
     pk.add(L.reac, &d.reac);
     pk.add(L.prod, &d.prod);
     pk.add(L.info, &d.info);
+    pk.add(L.ord, &d.ord);
     pk.add(L.orig, &d.orig);
     pk.add(L.grp_off, &d.grp_off);
     pk.add(L.grp_sp, &d.grp_sp);
@@ -287,10 +290,11 @@ int check_ctx(emx_context* c) {
     return 0;
 }
 
-// PLOG, Chebyshev or SRI reactions (no adjoint, no plain baseline kernel for them)
+// PLOG, Chebyshev or SRI reactions, or explicit reaction orders (no adjoint, no plain
+// baseline kernel for them)
 bool has_kinds_beyond_troe(const HostMech& m) {
     for (const auto& r : m.rxn)
-        if (r.type >= 4) return true;
+        if (r.type >= 4 || r.hasOrd) return true;
     return false;
 }
 
@@ -558,7 +562,8 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
         if (c->opts.chemistry_integrator != EMX_QSS || c->opts.qss_stability_check || c->ratefn_n ||
             c->opts.ignition_energy != 0.0 || has_kinds_beyond_troe(c->hm))
             return fail(-1, "the plain baseline kernel implements QSS without stability check, "
-                            "rate multiplier function, igniter, PLOG, Chebyshev or SRI rates only");
+                            "rate multiplier function, igniter, PLOG, Chebyshev, SRI rates or "
+                            "explicit reaction orders only");
         HIP_TRY(c->plain_ws.ensure(plain_workspace_doubles(c->dm, (int)nPoints) * sizeof(double)));
         int rc = launch_qss_plain(c->dm, c->dopts, state_in, split_const, qloss, c->perturb, dt,
                                   (int)nPoints, state_out, status, stat1, stat2,
@@ -677,7 +682,7 @@ static int adjoint_check(emx_context* c, size_t nPoints, size_t nVars) {
                         "remove the rate multiplier function");
     if (c->opts.ignition_energy != 0.0) return fail(-1, "the adjoint does not support the igniter");
     if (has_kinds_beyond_troe(c->hm))
-        return fail(-1, "the adjoint does not support PLOG, Chebyshev or SRI reactions");
+        return fail(-1, "the adjoint does not support PLOG, Chebyshev, SRI reactions or explicit reaction orders");
     return 0;
 }
 

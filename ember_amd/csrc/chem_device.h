@@ -486,7 +486,21 @@ __device__ inline void third_body_conc(const DevMech& m, int lane, const GasStat
 // k_f = 0 and padding stoichiometry, and is written as exact zeros; a slot without third
 // body reads the unit cell Mg[kNoGroup]), so the loads of a round are in flight together.
 // Only the falloff rounds keep a branch around their pressure-dependent factor.
-template <int NR>
+// Cantera's C_AnyN::multiply for a reaction with explicit forward orders: the product of
+// c_j^o_j over the reactant positions with a nonzero order, zero when any of those
+// concentrations is <= 0.  c_j^o_j is formed as exp(sum_j o_j ln c_j).
+__device__ __forceinline__ double ordered_conc_product(double c0, double c1, double c2, double o0,
+                                                       double o1, double o2, const double* exp2t) {
+    const bool zero = (o0 != 0.0 && !(c0 > 0.0)) || (o1 != 0.0 && !(c1 > 0.0)) || (o2 != 0.0 && !(c2 > 0.0));
+    const double l = (o0 != 0.0 ? o0 * emx_log(fmax(c0, kSmallNumber)) : 0.0) +
+                     (o1 != 0.0 ? o1 * emx_log(fmax(c1, kSmallNumber)) : 0.0) +
+                     (o2 != 0.0 ? o2 * emx_log(fmax(c2, kSmallNumber)) : 0.0);
+    return zero ? 0.0 : emx_exp(l, exp2t);
+}
+
+// ORD: the mechanism has reactions with explicit forward orders (m.has_orders); the
+// instantiation without it is the plain mass-action kernel.
+template <int NR, bool ORD = false>
 __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasState& g,
                                          const RateRegs<NR>& R, const double* perturb, double fa,
                                          const WaveLds& L) {
@@ -495,6 +509,7 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
     int wd[NR];
     uint32_t rsw[NR], psw[NR];
     double ptw[NR];
+    double ow[ORD ? 3 * NR : 1];
 #pragma unroll
     for (int j = 0; j < NR; j++) {
         wd[j] = 0;
@@ -505,6 +520,11 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
             rsw[j] = m.reac[j * 64 + lane];
             psw[j] = m.prod[j * 64 + lane];
             ptw[j] = perturb[j * 64 + lane];
+        }
+        if (ORD) {
+            const int S = m.nRounds * 64;
+#pragma unroll
+            for (int q = 0; q < 3; q++) ow[ORD ? 3 * j + q : 0] = (j < m.nRounds) ? m.ord[q * S + j * 64 + lane] : 0.0;
         }
     }
 #pragma unroll
@@ -520,7 +540,10 @@ __device__ inline void rates_of_progress(const DevMech& m, int lane, const GasSt
             const double c0 = L.C[st_k(rs, 0)], c1 = L.C[st_k(rs, 1)], c2 = L.C[st_k(rs, 2)];
             const double d0 = L.C[st_k(ps, 0)], d1 = L.C[st_k(ps, 1)], d2 = L.C[st_k(ps, 2)];
             __builtin_amdgcn_sched_barrier(0);
-            const double cr = c0 * c1 * c2;
+            double cr = c0 * c1 * c2;
+            if (ORD && info_ord(w))
+                cr = ordered_conc_product(c0, c1, c2, ow[ORD ? 3 * j : 0], ow[ORD ? 3 * j + 1 : 0],
+                                          ow[ORD ? 3 * j + 2 : 0], m.exp2t);
             const double cp = d0 * d1 * d2;
             double kf = R.kf[j];
             if (j < RateRegs<NR>::NF && (kind == kLindemann || kind == kTroe || kind == kSri)) {

@@ -32,6 +32,9 @@ __host__ __device__ inline int info_kind(int32_t w) { return w & 7; }
 __host__ __device__ inline int info_rev(int32_t w) { return (w >> 3) & 1; }
 __host__ __device__ inline int info_group(int32_t w) { return (w >> 8) & 0xff; }   // 255 = none
 __host__ __device__ inline int info_hasT2(int32_t w) { return (w >> 4) & 1; }
+// explicit forward orders (Cantera C_AnyN): the concentration product is the ordered one
+constexpr int32_t kOrdBit = 32;
+__host__ __device__ inline int info_ord(int32_t w) { return (w >> 5) & 1; }
 
 // packed stoichiometry: byte0 = molecule count, bytes 1..3 = species indices
 // (unused index bytes hold kPadSpecies)
@@ -114,6 +117,10 @@ struct DevMech {
     const int32_t* aux;
     const double* auxd;
     int has_pdep;  // 1 if the mechanism has PLOG or Chebyshev reactions
+    // explicit forward orders: ord[j * nRounds*64 + s] = order of reactant position j of
+    // slot s (slots flagged kOrdBit; 0 elsewhere)
+    const double* ord;
+    int has_orders;
     // transport fits (polynomials in ln T, 5 coefficients each): sqrt(mu_k)/T^(1/4) and
     // lambda_k/sqrt(T) per species, D_jk(T, 1 Pa)/T^(3/2) per pair j <= k (row by row);
     // NULL without transport data

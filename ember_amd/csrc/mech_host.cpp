@@ -73,6 +73,15 @@ std::string read_table(const std::string& path, HostMech& m) {
                 if (fscanf(f, "%lf", &v[j]) != 1) err = "bad PLOG data";
             for (int j = 1; j < n && err.empty(); j++)
                 if (v[4 * j] < v[4 * (j - 1)]) err = "PLOG pressures must not decrease";
+        } else if (!strcmp(tag, "ORD")) {
+            // explicit forward orders of the preceding reaction (irreversible)
+            HostRxn* r = m.rxn.empty() ? nullptr : &m.rxn.back();
+            if (!r || r->rev || fscanf(f, "%lf %lf %lf", &r->ord[0], &r->ord[1], &r->ord[2]) != 3 ||
+                r->ord[0] < 0 || r->ord[1] < 0 || r->ord[2] < 0) {
+                err = "bad ORD record";
+                break;
+            }
+            r->hasOrd = 1;
         } else if (!strcmp(tag, "SRI")) {
             HostRxn* r = m.rxn.empty() ? nullptr : &m.rxn.back();
             if (!r || r->type != 6 ||
@@ -155,6 +164,7 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
     L.A0.assign(S, 0); L.b0.assign(S, 0); L.EaR0.assign(S, 0);
     L.troeA.assign(S, 0); L.rT3.assign(S, 0); L.rT1.assign(S, 0); L.T2.assign(S, 0);
     L.reac.assign(S, 0); L.prod.assign(S, 0); L.info.assign(S, kNone);
+    L.ord.assign(3 * (size_t)S, 0.0);
     // third-body efficiency groups
     std::map<std::pair<double, std::vector<std::pair<int, double>>>, int> groups;
     std::vector<std::pair<double, std::vector<std::pair<int, double>>>> glist;
@@ -186,7 +196,9 @@ std::string build_layout(const HostMech& m, DevLayout& L) {
         }
         int kind = r.type == 3 ? kTroe : r.type == 2 ? kLindemann : r.type == 1 ? kThreeBody
                  : r.type == 4 ? kPlog : r.type == 5 ? kCheb : r.type == 6 ? kSri : kElementary;
-        L.info[s] = kind | (r.rev ? 8 : 0) | (r.hasT2 ? 16 : 0) | (group << 8);
+        L.info[s] = kind | (r.rev ? 8 : 0) | (r.hasT2 ? 16 : 0) | (r.hasOrd ? kOrdBit : 0) | (group << 8);
+        if (r.hasOrd)
+            for (int j = 0; j < 3; j++) L.ord[(size_t)j * S + s] = j < r.nr ? r.ord[j] : 0.0;
     }
     if ((int)glist.size() > 63) return "more than 63 distinct third-body efficiency sets";
     L.nGroups = (int)glist.size();

@@ -26,6 +26,9 @@ struct HostRxn {
     std::vector<double> cheb;
     // type 6 (SRI falloff): a, b, c, d, e
     double sri[5] = {0, 0, 0, 1, 0};
+    // explicit forward orders of reactant positions 0..2 (ORD record)
+    int hasOrd = 0;
+    double ord[3] = {0, 0, 0};
 };
 
 struct HostMech {
@@ -60,6 +63,7 @@ struct DevLayout {
     std::vector<double> A, b, EaR, dn, A0, b0, EaR0, troeA, rT3, rT1, T2;
     std::vector<uint32_t> reac, prod;
     std::vector<int32_t> info;
+    std::vector<double> ord;  // [3][S]: orders of reactant positions (kOrdBit slots)
     std::vector<int32_t> grp_off, grp_sp;
     std::vector<double> grp_eff, grp_def;
     std::vector<uint32_t> entry;     // nChunkRounds * 64 * kChunkLen

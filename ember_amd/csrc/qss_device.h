@@ -64,7 +64,8 @@ __device__ __forceinline__ void alternate_priority(int& prio) {
 }
 
 // EXT: the call carries the time-dependent external terms (rate multiplier factor fa,
-// igniter heat release qi); without them fa = 1 and qi = 0 compile away.
+// igniter heat release qi) and the explicit-order concentration products; without them
+// fa = 1 and qi = 0 compile away and every product is the plain mass-action one.
 template <int NR, bool INTERP, bool EXT = false>
 __device__ inline bool qss_odefun(const DevMech& m, int lane, double y, bool corrector,
                                   QssSystem& S, RateRegs<NR>& R, const double* perturb,
@@ -85,7 +86,7 @@ __device__ inline bool qss_odefun(const DevMech& m, int lane, double y, bool cor
         S.rrho = emx_div(1.0, S.rho);
         S.rrc = emx_div(1.0, S.rho * S.cp);
     }
-    rates_of_progress<NR>(m, lane, g, R, perturb, fa, L);
+    rates_of_progress<NR, EXT>(m, lane, g, R, perturb, fa, L);
     double wQ, wD;
     species_rates(m, lane, L, wQ, wD);
     S.wQ = wQ;

@@ -18,7 +18,8 @@ namespace {
 
 // OCC = waves per SIMD the register allocation targets: 2 (up to 8 waves per workgroup,
 // no spills) or 3 (9..12 waves per workgroup, register budget 168).  EXT: Ember's rate
-// multiplier function (o.ratefn) or the igniter (o.qign) is on.
+// multiplier function (o.ratefn) or the igniter (o.qign) is on, or the mechanism has
+// reactions with explicit forward orders.
 template <int NR, bool INTERP, int OCC, bool EXT>
 __global__ void __launch_bounds__(OCC == 2 ? 512 : 768, OCC) qss_stage_kernel(
     DevMech mg, DevOpts o, const double* __restrict__ state_in,
@@ -94,7 +95,7 @@ __global__ void reset_queue_kernel(unsigned int* q) {
 }
 
 // Rates probe: creation / destruction / net production rates at given states
-template <int NR, bool INTERP>
+template <int NR, bool INTERP, bool ORD>
 __global__ void __launch_bounds__(256) rates_kernel(DevMech mg, const double* __restrict__ T,
                                                     const double* __restrict__ Y, int nonorm,
                                                     const double* __restrict__ perturb_g, int n,
@@ -119,7 +120,7 @@ __global__ void __launch_bounds__(256) rates_kernel(DevMech mg, const double* __
     GasState g = set_state(m, lane, y, rW, T[p], nonorm != 0, L);
     double hRT, cpR;
     bool ok = update_T<NR, INTERP>(m, lane, g, R, L, hRT, cpR);
-    rates_of_progress<NR>(m, lane, g, R, perturb, 1.0, L);
+    rates_of_progress<NR, ORD>(m, lane, g, R, perturb, 1.0, L);
     double wQ, wD;
     species_rates(m, lane, L, wQ, wD);
     if (is_species_lane(lane, m)) {
@@ -135,7 +136,7 @@ __global__ void __launch_bounds__(256) rates_kernel(DevMech mg, const double* __
 // function's a(t_now) when one is set, setStateMass, getEnthalpies, getReactionRates,
 // qdot = -sum_k wdot_k h_k [W/m^3].  One wavefront per point over Ember's
 // point-contiguous state (U, T, Y).
-template <int NR, bool INTERP>
+template <int NR, bool INTERP, bool ORD>
 __global__ void __launch_bounds__(256) heat_release_kernel(DevMech mg, const double* __restrict__ state,
                                                            const double* __restrict__ perturb_g, int n,
                                                            DevOpts o, double* qdot, int* status) {
@@ -160,7 +161,7 @@ __global__ void __launch_bounds__(256) heat_release_kernel(DevMech mg, const dou
     GasState g = set_state(m, lane, y, rW, T, false, L);
     double hRT, cpR;
     const bool ok = update_T<NR, INTERP>(m, lane, g, R, L, hRT, cpR);
-    rates_of_progress<NR>(m, lane, g, R, perturb, fn_mult(o, 0.0), L);
+    rates_of_progress<NR, ORD>(m, lane, g, R, perturb, fn_mult(o, 0.0), L);
     double wQ, wD;
     species_rates(m, lane, L, wQ, wD);
     const double hk = sp ? hRT * kGasConstant * T : 0.0;  // J/kmol
@@ -259,7 +260,7 @@ int launch_qss_stage(const DevMech& m, const DevOpts& o, const double* state_in,
                        dim3(64 * waves_per_block), lds, stream, m, o, state_in, split_const, qloss,   \
                        perturb, dt, nPoints, order, queue, state_out, status, stat1, stat2, cost, qint)
     const bool occ3 = waves_per_block > 8;
-    const bool fn = o.ratefn != nullptr || o.qign != nullptr;
+    const bool fn = o.ratefn != nullptr || o.qign != nullptr || m.has_orders;
     if (fn && occ3) return (int)hipErrorInvalidConfiguration;  // external terms: up to 8 waves
     EMX_DISPATCH_NR(m.nRounds, {
         if (m.interp) {
@@ -282,14 +283,19 @@ int launch_rates(const DevMech& m, const double* T, const double* Y, int nonorm,
     const int wpb = 4;
     size_t lds = ((size_t)mech_lds_doubles(m) + (size_t)wpb * wave_lds_doubles(m.nRounds, m.nChunkRounds)) * sizeof(double);
     int blocks = (n + wpb - 1) / wpb;
+#define EMX_RATES_LAUNCH(INTERP, ORD)                                                              \
+    hipLaunchKernelGGL((rates_kernel<NR, INTERP, ORD>), dim3(blocks), dim3(64 * wpb), lds, stream, m, T, \
+                       Y, nonorm, perturb, n, wQ, wD, wnet, status)
     EMX_DISPATCH_NR(m.nRounds, {
-        if (m.interp)
-            hipLaunchKernelGGL((rates_kernel<NR, true>), dim3(blocks), dim3(64 * wpb), lds, stream, m, T,
-                               Y, nonorm, perturb, n, wQ, wD, wnet, status);
-        else
-            hipLaunchKernelGGL((rates_kernel<NR, false>), dim3(blocks), dim3(64 * wpb), lds, stream, m, T,
-                               Y, nonorm, perturb, n, wQ, wD, wnet, status);
+        if (m.has_orders) {
+            if (m.interp) EMX_RATES_LAUNCH(true, true);
+            else EMX_RATES_LAUNCH(false, true);
+        } else {
+            if (m.interp) EMX_RATES_LAUNCH(true, false);
+            else EMX_RATES_LAUNCH(false, false);
+        }
     });
+#undef EMX_RATES_LAUNCH
     return (int)hipGetLastError();
 }
 
@@ -298,14 +304,19 @@ int launch_heat_release(const DevMech& m, const DevOpts& o, const double* state,
     const int wpb = 4;
     const size_t lds = qss_lds_bytes(m, wpb);
     const int blocks = (int)(((long)n + wpb - 1) / wpb);
+#define EMX_QDOT_LAUNCH(INTERP, ORD)                                                               \
+    hipLaunchKernelGGL((heat_release_kernel<NR, INTERP, ORD>), dim3(blocks), dim3(64 * wpb), lds, stream, \
+                       m, state, perturb, n, o, qdot, status)
     EMX_DISPATCH_NR(m.nRounds, {
-        if (m.interp)
-            hipLaunchKernelGGL((heat_release_kernel<NR, true>), dim3(blocks), dim3(64 * wpb), lds,
-                               stream, m, state, perturb, n, o, qdot, status);
-        else
-            hipLaunchKernelGGL((heat_release_kernel<NR, false>), dim3(blocks), dim3(64 * wpb), lds,
-                               stream, m, state, perturb, n, o, qdot, status);
+        if (m.has_orders) {
+            if (m.interp) EMX_QDOT_LAUNCH(true, true);
+            else EMX_QDOT_LAUNCH(false, true);
+        } else {
+            if (m.interp) EMX_QDOT_LAUNCH(true, false);
+            else EMX_QDOT_LAUNCH(false, false);
+        }
     });
+#undef EMX_QDOT_LAUNCH
     return (int)hipGetLastError();
 }
 

@@ -70,6 +70,10 @@ class Reaction:
     cheb: Optional[np.ndarray] = None
     # SRI falloff: (a, b, c, d, e)
     sri: Tuple[float, float, float, float, float] = (0.0, 0.0, 0.0, 1.0, 0.0)
+    # explicit forward orders (Cantera 'orders') per reactant position: the first position
+    # of each reactant species carries that species' order, repeated positions 0; None when
+    # every order equals the stoichiometric coefficient
+    orders: Optional[List[float]] = None
 
 
 @dataclass
@@ -162,6 +166,10 @@ class Mechanism:
             for k, e in eff:
                 line += f" {k} {e!r}"
             out.append(line)
+            if r.orders is not None:
+                # ORD: forward reaction orders of reactant positions 0..2 (Cantera C_AnyN)
+                oo = list(r.orders) + [0.0] * (3 - len(r.orders))
+                out.append("ORD " + " ".join(repr(float(v)) for v in oo))
             if r.rtype == TYPE_PLOG:
                 out.append(f"PLOG {len(r.plog)} " +
                            " ".join(repr(float(v)) for lvl in r.plog for v in lvl))
@@ -275,12 +283,47 @@ _COMMON_KEYS = {"equation", "type", "duplicate", "note", "id"}
 _TB_KEYS = {"efficiencies", "default-efficiency"}
 # the fields each Cantera reaction type may carry here; anything else is refused
 _KNOWN_REACTION_KEYS = {
-    "elementary": _COMMON_KEYS | _TB_KEYS | {"rate-constant"},
-    "three-body": _COMMON_KEYS | _TB_KEYS | {"rate-constant"},
-    "falloff": _COMMON_KEYS | _TB_KEYS | {"high-P-rate-constant", "low-P-rate-constant", "Troe", "SRI"},
-    "pressure-dependent-Arrhenius": _COMMON_KEYS | {"rate-constants"},
-    "Chebyshev": _COMMON_KEYS | {"temperature-range", "pressure-range", "data"},
+    "elementary": _COMMON_KEYS | _TB_KEYS | {"rate-constant", "orders"},
+    "three-body": _COMMON_KEYS | _TB_KEYS | {"rate-constant", "orders"},
+    "falloff": _COMMON_KEYS | _TB_KEYS | {"high-P-rate-constant", "low-P-rate-constant", "Troe", "SRI",
+                                          "orders"},
+    "pressure-dependent-Arrhenius": _COMMON_KEYS | {"rate-constants", "orders"},
+    "Chebyshev": _COMMON_KEYS | {"temperature-range", "pressure-range", "data", "orders"},
 }
+
+
+def _reaction_orders(rd: dict, eq: str, rev: bool, R: List[int], names: List[str]) -> Tuple[Optional[List[float]], float]:
+    """Cantera's explicit forward orders ('orders'): the per-position orders of the
+    reactant list R (None when every order equals the stoichiometric coefficient, as
+    Cantera then keeps the plain concentration product) and the reaction's total forward
+    order, which sets the units of its rate constant (Reaction::rate_units).  Refused as
+    Cantera refuses them: on reversible reactions, for species that are not reactants
+    (nonreactant-orders) and negative orders (negative-orders)."""
+    spec = rd.get("orders")
+    stoich: Dict[int, int] = {}
+    for k in R:
+        stoich[k] = stoich.get(k, 0) + 1
+    if not spec:
+        return None, float(len(R))
+    if rev:
+        raise ValueError(f"reaction {eq!r}: reaction orders may only be given for irreversible reactions")
+    eff = {k: float(n) for k, n in stoich.items()}
+    for sp, o in spec.items():
+        if sp not in names or names.index(sp) not in stoich:
+            raise ValueError(f"reaction {eq!r}: order for {sp!r}, which is not a reactant (nonreactant-orders)")
+        o = float(o)
+        if o < 0:
+            raise ValueError(f"reaction {eq!r}: negative order for {sp!r} (negative-orders)")
+        eff[names.index(sp)] = o
+    total = float(sum(eff.values()))
+    if all(eff[k] == stoich[k] for k in stoich):
+        return None, total
+    pos: List[float] = []
+    seen = set()
+    for k in R:
+        pos.append(0.0 if k in seen else eff[k])
+        seen.add(k)
+    return pos, total
 
 
 def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
@@ -340,7 +383,8 @@ def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
         R, mR, fR = _parse_side(lhs, names)
         P, mP, fP = _parse_side(rhs, names)
         rtype_s = rd.get("type", "elementary")
-        order = len(R)
+        # the forward order sets the rate constant's units: the explicit orders where given
+        rorders, order = _reaction_orders(rd, eq, rev, R, names)
         if rtype_s == "three-body" or mR:
             rt = TYPE_THREE_BODY
             k = rd["rate-constant"]
@@ -395,6 +439,7 @@ def load_yaml(path: str, phase: Optional[str] = None) -> Mechanism:
                 if sp in names:
                     r.efficiencies[names.index(sp)] = float(e)
         r.duplicate = bool(rd.get("duplicate", False))
+        r.orders = rorders
         rxns.append(r)
     name = os.path.splitext(os.path.basename(path))[0]
     transport = None

@@ -61,6 +61,10 @@ struct Rxn {
     std::vector<double> cheb;
     // type 6, Cantera SriRate falloff: a, b, c, d, e
     double sri[5] = {0, 0, 0, 1, 0};
+    // explicit forward orders of reactant positions 0..2 (irreversible reactions; the
+    // first position of a species carries its order, repeated positions 0)
+    int hasOrd = 0;
+    double ord[3] = {0, 0, 0};
 };
 
 struct Mech {
@@ -184,6 +188,10 @@ Mech* load_mech(const char* path) {
             ok = !m->rxn.empty() && m->rxn.back().type == 4 && fscanf(f, "%d", &n) == 1 && n >= 1;
             if (ok) m->rxn.back().plog.resize(4 * (size_t)n);
             for (int j = 0; j < 4 * n && ok; j++) ok = fscanf(f, "%lf", &m->rxn.back().plog[j]) == 1;
+        } else if (!strcmp(tag, "ORD")) {
+            Rxn* r = m->rxn.empty() ? nullptr : &m->rxn.back();
+            ok = r && !r->rev && fscanf(f, "%lf %lf %lf", &r->ord[0], &r->ord[1], &r->ord[2]) == 3;
+            if (ok) r->hasOrd = 1;
         } else if (!strcmp(tag, "SRI")) {
             Rxn* r = m->rxn.empty() ? nullptr : &m->rxn.back();
             ok = r && r->type == 6 &&
@@ -517,7 +525,17 @@ struct Gas {
             if (m.rxn[i].type == 1) ropf[i] *= concm[m.tb_slot[i]];
         for (int i = 0; i < m.I; i++) ropr[i] = ropf[i];
         for (const auto& s : m.reac) {
-            if (s.n == 1) ropf[s.rxn] *= conc[s.k[0]];
+            const Rxn& rx = m.rxn[s.rxn];
+            if (rx.hasOrd) {
+                // Cantera C_AnyN::multiply (StoichManager.h): each reactant with a nonzero
+                // order multiplies by c^order; a concentration <= 0 zeroes the rate
+                for (int j = 0; j < rx.nr; j++) {
+                    if (rx.ord[j] == 0.0) continue;
+                    const double c = conc[rx.r[j]];
+                    if (c > 0.0) ropf[s.rxn] *= std::pow(c, rx.ord[j]);
+                    else ropf[s.rxn] = 0.0;
+                }
+            } else if (s.n == 1) ropf[s.rxn] *= conc[s.k[0]];
             else if (s.n == 2) ropf[s.rxn] *= conc[s.k[0]] * conc[s.k[1]];
             else ropf[s.rxn] *= conc[s.k[0]] * conc[s.k[1]] * conc[s.k[2]];
         }
@@ -1733,7 +1751,7 @@ int orc_qss_adjoint(void* mech, const orc_options* o, int nPoints, const double*
     if (!m.ratefn.empty()) return -8;  // the ln-multiplier parameters need per-reaction multipliers
     if (o->ignition_energy != 0.0) return -9;
     for (const Rxn& r : m.rxn)
-        if (r.type >= 4) return -10;  // PLOG / Chebyshev / SRI: no adjoint
+        if (r.type >= 4 || r.hasOrd) return -10;  // PLOG / Chebyshev / SRI / orders: no adjoint
     const int N = m.K + 2;
     prepare_interp(m, o);
 #ifdef _OPENMP
@@ -1789,7 +1807,7 @@ int orc_qss_fd_gradient(void* mech, const orc_options* o, int nPoints, const dou
     if (!m.ratefn.empty()) return -8;  // the ln-multiplier parameters need per-reaction multipliers
     if (o->ignition_energy != 0.0) return -9;
     for (const Rxn& r : m.rxn)
-        if (r.type >= 4) return -10;  // PLOG / Chebyshev / SRI: no adjoint
+        if (r.type >= 4 || r.hasOrd) return -10;  // PLOG / Chebyshev / SRI / orders: no adjoint
     const int N = m.K + 2;
     prepare_interp(m, o);
 #ifdef _OPENMP

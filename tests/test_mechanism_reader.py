@@ -1,8 +1,9 @@
 """The YAML mechanism reader refuses what it cannot compile.
 
-A reaction field the compiler does not implement (Cantera's explicit 'orders',
-'negative-A', sticking or Blowers-Masel rates) must raise instead of compiling the
-reaction with mass-action orders or a different rate.
+A reaction field the compiler does not implement ('negative-A', sticking or Blowers-Masel
+rates, orders for non-reactant species or negative orders) must raise instead of
+compiling the reaction with mass-action orders or a different rate.  Explicit forward
+orders of reactants on irreversible reactions are compiled (tests/test_oracle_orders.py).
 """
 import os
 
@@ -23,11 +24,15 @@ def _with_reaction(tmp_path, extra):
     return str(p)
 
 
-def test_orders_refused(tmp_path):
+def test_orders_compiled_or_refused(tmp_path):
     p = _with_reaction(tmp_path, {"equation": "H2 + O2 => 2 OH",
                                   "rate-constant": {"A": 1.0e13, "b": 0.0, "Ea": 0.0},
                                   "orders": {"H2": 0.5, "O2": 1.0}})
-    with pytest.raises(ValueError, match="orders"):
+    assert mechanism.load_yaml(p).reactions[-1].orders == [0.5, 1.0]
+    p = _with_reaction(tmp_path, {"equation": "H2 + O2 => 2 OH", "nonreactant-orders": True,
+                                  "rate-constant": {"A": 1.0e13, "b": 0.0, "Ea": 0.0},
+                                  "orders": {"H2": 0.5, "H2O": 1.0}})
+    with pytest.raises(ValueError, match="nonreactant-orders"):
         mechanism.load_yaml(p)
 
 
