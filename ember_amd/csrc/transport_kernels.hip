@@ -3,7 +3,7 @@
 // the mixture viscosity, thermal conductivity and mass-weighted mixture diffusion
 // coefficients rho D_k of every point of a point-major state.
 //
-// One wavefront per point, lane k = species k.  The pure-species viscosities and
+// One wavefront per point at a time, lane k = species k.  The pure-species viscosities and
 // conductivities and the binary diffusion coefficients come from the mechanism table's
 // temperature fits, evaluated as Cantera's MixTransport evaluates them (update_T: sqrt(T),
 // T^(1/4) and [1, ln T, ..., ln^4 T]; the products summed left to right as dot5):
@@ -48,88 +48,116 @@ __device__ __forceinline__ double phi_direct(double sqk, double sqj, double Wk, 
     return f * f / (sqrt(8.0) * wratkj1);
 }
 
-constexpr int kWavesPerBlock = 4;
+// A workgroup of kWavesPerBlock waves takes kWavesPerBlock * kPointsPerWave consecutive
+// points; the species' fits and the pair fits (5 K(K+1)/2 doubles: 57 KB for GRI) are
+// staged in LDS once per workgroup, so the pair terms read LDS instead of gathering their
+// coefficients from L2 for every point (112 KB per GRI point).
+constexpr int kWavesPerBlock = 8;
+constexpr int kPointsPerWave = 16;
+
+__host__ __device__ inline int transport_lds_doubles(int K) {
+    return 5 * (K * (K + 1) / 2) + 10 * K + K + 3 * 64 * kWavesPerBlock;
+}
 
 __global__ void __launch_bounds__(64 * kWavesPerBlock) transport_props_kernel(
     DevMech m, const double* __restrict__ state, int n, double threshold, double* __restrict__ mu_out,
     double* __restrict__ lambda_out, int ml_stride, double* __restrict__ rhoD_out, int rd_stride) {
-    __shared__ double sX[kWavesPerBlock][64], sSq[kWavesPerBlock][64], sVisc[kWavesPerBlock][64];
+    extern __shared__ double lds[];
+    const int K = m.K, nv = m.nVars;
+    const int npair = K * (K + 1) / 2;
+    double* tdiff = lds;                // D_jk fits, pair slot order (pair_slot)
+    double* tvisc = tdiff + 5 * npair;  // sqrt(mu_k) / T^(1/4) fits
+    double* tcond = tvisc + 5 * K;      // lambda_k / sqrt(T) fits
+    double* tW = tcond + 5 * K;         // molecular weights
+    for (int i = threadIdx.x; i < 5 * npair; i += blockDim.x) tdiff[i] = m.tr_diff[i];
+    for (int i = threadIdx.x; i < 5 * K; i += blockDim.x) {
+        tvisc[i] = m.tr_visc[i];
+        tcond[i] = m.tr_cond[i];
+    }
+    for (int i = threadIdx.x; i < K; i += blockDim.x) tW[i] = m.W[i];
+    __syncthreads();
     const int lane = lane_id();
     const int wave = wave_in_block();
-    const int p = blockIdx.x * kWavesPerBlock + wave;
-    if (p >= n) return;  // whole wavefronts leave together
-    const int K = m.K, nv = m.nVars;
+    double* sX = tW + K + wave * 192;  // per wave: X_k, sqrt(mu_k), mu_k of the current point
+    double* sSq = sX + 64;
+    double* sVisc = sSq + 64;
     const bool sp = lane < K;
-    const double* s = state + (size_t)p * nv;
-    const double T = s[1];
-    const double y = sp ? s[2 + lane] : 0.0;
-    const double Wk = sp ? m.W[lane] : 1.0;
-    // Phase::setMassFractions: clip negatives, normalise; mean molecular weight
-    const double yc = fmax(y, 0.0);
-    const double norm = wave_sum(yc);
-    const double ym = sp ? (yc * (1.0 / norm)) * (1.0 / Wk) : 0.0;
-    const double mmw = 1.0 / wave_sum(ym);
-    const double dens = m.P * mmw / (T * kGasConstant);
-    const double X = sp ? fmax(1e-20, ym * mmw) : 0.0;
-    const bool major = sp && X >= threshold;
-    // MixTransport::update_T
-    const double logt = log(T), sqrt_t = sqrt(T), t14 = sqrt(sqrt_t);
-    const double pt[5] = {1.0, logt, logt * logt, logt * logt * logt, logt * logt * logt * logt};
-    double sqv = 0.0, visc = 0.0, cond = 1.0;
-    if (sp) {
-        sqv = t14 * dot5(pt, m.tr_visc + 5 * lane);
-        visc = sqv * sqv;
-        cond = sqrt_t * dot5(pt, m.tr_cond + 5 * lane);
-    }
-    // MixTransport::thermalConductivity
-    const double lam = 0.5 * (wave_sum(sp ? X * cond : 0.0) + 1.0 / wave_sum(sp ? X / cond : 0.0));
-    sX[wave][lane] = X;  // species lanes; 0 elsewhere
-    sSq[wave][lane] = sqv;
-    sVisc[wave][lane] = visc;
-    wave_lds_sync();
-    // ApproxMixTransport::viscosity over the major species
-    double contrib = 0.0;
-    if (major) {
-        double sum = 0.0;
-        for (int j = 0; j < K; j++) {
-            const double Xj = sX[wave][j];
-            if (!(Xj >= threshold)) continue;
-            const double Wj = m.W[j];
-            double phi;
-            if (j <= lane) {
-                phi = phi_direct(sqv, sSq[wave][j], Wk, Wj);
-            } else {
-                // the pair's Phi_jk, then Phi_kj = Phi_jk / (vratiokj wratiojk) with
-                // vratiokj = mu_j / mu_k and wratiojk = W_k / W_j (j > k)
-                const double vratio = sVisc[wave][j] / visc;
-                const double wratio = Wk / Wj;
-                phi = phi_direct(sSq[wave][j], sqv, Wj, Wk) / (vratio * wratio);
+    const double Wk = sp ? tW[lane] : 1.0;
+    for (int it = 0; it < kPointsPerWave; it++) {
+        const int p = (blockIdx.x * kPointsPerWave + it) * kWavesPerBlock + wave;
+        if (p >= n) break;  // uniform over the wave
+        wave_lds_sync();  // the previous point's reads of sX, sSq, sVisc are done
+        const double* s = state + (size_t)p * nv;
+        const double T = s[1];
+        const double y = sp ? s[2 + lane] : 0.0;
+        // Phase::setMassFractions: clip negatives, normalise; mean molecular weight
+        const double yc = fmax(y, 0.0);
+        const double norm = wave_sum(yc);
+        const double ym = sp ? (yc * (1.0 / norm)) * (1.0 / Wk) : 0.0;
+        const double mmw = 1.0 / wave_sum(ym);
+        const double dens = m.P * mmw / (T * kGasConstant);
+        const double X = sp ? fmax(1e-20, ym * mmw) : 0.0;
+        const bool major = sp && X >= threshold;
+        // MixTransport::update_T
+        const double logt = log(T), sqrt_t = sqrt(T), t14 = sqrt(sqrt_t);
+        const double pt[5] = {1.0, logt, logt * logt, logt * logt * logt, logt * logt * logt * logt};
+        double sqv = 0.0, visc = 0.0, cond = 1.0;
+        if (sp) {
+            sqv = t14 * dot5(pt, tvisc + 5 * lane);
+            visc = sqv * sqv;
+            cond = sqrt_t * dot5(pt, tcond + 5 * lane);
+        }
+        // MixTransport::thermalConductivity
+        const double lam = 0.5 * (wave_sum(sp ? X * cond : 0.0) + 1.0 / wave_sum(sp ? X / cond : 0.0));
+        sX[lane] = X;  // species lanes; 0 elsewhere
+        sSq[lane] = sqv;
+        sVisc[lane] = visc;
+        wave_lds_sync();
+        // ApproxMixTransport::viscosity over the major species
+        double contrib = 0.0;
+        if (major) {
+            double sum = 0.0;
+            for (int j = 0; j < K; j++) {
+                const double Xj = sX[j];
+                if (!(Xj >= threshold)) continue;
+                const double Wj = tW[j];
+                double phi;
+                if (j <= lane) {
+                    phi = phi_direct(sqv, sSq[j], Wk, Wj);
+                } else {
+                    // the pair's Phi_jk, then Phi_kj = Phi_jk / (vratiokj wratiojk) with
+                    // vratiokj = mu_j / mu_k and wratiojk = W_k / W_j (j > k)
+                    const double vratio = sVisc[j] / visc;
+                    const double wratio = Wk / Wj;
+                    phi = phi_direct(sSq[j], sqv, Wj, Wk) / (vratio * wratio);
+                }
+                sum += Xj * phi;
             }
-            sum += Xj * phi;
+            contrib = X * visc / sum;
         }
-        contrib = X * visc / sum;
-    }
-    const double mu = wave_sum(contrib);
-    // ApproxMixTransport::getMixDiffCoeffsMass (pairs with a major member, as
-    // ApproxMixTransport::updateDiff_T fills them), times rho
-    if (sp) {
-        const double tts = T * sqrt_t;
-        double s1 = 0.0, s2 = 0.0;
-        for (int i = 0; i < K; i++) {
-            if (i == lane) continue;
-            const double Xi = sX[wave][i];
-            if (!major && !(Xi >= threshold)) continue;
-            const double b = tts * dot5(pt, m.tr_diff + 5 * pair_slot(lane, i, K));
-            s1 += Xi / b;
-            s2 += Xi * m.W[i] / b;
+        const double mu = wave_sum(contrib);
+        // ApproxMixTransport::getMixDiffCoeffsMass (pairs with a major member, as
+        // ApproxMixTransport::updateDiff_T fills them), times rho; X_i / D and
+        // X_i W_i / D share one reciprocal of D
+        if (sp) {
+            const double tts = T * sqrt_t;
+            double s1 = 0.0, s2 = 0.0;
+            for (int i = 0; i < K; i++) {
+                if (i == lane) continue;
+                const double Xi = sX[i];
+                if (!major && !(Xi >= threshold)) continue;
+                const double rb = 1.0 / (tts * dot5(pt, tdiff + 5 * pair_slot(lane, i, K)));
+                s1 += Xi * rb;
+                s2 += Xi * tW[i] * rb;
+            }
+            s1 *= m.P;
+            s2 *= m.P * X / (mmw - Wk * X);
+            rhoD_out[(size_t)p * rd_stride + lane] = dens * (1.0 / (s1 + s2));
         }
-        s1 *= m.P;
-        s2 *= m.P * X / (mmw - Wk * X);
-        rhoD_out[(size_t)p * rd_stride + lane] = dens * (1.0 / (s1 + s2));
-    }
-    if (lane == 0) {
-        mu_out[(size_t)p * ml_stride] = mu;
-        lambda_out[(size_t)p * ml_stride] = lam;
+        if (lane == 0) {
+            mu_out[(size_t)p * ml_stride] = mu;
+            lambda_out[(size_t)p * ml_stride] = lam;
+        }
     }
 }
 
@@ -137,8 +165,10 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) transport_props_kernel(
 
 int launch_transport_props(const DevMech& m, const double* state, int n, double threshold, double* mu,
                            double* lambda, int ml_stride, double* rhoD, int rd_stride, hipStream_t stream) {
-    const int blocks = (n + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL(transport_props_kernel, dim3(blocks), dim3(64 * kWavesPerBlock), 0, stream, m, state,
+    const int per_block = kWavesPerBlock * kPointsPerWave;
+    const int blocks = (int)(((long)n + per_block - 1) / per_block);
+    const size_t lds = (size_t)transport_lds_doubles(m.K) * sizeof(double);
+    hipLaunchKernelGGL(transport_props_kernel, dim3(blocks), dim3(64 * kWavesPerBlock), lds, stream, m, state,
                        n, threshold, mu, lambda, ml_stride, rhoD, rd_stride);
     return (int)hipGetLastError();
 }
