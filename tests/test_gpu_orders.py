@@ -23,6 +23,51 @@ def _states(n, seed):
     return W.flame_front_states(n, seed=seed, name="h2o2")
 
 
+# GRI-Mech 3.0 with three made-up global reactions with explicit orders appended: the
+# six-round ordered kernels (the H2/O2 mechanism above has one round)
+_GRI_EXTRA = """
+- equation: CH4 + 2 O2 => CO2 + 2 H2O
+  rate-constant: {A: 1.1e+10, b: 0.0, Ea: 4.8e+04}
+  orders: {CH4: 0.7, O2: 0.8}
+- equation: CO + OH => CO2 + H
+  rate-constant: {A: 2.0e+10, b: 0.0, Ea: 3.0e+04}
+  orders: {CO: 1.2}
+- equation: 2 H2 + O2 => 2 H2O
+  rate-constant: {A: 3.0e+11, b: 0.0, Ea: 4.0e+04}
+  orders: {H2: 1.0, O2: 0.5}
+"""
+
+
+@pytest.fixture(scope="module")
+def gri_orders(tmp_path_factory):
+    src = open(mechanism.builtin_yaml("gri30")).read().rstrip("\n") + "\n" + _GRI_EXTRA.lstrip("\n")
+    p = tmp_path_factory.mktemp("gri_orders") / "gri30_orders.yaml"
+    p.write_text(src)
+    return str(p)
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_orders_gri_stage(gpu, oracle_mod, gri_orders, integrator):
+    print(f"[orders] gri30 + ordered reactions, integrator={integrator}", flush=True)
+    m = mechanism.load(gri_orders)
+    assert sum(r.orders is not None for r in m.reactions) == 3
+    st = W.flame_front_states(32, seed=5, name="gri30")
+    ctx = capi.Context(gri_orders, chemistry_integrator=integrator, integrator_species_abs_tol=1e-12)
+    wQ, wD, wN, rs = ctx.rates(st[:, 1], st[:, 2:])
+    om = oracle_mod.Mech(mechanism.table_path(gri_orders, out_dir=os.path.dirname(gri_orders)))
+    r = om.rates(st[:, 1], st[:, 2:], oracle_mod.options())
+    scale = np.abs(r["wdotQ"]).max(axis=1, keepdims=True)
+    assert (rs == 0).all()
+    assert (np.abs(wN - r["wdot"]) <= 1e-10 * np.abs(r["wdot"]) + 1e-13 * scale).all()
+    out, status, *_ = ctx.source_integrate(st, 2e-5)
+    ref, rstat, *_ = om.integrate(st, 2e-5, oracle_mod.options(integrator=integrator, atol_species=1e-12))
+    assert (status == 0).all() and (rstat == 0).all()
+    assert (np.abs(out[:, 1] - ref[:, 1]) <= 1e-6 * ref[:, 1]).all()
+    floor = 5e-11 if integrator == 1 else 1e-12  # the BDF's one-ulp sensitivity on GRI states
+    dY = np.abs(out[:, 2:] - ref[:, 2:])
+    assert (dY <= 1e-6 * np.abs(ref[:, 2:]) + floor).all(), dY.max()
+
+
 @pytest.mark.parametrize("kinetics", [0, 1])
 @pytest.mark.parametrize("nonorm", [False, True])
 def test_orders_rates(gpu, oracle_mod, kinetics, nonorm):
