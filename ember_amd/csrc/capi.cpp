@@ -77,6 +77,8 @@ struct emx_context {
     double* perturb = nullptr;  // device, slot order
     unsigned int* queue = nullptr;
     int waves_per_block = 4;
+    bool waves_set = false;  // waves_per_block chosen by the caller (else the default, which
+                             // the BDF lowers to what fits in LDS)
     int blocks_per_cu = 0;  // 0: as many as LDS allows
     DevBuf b_in, b_out, b_sc, b_ql, b_status, b_s1, b_s2;
     DevBuf d_B, d_Brow, d_D, d_sc, d_x, d_y, d_h, d_ns, d_work, d_hmin;
@@ -530,6 +532,7 @@ int emx_set_launch(emx_context* c, int waves_per_block, int blocks_per_cu) {
     if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu < 0)
         return fail(-1, "invalid launch shape");
     c->waves_per_block = waves_per_block ? waves_per_block : 4;
+    c->waves_set = waves_per_block != 0;
     c->blocks_per_cu = blocks_per_cu;
     return 0;
 }
@@ -584,13 +587,19 @@ int emx_source_integrate_device(emx_context* c, size_t nPoints, size_t nVars, co
     if (c->opts.chemistry_integrator == EMX_BDF) {
         if (c->variant == EMX_VARIANT_WAVE_PER_POINT)
             return fail(-1, "the per-point wavefront variant implements QSS only");
-        if (bdf_lds_bytes(c->dm, c->waves_per_block) > 160 * 1024)
+        // a wave's LU pair matrix grows with the state (32 KB at 64 variables): from about 55
+        // species on, four waves of a CU no longer fit in its 160 KB, and the default shape
+        // runs fewer waves per workgroup (one workgroup per CU)
+        int wpb = c->waves_per_block;
+        if (!c->waves_set)
+            while (wpb > 1 && bdf_lds_bytes(c->dm, wpb) > 160 * 1024) wpb--;
+        if (bdf_lds_bytes(c->dm, wpb) > 160 * 1024)
             return fail(-1, "BDF workgroup needs more than 160 KB of LDS; use fewer waves per block");
-        const int blocks = grid_blocks(c, (int)nPoints, true, c->waves_per_block);
-        HIP_TRY(c->jac.ensure((size_t)blocks * c->waves_per_block * 64 * 64 * sizeof(double)));
+        const int blocks = grid_blocks(c, (int)nPoints, true, wpb);
+        HIP_TRY(c->jac.ensure((size_t)blocks * wpb * 64 * 64 * sizeof(double)));
         int rc = launch_bdf_stage(c->dm, stage_opts(c, t_stage_start), state_in, split_const, qloss, stage_perturb(c), dt,
                                   (int)nPoints, order, c->queue, state_out, status, stat1, stat2,
-                                  c->cost.as<int>(), qint, blocks, c->waves_per_block,
+                                  c->cost.as<int>(), qint, blocks, wpb,
                                   c->jac.as<double>(), c->stream);
         if (rc) return fail(-10, std::string("bdf launch: ") + hipGetErrorString((hipError_t)rc));
         c->cost_n = nPoints;

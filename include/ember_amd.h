@@ -135,7 +135,9 @@ int emx_set_point_positions(emx_context* ctx, size_t nPoints, const double* x);
 int emx_set_stream(emx_context* ctx, void* stream);
 
 /* Launch shape for the chemistry kernels: waves per workgroup and workgroups per CU
- * (0 = built-in default).  The autotuner in ember_amd.autotune picks these. */
+ * (0 = built-in default: 4 waves per workgroup, which the BDF lowers to the most that fit in
+ * a CU's 160 KB of LDS; a shape set here that does not fit is an error).  The autotuner in
+ * ember_amd.autotune picks these. */
 int emx_set_launch(emx_context* ctx, int waves_per_block, int blocks_per_cu);
 
 /* Chemistry kernel variant for the *_integrate entry points.  EMX_VARIANT_WAVE (default):
