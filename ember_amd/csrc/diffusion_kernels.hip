@@ -387,23 +387,6 @@ constexpr int kBufferFlags = 0x00020000;
 // waves per workgroup of diff_step_kernel
 constexpr int kDiffWaves = 4;
 
-// Grid metrics of OneDimGrid (src/grid.cpp:43-69) as they enter DiffusionSystem::get_A
-// (src/diffusion_system.cpp:10-50): c1_j = B_j g1_j with g1_j = 0.5 / (dlj r_j), and
-// c2_j = (D_j + D_{j+1}) g2_j with g2_j = rphalf_j / hh_j.  They depend on the grid only,
-// so they are formed once per stage instead of once per variable and solve.
-__global__ void grid_metrics_kernel(const double* x, int N, int alpha, double* g1, double* g2) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= N) return;
-    double a = 0.0, b = 0.0;
-    if (j >= 1 && j <= N - 2) {
-        const double dlj = 0.5 * (x[j + 1] - x[j - 1]);
-        a = 0.5 / (dlj * (alpha ? x[j] : 1.0));
-    }
-    if (j <= N - 2) b = (alpha ? 0.5 * (x[j] + x[j + 1]) : 1.0) / (x[j + 1] - x[j]);
-    g1[j] = a;
-    g2[j] = b;
-}
-
 // 1 / b by Newton-Raphson from the hardware estimate (v_rcp_f64, relative error below
 // 2^-22): each step r <- r + r (1 - b r) squares the relative error, so two steps reach
 // the rounding level.  For the pivots of diagonally dominant rows (no zero, overflow or
@@ -461,55 +444,79 @@ struct PtBuf {
     }
 };
 
-// Matrix rows s .. s+m-1 of variable v, and the c coefficient of row s-1 (cp).  Chunks
-// away from both ends (all rows generic interior rows of DiffusionSystem::get_A) take the
-// grid metrics from g1/g2; the end chunks take the per-row path with the boundary
-// conditions.  FULL: m == P (every chunk but the last), so no row needs a guard and the
-// loads issue together.
-template <int P, bool FULL>
-__device__ __forceinline__ void chunk_matrix(const DiffRows& R, const double* __restrict__ g1,
-                                             const double* __restrict__ g2, int v, int s, int m,
-                                             double (&A)[P], double (&Bm)[P], double (&Cm)[P],
-                                             double* K, double& cp) {
-    const int N = R.N;
-    const unsigned v8 = (unsigned)v * sizeof(double);
-    const PtBuf Kb(R.sc, N, R.nV);
-#pragma unroll
-    for (int t = 0; t < P; t++)
-        if (FULL || t < m) K[t * 64] = Kb.ld(v8, s + t);
-    if (s >= 2 && s + m - 1 <= N - 3) {
-        const PtBuf Db(R.D, N, R.nV);
-        const double* Brow = R.B + (size_t)(R.B_row ? R.B_row[v] : v) * N;
-        double D0 = Db.ld(v8, s);
-        double c2m = (Db.ld(v8, s - 1) + D0) * g2[s - 1];
-        cp = Brow[s - 1] * g1[s - 1] * c2m;
-#pragma unroll
-        for (int t = 0; t < P; t++) {
-            if (FULL || t < m) {
-                const int j = s + t;
-                const double Dp = Db.ld(v8, j + 1);
-                const double c1 = Brow[j] * g1[j];
-                const double c2 = (D0 + Dp) * g2[j];
-                A[t] = c1 * c2m;
-                Bm[t] = -c1 * (c2m + c2);
-                Cm[t] = c1 * c2;
-                c2m = c2;
-                D0 = Dp;
-            }
-        }
-    } else {
-        double k;
-#pragma unroll
-        for (int t = 0; t < P; t++)
-            if (FULL || t < m) R.matrix(v, s + t, A[t], Bm[t], Cm[t], k);
-        if (s == 0 && R.left == 3 && v == R.wallVar)  // WallFlux constant of get_k
-            K[0] += R.Bv(v, 0) * (R.alpha + 1) / (R.x[1] - R.x[0]) * R.wallConst * R.yInf;
-        cp = 0.0;
-        if (s > 0) {
-            double ap, bp;
-            R.matrix(v, s - 1, ap, bp, cp, k);
-        }
+// Edge chunks: the chunks that are not all generic interior rows (chunk 0, and the one or
+// two chunks holding rows N-2 and N-1: chunk c is interior when s = cP >= 2 and
+// s + P - 1 <= N - 3, so only chunks 0, C-2 and C-1 can fail it).  Their matrix rows
+// (DiffusionSystem::get_A with the boundary conditions), right-hand-side constants (splitConst
+// plus the WallFlux term of get_k) and the c coefficient of the row before the chunk depend
+// on the grid, B, D and splitConst only, which a stage holds fixed, so they are formed once
+// per stage (stage_setup_kernel) into a table that the edge chunks of every solve read with
+// one batch of loads.  Table: slot e = 0, 1, 2 for chunks 0, C-2, C-1; part k = a, b, c, K
+// of row t at ((e * 4 + k) * P + t) * 64 + v; cp of slot e at 3 * 4 * P * 64 + e * 64 + v.
+__host__ __device__ __forceinline__ int edge_slot_chunk(int e, int C) {
+    return e == 0 ? 0 : (e == 1 ? C - 2 : C - 1);
+}
+__device__ __forceinline__ int edge_slot(int chunk, int C) {
+    return chunk == 0 ? 0 : (chunk == C - 1 ? 2 : 1);
+}
+template <int P>
+constexpr int edge_doubles() { return 3 * 4 * P * 64 + 3 * 64; }
+
+// One thread per (slot, row t in 0..P, variable): rows t < P of the chunk, and t = P for cp.
+template <int P>
+__device__ void edge_rows(const DiffRows& R, int idx, double* edge) {
+    const int e = idx / ((P + 1) * 64), t = (idx / 64) % (P + 1), v = idx % 64;
+    const int N = R.N, C = (N + P - 1) / P;
+    if (e >= 3 || v >= R.nV) return;
+    const int chunk = edge_slot_chunk(e, C);
+    if (chunk < 0 || (e == 1 && chunk == 0)) return;  // C = 2: slot 1 would repeat chunk 0
+    const int s = chunk * P, m = min(s + P, N) - s;
+    if (t == P) {
+        double a, b, c = 0.0, k;
+        if (s > 0) R.matrix(v, s - 1, a, b, c, k);
+        edge[3 * 4 * P * 64 + e * 64 + v] = c;
+        return;
     }
+    double a = 0.0, b = 0.0, c = 0.0, kk = 0.0;
+    if (t < m) {
+        const int j = s + t;
+        double k;
+        R.matrix(v, j, a, b, c, k);
+        kk = R.sc ? R.sc[(size_t)j * R.nV + v] : 0.0;
+        if (j == 0 && R.left == 3 && v == R.wallVar)  // WallFlux constant of get_k
+            kk += R.Bv(v, 0) * (R.alpha + 1) / (R.x[1] - R.x[0]) * R.wallConst * R.yInf;
+    }
+    double* E = edge + (size_t)e * 4 * P * 64 + t * 64 + v;
+    E[0] = a;
+    E[P * 64] = b;
+    E[2 * P * 64] = c;
+    E[3 * P * 64] = kk;
+}
+
+// Once per stage: the grid metrics of OneDimGrid (src/grid.cpp:43-69) as they enter
+// DiffusionSystem::get_A (src/diffusionSystem.cpp:10-50), c1_j = B_j g1_j with
+// g1_j = 0.5 / (dlj r_j) and c2_j = (D_j + D_{j+1}) g2_j with g2_j = rphalf_j / hh_j (they
+// depend on the grid only, so they are formed once per stage instead of once per variable
+// and solve), in blocks [0, nbm); the edge table in the blocks after.
+template <int P>
+__global__ void __launch_bounds__(256) stage_setup_kernel(DiffRows R, int nbm, double* g1, double* g2,
+                                                          double* edge) {
+    if ((int)blockIdx.x >= nbm) {
+        edge_rows<P>(R, ((int)blockIdx.x - nbm) * blockDim.x + threadIdx.x, edge);
+        return;
+    }
+    const double* x = R.x;
+    const int N = R.N, alpha = R.alpha;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    double a = 0.0, b = 0.0;
+    if (j >= 1 && j <= N - 2) {
+        const double dlj = 0.5 * (x[j + 1] - x[j - 1]);
+        a = 0.5 / (dlj * (alpha ? x[j] : 1.0));
+    }
+    if (j <= N - 2) b = (alpha ? 0.5 * (x[j] + x[j + 1]) : 1.0) / (x[j + 1] - x[j]);
+    g1[j] = a;
+    g2[j] = b;
 }
 
 // Downward pivots of the chunk interior rows 0 .. m-2 for the shift sh: T1[t] = 1 / bt_t
@@ -531,28 +538,46 @@ __device__ __forceinline__ void down_pivots(int m, double sh, const double (&A)[
     }
 }
 
+// Edge chunk (rows from the edge table; FULL: m == P, so no row needs a guard).
 template <int P, bool FULL>
-__device__ __forceinline__ void chunk_step(const DiffRows& R, const double* __restrict__ g1,
-                                           const double* __restrict__ g2, const int* nsteps,
-                                           int q_solve, int q_next, const double* X, double* own,
-                                           double* nxt, double* K, int v, int chunk, int C) {
+__device__ __forceinline__ void chunk_step(const DiffRows& R, const double* __restrict__ edge,
+                                           const int* nsteps, int q_solve, int q_next,
+                                           const double* X, double* own, double* nxt, double* K,
+                                           int v, int chunk, int C) {
     const int N = R.N, nV = R.nV;
     const int s = chunk * P;
     const int m = FULL ? P : min(s + P, N) - s;  // rows in the chunk; the last is the separator
     const unsigned v8 = (unsigned)v * sizeof(double);
     const PtBuf Yb(R.y, N, nV), Pb(R.yprev, N, nV);
-    double A[P], Bm[P], Cm[P], Y[P], T1[P], T2[P];
-    double cp;
-    chunk_matrix<P, FULL>(R, g1, g2, v, s, m, A, Bm, Cm, K, cp);
+    double A[P], Bm[P], Cm[P], Y[P], T1[P], T2[P], Kr[P];
+    // one batch of loads: the chunk's rows from the edge table, y and the separators
+    const double* E = edge + (size_t)edge_slot(chunk, C) * 4 * P * 64 + v;
+#pragma unroll
+    for (int t = 0; t < P; t++) {
+        if (FULL || t < m) {
+            A[t] = E[t * 64];
+            Bm[t] = E[(P + t) * 64];
+            Cm[t] = E[(2 * P + t) * 64];
+            Kr[t] = E[(3 * P + t) * 64];
+        }
+    }
+    const double cp = edge[3 * 4 * P * 64 + edge_slot(chunk, C) * 64 + v];
 #pragma unroll
     for (int t = 0; t < P; t++)
         if (FULL || t < m) Y[t] = Yb.ld(v8, s + t);
+    double xL = 0.0, xR = 0.0;
+    if (q_solve >= 0) {
+        const double* Xv = X + (size_t)v * C;
+        xL = chunk > 0 ? Xv[chunk - 1] : 0.0;
+        xR = Xv[chunk];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < P; t++)
+        if (FULL || t < m) K[t * 64] = Kr[t];
     const double hv = R.h[v];
     int pivots = -1;         // shift class (0: BDF1, 1: BDF2) of the pivots held in T1
     if (q_solve >= 0 && var_active(nsteps, v, q_solve)) {
-        const double* Xv = X + (size_t)v * C;
-        const double xL = chunk > 0 ? Xv[chunk - 1] : 0.0;
-        const double xR = Xv[chunk];
         if (q_solve >= 8) {
 #pragma unroll
             for (int t = 0; t < P; t++)
@@ -676,13 +701,176 @@ __device__ __forceinline__ void chunk_step(const DiffRows& R, const double* __re
     nxt[2 * pl + e] = n2;
 }
 
+// chunk_step for a full chunk away from both ends (rows s-1 .. s+P are generic interior
+// rows of DiffusionSystem::get_A), with the same arithmetic as chunk_step<P, true> and one
+// memory latency instead of four: every load of the chunk is issued in one batch before
+// any is used (the B row index first, since the B row loads depend on it; then the
+// splitConst, D and y rows, the separators and, for a BDF2 solve, y_(n-1)), and the
+// splitConst rows go to LDS only after the whole batch is in flight.  The downward pivots
+// live in LDS (T1[t * 64], written once per shift, read by four passes), which leaves the
+// registers to the batch.  A BDF2 solve leaves the old y in registers as the next BDF2
+// solve's y_(n-1) instead of reloading it.  BDF2 false: an instance for launches whose
+// solves are both BDF1 substeps (q_solve, q_next < 8), without the y_(n-1) registers.
+template <int P, bool BDF2>
+__device__ __forceinline__ void chunk_step_interior(const DiffRows& R, const double* __restrict__ g1,
+                                                    const double* __restrict__ g2, const int* nsteps,
+                                                    int q_solve, int q_next, const double* X,
+                                                    double* own, double* nxt, double* K, double* T1,
+                                                    int v, int chunk, int C) {
+    const int N = R.N, nV = R.nV;
+    const int s = chunk * P;
+    const unsigned v8 = (unsigned)v * sizeof(double);
+    const PtBuf Yb(R.y, N, nV), Pb(R.yprev, N, nV), Kb(R.sc, N, nV), Db(R.D, N, nV);
+    const bool s2 = BDF2 && q_solve >= 8, n2 = BDF2 && q_next >= 8;
+    // ---- the load batch
+    const int brow = R.B_row ? R.B_row[v] : v;
+    double Kr[P], Dr[P + 2], Y[P], T2[P], Bs[P + 1];
+#pragma unroll
+    for (int t = 0; t < P; t++) Kr[t] = Kb.ld(v8, s + t);
+#pragma unroll
+    for (int t = 0; t < P + 2; t++) Dr[t] = Db.ld(v8, s - 1 + t);
+#pragma unroll
+    for (int t = 0; t < P; t++) Y[t] = Yb.ld(v8, s + t);
+    double xL = 0.0, xR = 0.0;
+    if (q_solve >= 0) {
+        const double* Xv = X + (size_t)v * C;
+        xL = Xv[chunk - 1];  // chunk >= 1: s >= 2
+        xR = Xv[chunk];
+    }
+    if (s2) {
+#pragma unroll
+        for (int t = 0; t < P; t++) T2[t] = Pb.ld(v8, s + t);
+    }
+    const double* Brow = R.B + (size_t)brow * N;
+#pragma unroll
+    for (int t = 0; t < P + 1; t++) Bs[t] = Brow[s - 1 + t];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < P; t++) K[t * 64] = Kr[t];
+    // ---- matrix rows (the interior branch of chunk_matrix)
+    double A[P], Bm[P], Cm[P];
+    double D0 = Dr[1];
+    double c2m = (Dr[0] + D0) * g2[s - 1];
+    const double cp = Bs[0] * g1[s - 1] * c2m;
+#pragma unroll
+    for (int t = 0; t < P; t++) {
+        const int j = s + t;
+        const double Dp = Dr[t + 2];
+        const double c1 = Bs[t + 1] * g1[j];
+        const double c2 = (D0 + Dp) * g2[j];
+        A[t] = c1 * c2m;
+        Bm[t] = -c1 * (c2m + c2);
+        Cm[t] = c1 * c2;
+        c2m = c2;
+        D0 = Dp;
+    }
+    // downward pivots of down_pivots, into LDS
+    auto pivots_for = [&](double sh) {
+        double tp = 0.0;
+#pragma unroll
+        for (int t = 0; t < P - 1; t++) {
+            const double bs = Bm[t] - sh;
+            tp = t == 0 ? rcp_nr(bs) : rcp_nr(bs - A[t] * tp * Cm[t - 1]);
+            T1[t * 64] = tp;
+        }
+    };
+    const double hv = R.h[v];
+    int pivots = -1;  // shift class (0: BDF1, 1: BDF2) of the pivots held in T1
+    if (q_solve >= 0) {
+        if (!var_active(nsteps, v, q_solve)) return;  // variable finished its steps
+        const BdfCoef bc(q_solve, hv);
+        pivots_for(bc.sh);
+        pivots = q_solve >= 8;
+#pragma unroll
+        for (int t = 0; t < P - 1; t++) {
+            double r = bc.rhs(Y[t], s2 ? T2[t] : 0.0, K[t * 64]);
+            if (t == 0) r -= A[t] * xL;
+            if (t == P - 2) r -= Cm[t] * xR;
+            if (t == 0)
+                T2[t] = r;
+            else
+                T2[t] = r - A[t] * T1[(t - 1) * 64] * T2[t - 1];
+        }
+        const bool save_prev = q_solve == 0 || q_solve >= 8;
+        double xn = xR;
+#pragma unroll
+        for (int t = P - 1; t >= 0; t--) {
+            double xi = xR;
+            if (t < P - 1)
+                xi = (t == P - 2) ? T2[t] * T1[t * 64] : (T2[t] - Cm[t] * xn) * T1[t * 64];
+            if (save_prev) Pb.st(Y[t], v8, s + t);
+            if (s2) T2[t] = Y[t];  // y_(n-1) of the next BDF2 solve
+            Y[t] = xi;
+            Yb.st(xi, v8, s + t);
+            xn = xi;
+        }
+    }
+    if (q_next < 0 || !var_active(nsteps, v, q_next)) return;
+    if (n2 && q_solve < 8) {  // first BDF2 solve: y_(n-1) as stored by solve 0
+#pragma unroll
+        for (int t = 0; t < P; t++) T2[t] = Pb.ld(v8, s + t);
+    }
+    const BdfCoef bc(q_next, hv);
+    if (pivots != (q_next >= 8 ? 1 : 0)) pivots_for(bc.sh);
+    // ---- chunk summary for solve q_next (as in chunk_step with m = P)
+    double rt = 0.0, at = 0.0;
+#pragma unroll
+    for (int t = 0; t < P - 1; t++) {
+        const double r = bc.rhs(Y[t], n2 ? T2[t] : 0.0, K[t * 64]);
+        if (t == 0) {
+            rt = r;
+            at = A[t];
+        } else {
+            const double f = A[t] * T1[(t - 1) * 64];
+            rt = r - f * rt;
+            at = -f * at;
+        }
+    }
+    double bh = 1.0, rh = 0.0, ch = 0.0, aN = 0.0;
+#pragma unroll
+    for (int t = P - 2; t >= 0; t--) {
+        const double bs = Bm[t] - bc.sh;
+        const double r = bc.rhs(Y[t], n2 ? T2[t] : 0.0, K[t * 64]);
+        if (t == P - 2) {
+            bh = bs;
+            rh = r;
+            ch = Cm[t];
+        } else {
+            const double g = Cm[t] * rcp_nr(bh);
+            bh = bs - g * aN;
+            rh = r - g * rh;
+            ch = -g * ch;
+        }
+        aN = A[t];
+    }
+    const double be = Bm[P - 1] - bc.sh;
+    const double re = bc.rhs(Y[P - 1], n2 ? T2[P - 1] : 0.0, K[(P - 1) * 64]);
+    const double ae = A[P - 1];
+    const double ib = T1[(P - 2) * 64];
+    const double o0 = -ae * at * ib;
+    const double o1 = be - ae * Cm[P - 2] * ib;
+    const double o2 = re - ae * rt * ib;
+    const double ibh = rcp_nr(bh);
+    const double n0 = -cp * aN * ibh;
+    const double n1 = -cp * ch * ibh;
+    const double n2v = -cp * rh * ibh;
+    const size_t pl = (size_t)nV * C, e = (size_t)v * C + chunk;
+    own[e] = o0;
+    own[pl + e] = o1;
+    own[2 * pl + e] = o2;
+    nxt[e] = n0;
+    nxt[pl + e] = n1;
+    nxt[2 * pl + e] = n2v;
+}
+
 // One wave per chunk of P rows [s, e] (e = separator), lane = variable.
 //  q_solve >= 0: solve q_solve's interior rows with the separators X and write y
 //                (saving y to yprev first when q_solve starts a BDF step);
 //  q_next >= 0:  eliminate the chunk for solve q_next and write its summary (own, nxt).
-template <int P>
+template <int P, bool BDF2>
 __global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const double* __restrict__ g1,
                                                         const double* __restrict__ g2,
+                                                        const double* __restrict__ edge,
                                                         const int* nsteps, int q_solve, int q_next,
                                                         const double* X, double* own, double* nxt) {
     const int v = lane_id();
@@ -699,10 +887,15 @@ __global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const dou
     // splitConst rows of the chunk, lane-interleaved in LDS (K[t * 64] = row t of this lane)
     __shared__ double kl[kDiffWaves * P * 64];
     double* K = kl + (threadIdx.x / 64) * P * 64 + v;
-    if ((chunk + 1) * P <= N)
-        chunk_step<P, true>(R, g1, g2, nsteps, q_solve, q_next, X, own, nxt, K, v, chunk, C);
+    if (chunk * P >= 2 && (chunk + 1) * P - 1 <= N - 3) {
+        // downward pivots of the interior path, lane-interleaved like K
+        __shared__ double tl[kDiffWaves * P * 64];
+        double* T1 = tl + (threadIdx.x / 64) * P * 64 + v;
+        chunk_step_interior<P, BDF2>(R, g1, g2, nsteps, q_solve, q_next, X, own, nxt, K, T1, v, chunk, C);
+    } else if ((chunk + 1) * P <= N)
+        chunk_step<P, true>(R, edge, nsteps, q_solve, q_next, X, own, nxt, K, v, chunk, C);
     else
-        chunk_step<P, false>(R, g1, g2, nsteps, q_solve, q_next, X, own, nxt, K, v, chunk, C);
+        chunk_step<P, false>(R, edge, nsteps, q_solve, q_next, X, own, nxt, K, v, chunk, C);
 }
 
 // Separators of one variable per workgroup of kPcrThreads threads, from the reduced
@@ -960,9 +1153,10 @@ int run_solve(const Rows& R, StateOut out, int n, int nV, const int* nsteps, int
 size_t diffusion_work_doubles(int nV, int N) {
     const int C1 = (N + P1 - 1) / P1;
     const int C2 = (C1 + P2 - 1) / P2;
-    // own1, nxt1 (3 each), X1, own2, nxt2, X2, cp, dp, hmin chunk mins, yprev; grid metrics
+    // own1, nxt1 (3 each), X1, own2, nxt2, X2, cp, dp, hmin chunk mins, yprev; grid metrics;
+    // edge table
     return (size_t)nV * (6 * C1 + C1 + 6 * C2 + C2 + 2 * (size_t)std::max(C1, C2) + C1 + N) +
-           2 * (size_t)N;
+           2 * (size_t)N + edge_doubles<P1>();
 }
 
 void diffusion_carve(double* base, int nV, int N, DiffWork& w) {
@@ -982,6 +1176,7 @@ void diffusion_carve(double* base, int nV, int N, DiffWork& w) {
     w.yprev = p; p += (size_t)N * nV;
     w.g1 = p; p += N;
     w.g2 = p; p += N;
+    w.edge = p; p += edge_doubles<P1>();
 }
 
 namespace {
@@ -1038,15 +1233,23 @@ int launch_diffusion_stage(const DiffArgs& a, double* y, const double* h, const 
     const int wpb = kDiffWaves;
     const dim3 grid((C1 + wpb - 1) / wpb), block(64 * wpb);
     const size_t pcr_lds = pcr_lds_doubles(C1) * sizeof(double);
-    hipLaunchKernelGGL(grid_metrics_kernel, dim3((a.N + 255) / 256), dim3(256), 0, st, a.x, a.N,
-                       a.alpha, w.g1, w.g2);
-    hipLaunchKernelGGL((diff_step_kernel<P1>), grid, block, 0, st, R, w.g1, w.g2, nsteps, -1, 0, w.X1,
-                       w.own1, w.nxt1);
+    const int nbm = (a.N + 255) / 256;
+    hipLaunchKernelGGL((stage_setup_kernel<P1>), dim3(nbm + (3 * (P1 + 1) * 64 + 255) / 256), dim3(256), 0,
+                       st, R, nbm, w.g1, w.g2, w.edge);
+    // the BDF1 instance for launches whose solves are both BDF1 substeps
+    auto step = [&](int qs, int qn) {
+        if (qs >= 8 || qn >= 8)
+            hipLaunchKernelGGL((diff_step_kernel<P1, true>), grid, block, 0, st, R, w.g1, w.g2, w.edge,
+                               nsteps, qs, qn, w.X1, w.own1, w.nxt1);
+        else
+            hipLaunchKernelGGL((diff_step_kernel<P1, false>), grid, block, 0, st, R, w.g1, w.g2, w.edge,
+                               nsteps, qs, qn, w.X1, w.own1, w.nxt1);
+    };
+    step(-1, 0);
     for (int q = 0; q < nsolve; q++) {
         hipLaunchKernelGGL(pcr_kernel, dim3(a.nV), dim3(kPcrThreads), pcr_lds, st, w.own1, w.nxt1, C1, a.nV,
                            nsteps, q, w.X1);
-        hipLaunchKernelGGL((diff_step_kernel<P1>), grid, block, 0, st, R, w.g1, w.g2, nsteps, q,
-                           q + 1 < nsolve ? q + 1 : -1, w.X1, w.own1, w.nxt1);
+        step(q, q + 1 < nsolve ? q + 1 : -1);
     }
     return (int)hipGetLastError();
 }

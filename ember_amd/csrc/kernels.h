@@ -96,7 +96,7 @@ struct DiffArgs {
 };
 
 struct DiffWork {
-    double *own1, *nxt1, *X1, *own2, *nxt2, *X2, *cp, *dp, *cmin, *yprev, *g1, *g2;
+    double *own1, *nxt1, *X1, *own2, *nxt2, *X2, *cp, *dp, *cmin, *yprev, *g1, *g2, *edge;
 };
 
 struct DiffSteps {
