@@ -952,10 +952,8 @@ static int diffusion_run(emx_context* c, const DiffArgs& a, const double* h_host
     }
     HIP_TRY(c->d_h.ensure(nV * sizeof(double)));
     HIP_TRY(c->d_ns.ensure(nV * sizeof(int)));
-    int e = launch_set_steps(steps, nV, c->d_h.as<double>(), c->d_ns.as<int>(), st);
-    if (e) return fail(-10, "step-size launch failed");
     const int nsolve = 8 + (maxs - 1);
-    e = launch_diffusion_stage(a, y_dev, c->d_h.as<double>(), c->d_ns.as<int>(), nsolve, w, st);
+    int e = launch_diffusion_stage(a, y_dev, steps, c->d_h.as<double>(), c->d_ns.as<int>(), nsolve, w, st);
     if (e) return fail(-10, std::string("diffusion launch: ") + hipGetErrorString((hipError_t)e));
     return 0;
 }

@@ -500,7 +500,14 @@ __device__ void edge_rows(const DiffRows& R, int idx, double* edge) {
 // and solve), in blocks [0, nbm); the edge table in the blocks after.
 template <int P>
 __global__ void __launch_bounds__(256) stage_setup_kernel(DiffRows R, int nbm, double* g1, double* g2,
-                                                          double* edge) {
+                                                          double* edge, DiffSteps steps, double* h,
+                                                          int* nsteps) {
+    // the per-variable step sizes and step counts, delivered as a kernel argument so that the
+    // values are fixed at launch (no host buffer has to outlive the call)
+    if (blockIdx.x == 0 && (int)threadIdx.x < R.nV) {
+        h[threadIdx.x] = steps.h[threadIdx.x];
+        nsteps[threadIdx.x] = steps.ns[threadIdx.x];
+    }
     if ((int)blockIdx.x >= nbm) {
         edge_rows<P>(R, ((int)blockIdx.x - nbm) * blockDim.x + threadIdx.x, edge);
         return;
@@ -538,12 +545,47 @@ __device__ __forceinline__ void down_pivots(int m, double sh, const double (&A)[
     }
 }
 
+// Two-level partition of the fused stage path.  The kDiffWaves chunks of a workgroup
+// (chunks 4g .. 4g + kc - 1, kc <= 4) form one workgroup chunk whose separator is the last
+// chunk's; the separators u_0 .. u_(kc-2) of its other chunks are eliminated inside the
+// workgroup (the chunk summaries meet in LDS), so the reduced system that the separator
+// kernel solves has one row per workgroup (G = ceil(C / 4) rows).  For the back-substitution
+// the workgroup keeps, per interior separator j, the downward-sweep row
+// bt_j u_j + at_j u_(-1) + c_j u_(j+1) = rt_j as (rt_j, at_j, c_j, 1 / bt_j) in coef at
+// ((g * 3 + j) * 4 + i) * 64 + v; u_(-1) and u_(kc-1) are the workgroup separators X2[g-1]
+// (0 for g = 0) and X2[g].
+constexpr int kWgCoef = 3 * 4 * 64;  // coef doubles per workgroup
+struct WgSep {
+    double UL, UR, cf[3][4];
+    __device__ __forceinline__ void load(const double* X2, const double* coef, int v, int G, int g,
+                                         int kc) {
+        const double* Xv = X2 + (size_t)v * G;
+        UL = g > 0 ? Xv[g - 1] : 0.0;
+        UR = Xv[g];
+        const double* cv = coef + (size_t)g * kWgCoef + v;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) cf[j][i] = j < kc - 1 ? cv[(j * 4 + i) * 64] : 0.0;
+    }
+    // the separators left and right of chunk 4g + w
+    __device__ __forceinline__ void get(int w, int kc, double& xL, double& xR) const {
+        double u[4];
+        u[3] = UR;
+#pragma unroll
+        for (int j = 2; j >= 0; j--)
+            u[j] = j < kc - 1 ? (cf[j][0] - cf[j][1] * UL - cf[j][2] * u[j + 1]) * cf[j][3] : UR;
+        xR = w == 0 ? u[0] : (w == 1 ? u[1] : (w == 2 ? u[2] : u[3]));
+        xL = w == 0 ? UL : (w == 1 ? u[0] : (w == 2 ? u[1] : u[2]));
+    }
+};
+
 // Edge chunk (rows from the edge table; FULL: m == P, so no row needs a guard).
 template <int P, bool FULL>
-__device__ __forceinline__ void chunk_step(const DiffRows& R, const double* __restrict__ edge,
+__device__ __forceinline__ bool chunk_step(const DiffRows& R, const double* __restrict__ edge,
                                            const int* nsteps, int q_solve, int q_next,
-                                           const double* X, double* own, double* nxt, double* K,
-                                           int v, int chunk, int C) {
+                                           const double* X2, const double* coef, int G, int kc,
+                                           double* K, int v, int chunk, int C, double (&sm)[6]) {
     const int N = R.N, nV = R.nV;
     const int s = chunk * P;
     const int m = FULL ? P : min(s + P, N) - s;  // rows in the chunk; the last is the separator
@@ -565,16 +607,14 @@ __device__ __forceinline__ void chunk_step(const DiffRows& R, const double* __re
 #pragma unroll
     for (int t = 0; t < P; t++)
         if (FULL || t < m) Y[t] = Yb.ld(v8, s + t);
-    double xL = 0.0, xR = 0.0;
-    if (q_solve >= 0) {
-        const double* Xv = X + (size_t)v * C;
-        xL = chunk > 0 ? Xv[chunk - 1] : 0.0;
-        xR = Xv[chunk];
-    }
+    WgSep ws;
+    if (q_solve >= 0) ws.load(X2, coef, v, G, chunk / 4, kc);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < P; t++)
         if (FULL || t < m) K[t * 64] = Kr[t];
+    double xL = 0.0, xR = 0.0;
+    if (q_solve >= 0) ws.get(chunk % 4, kc, xL, xR);
     const double hv = R.h[v];
     int pivots = -1;         // shift class (0: BDF1, 1: BDF2) of the pivots held in T1
     if (q_solve >= 0 && var_active(nsteps, v, q_solve)) {
@@ -614,9 +654,9 @@ __device__ __forceinline__ void chunk_step(const DiffRows& R, const double* __re
             }
         }
     } else if (q_solve >= 0) {
-        return;  // variable finished its steps: nothing further for it in this stage
+        return false;  // variable finished its steps: nothing further for it in this stage
     }
-    if (q_next < 0 || !var_active(nsteps, v, q_next)) return;
+    if (q_next < 0 || !var_active(nsteps, v, q_next)) return false;
     if (q_next >= 8) {  // y_(n-1) of the BDF2 step, as stored above or by an earlier solve
 #pragma unroll
         for (int t = 0; t < P; t++)
@@ -692,13 +732,13 @@ __device__ __forceinline__ void chunk_step(const DiffRows& R, const double* __re
             n2 = -cp * rh * ib;
         }
     }
-    const size_t pl = (size_t)nV * C, e = (size_t)v * C + chunk;
-    own[e] = o0;
-    own[pl + e] = o1;
-    own[2 * pl + e] = o2;
-    nxt[e] = n0;
-    nxt[pl + e] = n1;
-    nxt[2 * pl + e] = n2;
+    sm[0] = o0;
+    sm[1] = o1;
+    sm[2] = o2;
+    sm[3] = n0;
+    sm[4] = n1;
+    sm[5] = n2;
+    return true;
 }
 
 // chunk_step for a full chunk away from both ends (rows s-1 .. s+P are generic interior
@@ -712,11 +752,11 @@ __device__ __forceinline__ void chunk_step(const DiffRows& R, const double* __re
 // solve's y_(n-1) instead of reloading it.  BDF2 false: an instance for launches whose
 // solves are both BDF1 substeps (q_solve, q_next < 8), without the y_(n-1) registers.
 template <int P, bool BDF2>
-__device__ __forceinline__ void chunk_step_interior(const DiffRows& R, const double* __restrict__ g1,
+__device__ __forceinline__ bool chunk_step_interior(const DiffRows& R, const double* __restrict__ g1,
                                                     const double* __restrict__ g2, const int* nsteps,
-                                                    int q_solve, int q_next, const double* X,
-                                                    double* own, double* nxt, double* K, double* T1,
-                                                    int v, int chunk, int C) {
+                                                    int q_solve, int q_next, const double* X2,
+                                                    const double* coef, int G, int kc, double* K,
+                                                    double* T1, int v, int chunk, double (&sm)[6]) {
     const int N = R.N, nV = R.nV;
     const int s = chunk * P;
     const unsigned v8 = (unsigned)v * sizeof(double);
@@ -731,12 +771,8 @@ __device__ __forceinline__ void chunk_step_interior(const DiffRows& R, const dou
     for (int t = 0; t < P + 2; t++) Dr[t] = Db.ld(v8, s - 1 + t);
 #pragma unroll
     for (int t = 0; t < P; t++) Y[t] = Yb.ld(v8, s + t);
-    double xL = 0.0, xR = 0.0;
-    if (q_solve >= 0) {
-        const double* Xv = X + (size_t)v * C;
-        xL = Xv[chunk - 1];  // chunk >= 1: s >= 2
-        xR = Xv[chunk];
-    }
+    WgSep ws;
+    if (q_solve >= 0) ws.load(X2, coef, v, G, chunk / 4, kc);
     if (s2) {
 #pragma unroll
         for (int t = 0; t < P; t++) T2[t] = Pb.ld(v8, s + t);
@@ -747,6 +783,8 @@ __device__ __forceinline__ void chunk_step_interior(const DiffRows& R, const dou
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < P; t++) K[t * 64] = Kr[t];
+    double xL = 0.0, xR = 0.0;
+    if (q_solve >= 0) ws.get(chunk % 4, kc, xL, xR);
     // ---- matrix rows (the interior branch of chunk_matrix)
     double A[P], Bm[P], Cm[P];
     double D0 = Dr[1];
@@ -777,7 +815,7 @@ __device__ __forceinline__ void chunk_step_interior(const DiffRows& R, const dou
     const double hv = R.h[v];
     int pivots = -1;  // shift class (0: BDF1, 1: BDF2) of the pivots held in T1
     if (q_solve >= 0) {
-        if (!var_active(nsteps, v, q_solve)) return;  // variable finished its steps
+        if (!var_active(nsteps, v, q_solve)) return false;  // variable finished its steps
         const BdfCoef bc(q_solve, hv);
         pivots_for(bc.sh);
         pivots = q_solve >= 8;
@@ -805,7 +843,7 @@ __device__ __forceinline__ void chunk_step_interior(const DiffRows& R, const dou
             xn = xi;
         }
     }
-    if (q_next < 0 || !var_active(nsteps, v, q_next)) return;
+    if (q_next < 0 || !var_active(nsteps, v, q_next)) return false;
     if (n2 && q_solve < 8) {  // first BDF2 solve: y_(n-1) as stored by solve 0
 #pragma unroll
         for (int t = 0; t < P; t++) T2[t] = Pb.ld(v8, s + t);
@@ -854,69 +892,167 @@ __device__ __forceinline__ void chunk_step_interior(const DiffRows& R, const dou
     const double n0 = -cp * aN * ibh;
     const double n1 = -cp * ch * ibh;
     const double n2v = -cp * rh * ibh;
-    const size_t pl = (size_t)nV * C, e = (size_t)v * C + chunk;
-    own[e] = o0;
-    own[pl + e] = o1;
-    own[2 * pl + e] = o2;
-    nxt[e] = n0;
-    nxt[pl + e] = n1;
-    nxt[2 * pl + e] = n2v;
+    sm[0] = o0;
+    sm[1] = o1;
+    sm[2] = o2;
+    sm[3] = n0;
+    sm[4] = n1;
+    sm[5] = n2v;
+    return true;
 }
 
-// One wave per chunk of P rows [s, e] (e = separator), lane = variable.
-//  q_solve >= 0: solve q_solve's interior rows with the separators X and write y
-//                (saving y to yprev first when q_solve starts a BDF step);
-//  q_next >= 0:  eliminate the chunk for solve q_next and write its summary (own, nxt).
+// One wave per chunk of P rows [s, e] (e = separator), lane = variable; the kDiffWaves
+// chunks of a workgroup form one workgroup chunk (two-level partition, WgSep).
+//  q_solve >= 0: solve q_solve's interior rows with the separators from X2 and coef and
+//                write y (saving y to yprev first when q_solve starts a BDF step);
+//  q_next >= 0:  eliminate the chunks for solve q_next; the chunk summaries meet in LDS and
+//                wave 0 eliminates the workgroup's interior separators, writing the
+//                workgroup summary (own2, nxt2: plane k, variable v, workgroup g at
+//                (k * nV + v) * G + g) and the back-substitution rows (coef).
+// Every wave of a workgroup reaches the barrier when q_next >= 0 (q_next is uniform over
+// the grid): waves past the last chunk and lanes past nV only skip the work.
 template <int P, bool BDF2>
 __global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const double* __restrict__ g1,
                                                         const double* __restrict__ g2,
                                                         const double* __restrict__ edge,
                                                         const int* nsteps, int q_solve, int q_next,
-                                                        const double* X, double* own, double* nxt) {
+                                                        const double* X2, double* coef, double* own2,
+                                                        double* nxt2) {
     const int v = lane_id();
     // XCD-aware block order: workgroups are dispatched round-robin over the 8 XCDs, so
-    // block b runs on XCD b % 8; give the blocks of one XCD consecutive chunk groups, so
-    // that the chunks whose summaries share a cache line (consecutive chunks of a plane)
-    // are written through one L2 and leave it as whole lines
+    // block b runs on XCD b % 8; give the blocks of one XCD consecutive workgroup chunks, so
+    // that the summaries that share a cache line (consecutive workgroups of a plane) are
+    // written through one L2 and leave it as whole lines
     const int nb = gridDim.x;
-    const int lb = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
-    const int chunk = __builtin_amdgcn_readfirstlane(lb * (blockDim.x / 64) + threadIdx.x / 64);
-    const int N = R.N;
+    const int g = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
+    const int w = threadIdx.x / 64;
+    const int chunk = __builtin_amdgcn_readfirstlane(g * kDiffWaves + w);
+    const int N = R.N, nV = R.nV;
     const int C = (N + P - 1) / P;
-    if (chunk >= C || v >= R.nV) return;
-    // splitConst rows of the chunk, lane-interleaved in LDS (K[t * 64] = row t of this lane)
+    const int G = (C + kDiffWaves - 1) / kDiffWaves;
+    const int kc = min(kDiffWaves, C - g * kDiffWaves);  // chunks of this workgroup
+    // splitConst rows of the chunk, lane-interleaved in LDS (K[t * 64] = row t of this lane);
+    // after the chunk step the region holds the chunk's summary for the workgroup elimination
     __shared__ double kl[kDiffWaves * P * 64];
-    double* K = kl + (threadIdx.x / 64) * P * 64 + v;
-    if (chunk * P >= 2 && (chunk + 1) * P - 1 <= N - 3) {
-        // downward pivots of the interior path, lane-interleaved like K
-        __shared__ double tl[kDiffWaves * P * 64];
-        double* T1 = tl + (threadIdx.x / 64) * P * 64 + v;
-        chunk_step_interior<P, BDF2>(R, g1, g2, nsteps, q_solve, q_next, X, own, nxt, K, T1, v, chunk, C);
-    } else if ((chunk + 1) * P <= N)
-        chunk_step<P, true>(R, edge, nsteps, q_solve, q_next, X, own, nxt, K, v, chunk, C);
-    else
-        chunk_step<P, false>(R, edge, nsteps, q_solve, q_next, X, own, nxt, K, v, chunk, C);
+    double* K = kl + w * P * 64 + v;
+    double sm[6];
+    bool have = false;
+    if (chunk < C && v < nV) {
+        if (chunk * P >= 2 && (chunk + 1) * P - 1 <= N - 3) {
+            // downward pivots of the interior path, lane-interleaved like K
+            __shared__ double tl[kDiffWaves * P * 64];
+            double* T1 = tl + w * P * 64 + v;
+            have = chunk_step_interior<P, BDF2>(R, g1, g2, nsteps, q_solve, q_next, X2, coef, G, kc, K,
+                                                T1, v, chunk, sm);
+        } else if ((chunk + 1) * P <= N) {
+            have = chunk_step<P, true>(R, edge, nsteps, q_solve, q_next, X2, coef, G, kc, K, v, chunk,
+                                       C, sm);
+        } else {
+            have = chunk_step<P, false>(R, edge, nsteps, q_solve, q_next, X2, coef, G, kc, K, v, chunk,
+                                        C, sm);
+        }
+    }
+    if (q_next < 0) return;
+    if (have) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) K[k * 64] = sm[k];
+    }
+    __syncthreads();
+    if (w != 0 || v >= nV || !var_active(nsteps, v, q_next)) return;
+    // ---- workgroup elimination: reduced row j of separator u_j (chunk 4g + j),
+    // a_j u_(j-1) + b_j u_j + c_j u_(j+1) = r_j with a = own0(j), b = own1(j) + nxt0(j + 1),
+    // c = nxt1(j + 1), r = own2(j) + nxt2(j + 1)
+    double o[kDiffWaves][3], n[kDiffWaves][3];
+#pragma unroll
+    for (int j = 0; j < kDiffWaves; j++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            o[j][k] = j < kc ? kl[j * P * 64 + k * 64 + v] : 0.0;
+            n[j][k] = j < kc ? kl[j * P * 64 + (3 + k) * 64 + v] : 0.0;
+        }
+    double O0 = o[0][0], O1 = o[0][1], O2 = o[0][2];
+    double N0 = n[0][0], N1 = n[0][1], N2 = n[0][2];
+    if (kc > 1) {
+        double* cv = coef + (size_t)g * kWgCoef + v;
+        // downward over the interior separators j = 0 .. kc-2 (fill-in column at: u_(-1))
+        double bt = 0.0, rt = 0.0, at = 0.0, cL = 0.0, ib = 0.0;
+#pragma unroll
+        for (int j = 0; j < kDiffWaves - 1; j++) {
+            if (j < kc - 1) {
+                const double aj = o[j][0], bj = o[j][1] + n[j + 1][0], cj = n[j + 1][1],
+                             rj = o[j][2] + n[j + 1][2];
+                if (j == 0) {
+                    bt = bj; rt = rj; at = aj;
+                } else {
+                    const double f = aj * ib;
+                    bt = bj - f * cL; rt = rj - f * rt; at = -f * at;
+                }
+                cL = cj;
+                ib = rcp_nr(bt);
+                cv[(j * 4 + 0) * 64] = rt;
+                cv[(j * 4 + 1) * 64] = at;
+                cv[(j * 4 + 2) * 64] = cj;
+                cv[(j * 4 + 3) * 64] = ib;
+            }
+        }
+        // the workgroup separator's row (own part of its last chunk) without u_(kc-2)
+        double aE = 0.0, bE = 0.0, rE = 0.0;
+#pragma unroll
+        for (int j = 1; j < kDiffWaves; j++)
+            if (j == kc - 1) { aE = o[j][0]; bE = o[j][1]; rE = o[j][2]; }
+        O0 = -aE * at * ib;
+        O1 = bE - aE * cL * ib;
+        O2 = rE - aE * rt * ib;
+        // upward over the interior separators: the workgroup's part of the row of u_(-1)
+        double bh = 0.0, rh = 0.0, ch = 0.0, aN = 0.0;
+#pragma unroll
+        for (int j = kDiffWaves - 2; j >= 0; j--) {
+            if (j < kc - 1) {
+                const double aj = o[j][0], bj = o[j][1] + n[j + 1][0], cj = n[j + 1][1],
+                             rj = o[j][2] + n[j + 1][2];
+                if (j == kc - 2) {
+                    bh = bj; rh = rj; ch = cj;
+                } else {
+                    const double gg = cj * rcp_nr(bh);
+                    bh = bj - gg * aN; rh = rj - gg * rh; ch = -gg * ch;
+                }
+                aN = aj;
+            }
+        }
+        const double cpv = n[0][1], ibh = rcp_nr(bh);
+        N0 = n[0][0] - cpv * aN * ibh;
+        N1 = -cpv * ch * ibh;
+        N2 = n[0][2] - cpv * rh * ibh;
+    }
+    const size_t pl = (size_t)nV * G, e = (size_t)v * G + g;
+    own2[e] = O0;
+    own2[pl + e] = O1;
+    own2[2 * pl + e] = O2;
+    nxt2[e] = N0;
+    nxt2[pl + e] = N1;
+    nxt2[2 * pl + e] = N2;
 }
 
 // Separators of one variable per workgroup of kPcrThreads threads, from the reduced
-// tridiagonal system of the chunk summaries (C rows, C <= kPcrMax).  All threads read
-// the rows coalesced into LDS (8 waves keep twice the loads of 4 in flight); then each of
-// the first kPcrRuns threads owns a run of G = ceil(C / kPcrRuns) <= 16 consecutive rows
-// and eliminates its interior towards both ends (the same downward/upward elimination as
-// the chunks); the <= kPcrRuns run separators are solved by parallel cyclic reduction
-// (each level couples row i with rows i -/+ stride and doubles the stride; two buffers,
-// one barrier per level), and each run thread back-substitutes its run.  Few long runs
-// keep the O(S log S) reduction small: the workgroup is the only one of its variable, so
-// its latency is the critical path.
-constexpr int kPcrThreads = 512;
-constexpr int kPcrRuns = 256;
-constexpr int kRunMax = kPcrMax / kPcrRuns;
+// tridiagonal system of the workgroup summaries (C rows, C <= kPcrRowsMax).  All threads
+// read the rows coalesced into LDS; then each lane of wave 0 owns a run of kRunMax
+// consecutive rows (S = ceil(C / kRunMax) <= 64 runs) and eliminates its interior towards
+// both ends (the downward/upward elimination of the chunks; in a full run the two sweeps
+// are interleaved, independent chains of reciprocals); the S run separators are solved by
+// parallel cyclic reduction inside the wave (each level couples row i with rows i -/+ stride
+// through cross-lane reads and doubles the stride: no barriers), and each lane
+// back-substitutes its run.  The workgroup is the only one of its variable, so its latency
+// is the critical path of the solve.
+constexpr int kPcrThreads = 256;
+constexpr int kRunMax = 16;
+constexpr int kPcrRowsMax = kRunMax * 64;
+static_assert(kPcrMax / kDiffWaves <= kPcrRowsMax, "reduced system larger than one wave's runs");
 
 // LDS index of reduced row i: one pad double per 16 rows, so that the rows of consecutive
 // runs (16 apart) fall into different banks
 __host__ __device__ __forceinline__ int pcr_slot(int i) { return i + (i >> 4); }
 
-size_t pcr_lds_doubles(int C) { return 4 * (size_t)pcr_slot(C) + 2 * 4 * kPcrRuns; }
+size_t pcr_lds_doubles(int C) { return 4 * (size_t)pcr_slot(C); }
 
 __global__ void __launch_bounds__(kPcrThreads) pcr_kernel(const double* own, const double* nxt, int C,
                                                           int nV, const int* nsteps, int q,
@@ -929,31 +1065,31 @@ __global__ void __launch_bounds__(kPcrThreads) pcr_kernel(const double* own, con
     double* b = a + CS;
     double* c = b + CS;
     double* r = c + CS;
-    double* sm = r + CS;  // separator system: buffer p, array k at sm + (4 p + k) * kPcrRuns
-    auto sep = [&](int p, int k) { return sm + (4 * p + k) * kPcrRuns; };
     const size_t pl = (size_t)nV * C;
     const double* ov = own + (size_t)v * C;
     const double* nv = nxt + (size_t)v * C;
     double* Xv = X + (size_t)v * C;
     const int tid = threadIdx.x;
-    // reduced row i: own part of chunk i plus the interior contribution of chunk i + 1
-    // (batches of kLoadBatch rows per thread, all loads of a batch in flight together)
-    constexpr int kLoadBatch = kPcrMax / kPcrThreads;
-    for (int kb = 0; kb * kPcrThreads < C; kb += kLoadBatch) {
+    // reduced row i: own part of workgroup i plus the interior contribution of workgroup
+    // i + 1 (all of a thread's loads in flight together; batches past C are skipped)
+    constexpr int kLoadBatch = kPcrRowsMax / kPcrThreads;
+    {
         double la[kLoadBatch], lb[kLoadBatch], lc[kLoadBatch], lr[kLoadBatch];
 #pragma unroll
         for (int k = 0; k < kLoadBatch; k++) {
-            const int i = min(tid + (kb + k) * kPcrThreads, C - 1), i1 = min(i + 1, C - 1);
-            const double last = i + 1 < C ? 1.0 : 0.0;
-            la[k] = ov[i];
-            lb[k] = ov[pl + i] + last * nv[i1];
-            lc[k] = last * nv[pl + i1];
-            lr[k] = ov[2 * pl + i] + last * nv[2 * pl + i1];
+            if (k * kPcrThreads < C) {
+                const int i = min(tid + k * kPcrThreads, C - 1), i1 = min(i + 1, C - 1);
+                const double last = i + 1 < C ? 1.0 : 0.0;
+                la[k] = ov[i];
+                lb[k] = ov[pl + i] + last * nv[i1];
+                lc[k] = last * nv[pl + i1];
+                lr[k] = ov[2 * pl + i] + last * nv[2 * pl + i1];
+            }
         }
 #pragma unroll
         for (int k = 0; k < kLoadBatch; k++) {
-            const int i = tid + (kb + k) * kPcrThreads;
-            if (i < C) {
+            const int i = tid + k * kPcrThreads;
+            if (k * kPcrThreads < C && i < C) {
                 const int o = pcr_slot(i);
                 a[o] = la[k];
                 b[o] = lb[k];
@@ -962,24 +1098,26 @@ __global__ void __launch_bounds__(kPcrThreads) pcr_kernel(const double* own, con
             }
         }
     }
-    const int G = (C + kPcrRuns - 1) / kPcrRuns;  // rows per run
-    const int S = (C + G - 1) / G;                      // runs (separator rows)
-    const int s0 = tid * G;
-    const int e0 = min(s0 + G, C) - 1;  // separator of this run
-    const int mi = e0 - s0;             // interior rows of the run
     __syncthreads();
-    // the run's rows in registers (rows past the run are clamped copies, never used)
-    double ra[kRunMax], rb[kRunMax], rc[kRunMax], rr[kRunMax];
+    if (tid < 64) {
+        const int S = (C + kRunMax - 1) / kRunMax;  // runs (separator rows), <= 64
+        const bool act = tid < S;
+        const int s0 = tid * kRunMax;
+        const int e0 = act ? min(s0 + kRunMax, C) - 1 : s0;  // separator of this run
+        const int mi = e0 - s0;                               // interior rows of the run
+        // the run's rows in registers (rows past the run are clamped copies, never used)
+        double ra[kRunMax], rb[kRunMax], rc[kRunMax], rr[kRunMax];
 #pragma unroll
-    for (int k = 0; k < kRunMax; k++) {
-        const int o = pcr_slot(min(s0 + k, C - 1));
-        ra[k] = a[o];
-        rb[k] = b[o];
-        rc[k] = c[o];
-        rr[k] = r[o];
-    }
-    const double cp = tid > 0 && tid < S ? c[pcr_slot(s0 - 1)] : 0.0;
-    if (tid < S) {
+        for (int k = 0; k < kRunMax; k++) {
+            const int o = pcr_slot(min(s0 + k, C - 1));
+            ra[k] = a[o];
+            rb[k] = b[o];
+            rc[k] = c[o];
+            rr[k] = r[o];
+        }
+        const double cp = tid > 0 && act ? c[pcr_slot(s0 - 1)] : 0.0;
+        const bool full = mi == kRunMax - 1;
+        double iv[kRunMax - 1];  // full runs: reciprocals of the downward pivots
         double aE = 0.0, bE = 0.0, rE = 0.0;
 #pragma unroll
         for (int k = 0; k < kRunMax; k++)
@@ -988,31 +1126,41 @@ __global__ void __launch_bounds__(kPcrThreads) pcr_kernel(const double* own, con
                 bE = rb[k];
                 rE = rr[k];
             }
-        // downward over the interior: the separator row in terms of the previous separator
-        double bt = 1.0, rt = 0.0, at = 0.0, cL = 0.0;
-#pragma unroll
-        for (int k = 0; k < kRunMax; k++) {
-            if (k < mi) {
-                if (k == 0) {
-                    bt = rb[k]; rt = rr[k]; at = ra[k];
-                } else {
-                    const double f = ra[k] * rcp_nr(bt);
-                    bt = rb[k] - f * cL; rt = rr[k] - f * rt; at = -f * at;
-                }
-                cL = rc[k];
-            }
-        }
-        double oa = aE, ob = bE, orr = rE;
-        if (mi > 0) {
-            const double ib = rcp_nr(bt);
-            oa = -aE * at * ib;
-            ob = bE - aE * cL * ib;
-            orr = rE - aE * rt * ib;
-        }
+        // downward over the interior: the separator row in terms of the previous separator;
         // upward over the interior: the run's contribution to the previous separator's row
-        double nb_ = 0.0, nc_ = cp, nr_ = 0.0;
-        if (tid > 0 && mi > 0) {
-            double bh = 1.0, rh = 0.0, ch = 0.0, aN = 0.0;
+        double bt = 1.0, rt = 0.0, at = 0.0, cL = 0.0;
+        double bh = 1.0, rh = 0.0, ch = 0.0, aN = 0.0;
+        if (full) {
+            // the downward pivots' reciprocals stay in iv for the back-substitution, whose
+            // forward sweep has the same pivots
+            constexpr int M = kRunMax - 1;
+            bt = rb[0]; rt = rr[0]; at = ra[0]; cL = rc[0];
+            bh = rb[M - 1]; rh = rr[M - 1]; ch = rc[M - 1]; aN = ra[M - 1];
+#pragma unroll
+            for (int k = 1; k < M; k++) {
+                iv[k - 1] = rcp_nr(bt);
+                const double f = ra[k] * iv[k - 1];
+                bt = rb[k] - f * cL; rt = rr[k] - f * rt; at = -f * at;
+                cL = rc[k];
+                const int u = M - 1 - k;
+                const double g = rc[u] * rcp_nr(bh);
+                bh = rb[u] - g * aN; rh = rr[u] - g * rh; ch = -g * ch;
+                aN = ra[u];
+            }
+            iv[M - 1] = rcp_nr(bt);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kRunMax; k++) {
+                if (k < mi) {
+                    if (k == 0) {
+                        bt = rb[k]; rt = rr[k]; at = ra[k];
+                    } else {
+                        const double f = ra[k] * rcp_nr(bt);
+                        bt = rb[k] - f * cL; rt = rr[k] - f * rt; at = -f * at;
+                    }
+                    cL = rc[k];
+                }
+            }
 #pragma unroll
             for (int k = kRunMax - 1; k >= 0; k--) {
                 if (k < mi) {
@@ -1025,91 +1173,116 @@ __global__ void __launch_bounds__(kPcrThreads) pcr_kernel(const double* own, con
                     aN = ra[k];
                 }
             }
+        }
+        // separator row of the run: (sa, sb, sc, sr); runs past S are identity rows
+        double sa = aE, sb = bE, sc = 0.0, sr = rE;
+        if (mi > 0) {
+            const double ib = full ? iv[kRunMax - 2] : rcp_nr(bt);
+            sa = -aE * at * ib;
+            sb = bE - aE * cL * ib;
+            sr = rE - aE * rt * ib;
+        }
+        double nb_ = 0.0, nc_ = cp, nr_ = 0.0;
+        if (tid > 0 && mi > 0) {
             const double ib = rcp_nr(bh);
             nb_ = -cp * aN * ib;
             nc_ = -cp * ch * ib;
             nr_ = -cp * rh * ib;
         }
-        sep(0, 0)[tid] = oa;
-        sep(0, 1)[tid] = ob;
-        sep(0, 3)[tid] = orr;
-        sep(1, 1)[tid] = nb_;  // exchange slots (buffer 1 is free until the first level)
-        sep(1, 2)[tid] = nc_;
-        sep(1, 3)[tid] = nr_;
-    }
-    __syncthreads();
-    if (tid < S) {
-        double cc = 0.0;
-        if (tid + 1 < S) {
-            sep(0, 1)[tid] += sep(1, 1)[tid + 1];
-            cc = sep(1, 2)[tid + 1];
-            sep(0, 3)[tid] += sep(1, 3)[tid + 1];
+        if (!act) {
+            sa = 0.0; sb = 1.0; sr = 0.0;
+            nb_ = 0.0; nc_ = 0.0; nr_ = 0.0;
         }
-        sep(0, 2)[tid] = cc;
-    }
-    __syncthreads();
-    // parallel cyclic reduction on the S separator rows, one row per thread
-    int p = 0;
-    for (int st = 1; st < S; st <<= 1) {
-        if (tid < S) {
-            const double* sa = sep(p, 0);
-            const double* sb = sep(p, 1);
-            const double* sc = sep(p, 2);
-            const double* sr = sep(p, 3);
+        // the next run's contribution to this separator's row
+        {
+            const double nbn = __shfl_down(nb_, 1), ncn = __shfl_down(nc_, 1), nrn = __shfl_down(nr_, 1);
+            if (tid + 1 < S) {
+                sb += nbn;
+                sc = ncn;
+                sr += nrn;
+            }
+        }
+        // parallel cyclic reduction on the S separator rows, one row per lane
+        for (int st = 1; st < S; st <<= 1) {
             const int lo = tid - st, hi = tid + st;
-            double na = 0.0, nc = 0.0, nb = sb[tid], nr = sr[tid];
+            const int l = lo >= 0 ? lo : tid, h = hi < 64 ? hi : tid;
+            const double la_ = __shfl(sa, l), lb_ = __shfl(sb, l), lc_ = __shfl(sc, l), lr_ = __shfl(sr, l);
+            const double ha_ = __shfl(sa, h), hb_ = __shfl(sb, h), hc_ = __shfl(sc, h), hr_ = __shfl(sr, h);
+            double na = 0.0, nc = 0.0, nb = sb, nr = sr;
             if (lo >= 0) {
-                const double al = -sa[tid] * rcp_nr(sb[lo]);
-                na = al * sa[lo];
-                nb += al * sc[lo];
-                nr += al * sr[lo];
+                const double al = -sa * rcp_nr(lb_);
+                na = al * la_;
+                nb += al * lc_;
+                nr += al * lr_;
             }
             if (hi < S) {
-                const double ga = -sc[tid] * rcp_nr(sb[hi]);
-                nc = ga * sc[hi];
-                nb += ga * sa[hi];
-                nr += ga * sr[hi];
+                const double ga = -sc * rcp_nr(hb_);
+                nc = ga * hc_;
+                nb += ga * ha_;
+                nr += ga * hr_;
             }
-            sep(1 - p, 0)[tid] = na;
-            sep(1 - p, 1)[tid] = nb;
-            sep(1 - p, 2)[tid] = nc;
-            sep(1 - p, 3)[tid] = nr;
+            sa = na;
+            sb = nb;
+            sc = nc;
+            sr = nr;
         }
-        p = 1 - p;
-        __syncthreads();
-    }
-    // back-substitute each run's interior between its neighbouring separators
-    if (tid < S) {
-        const double xR = sep(p, 3)[tid] / sep(p, 1)[tid];
-        const double xL = tid > 0 ? sep(p, 3)[tid - 1] / sep(p, 1)[tid - 1] : 0.0;
-        // forward elimination with 1/diag (in ra) and swept right-hand sides (in rr), then
-        // back substitution
+        // back-substitute each run's interior between its neighbouring separators
+        const double xR = sr / sb;
+        const double xLn = __shfl_up(xR, 1);
+        const double xL = tid > 0 ? xLn : 0.0;
+        if (act) {
+            if (full) {
+                // the pivots of the forward elimination are the downward sweep's (iv)
+                constexpr int M = kRunMax - 1;
 #pragma unroll
-        for (int k = 0; k < kRunMax; k++) {
-            if (k < mi) {
-                double ri = rr[k];
-                if (k == 0) ri -= ra[k] * xL;
-                if (k == mi - 1) ri -= rc[k] * xR;
-                if (k == 0) {
-                    ra[k] = rcp_nr(rb[k]);
-                } else {
-                    const double f = ra[k] * ra[k - 1];
-                    ra[k] = rcp_nr(rb[k] - f * rc[k - 1]);
-                    ri -= f * rr[k - 1];
+                for (int k = 0; k < M; k++) {
+                    double ri = rr[k];
+                    if (k == 0) ri -= ra[k] * xL;
+                    if (k == M - 1) ri -= rc[k] * xR;
+                    if (k > 0) {
+                        const double f = ra[k] * iv[k - 1];
+                        ri -= f * rr[k - 1];
+                    }
+                    rr[k] = ri;
                 }
-                rr[k] = ri;
-            }
-        }
-        double xn = xR;
+                double xn = xR;
 #pragma unroll
-        for (int k = kRunMax - 1; k >= 0; k--) {
-            if (k < mi) {
-                const double xi = (k == mi - 1) ? rr[k] * ra[k] : (rr[k] - rc[k] * xn) * ra[k];
-                r[pcr_slot(s0 + k)] = xi;
-                xn = xi;
+                for (int k = M - 1; k >= 0; k--) {
+                    const double xi = (k == M - 1) ? rr[k] * iv[k] : (rr[k] - rc[k] * xn) * iv[k];
+                    r[pcr_slot(s0 + k)] = xi;
+                    xn = xi;
+                }
+            } else {
+                // forward elimination with 1/diag (in ra) and swept right-hand sides (in rr),
+                // then back substitution
+#pragma unroll
+                for (int k = 0; k < kRunMax; k++) {
+                    if (k < mi) {
+                        double ri = rr[k];
+                        if (k == 0) ri -= ra[k] * xL;
+                        if (k == mi - 1) ri -= rc[k] * xR;
+                        if (k == 0) {
+                            ra[k] = rcp_nr(rb[k]);
+                        } else {
+                            const double f = ra[k] * ra[k - 1];
+                            ra[k] = rcp_nr(rb[k] - f * rc[k - 1]);
+                            ri -= f * rr[k - 1];
+                        }
+                        rr[k] = ri;
+                    }
+                }
+                double xn = xR;
+#pragma unroll
+                for (int k = kRunMax - 1; k >= 0; k--) {
+                    if (k < mi) {
+                        const double xi = (k == mi - 1) ? rr[k] * ra[k] : (rr[k] - rc[k] * xn) * ra[k];
+                        r[pcr_slot(s0 + k)] = xi;
+                        xn = xi;
+                    }
+                }
             }
+            r[pcr_slot(e0)] = xR;
         }
-        r[pcr_slot(e0)] = xR;
     }
     __syncthreads();
     // coalesced store of the separators (each run wrote its rows' values into the r slots)
@@ -1154,9 +1327,9 @@ size_t diffusion_work_doubles(int nV, int N) {
     const int C1 = (N + P1 - 1) / P1;
     const int C2 = (C1 + P2 - 1) / P2;
     // own1, nxt1 (3 each), X1, own2, nxt2, X2, cp, dp, hmin chunk mins, yprev; grid metrics;
-    // edge table
+    // edge table; workgroup back-substitution rows of the fused path
     return (size_t)nV * (6 * C1 + C1 + 6 * C2 + C2 + 2 * (size_t)std::max(C1, C2) + C1 + N) +
-           2 * (size_t)N + edge_doubles<P1>();
+           2 * (size_t)N + edge_doubles<P1>() + (size_t)kWgCoef * ((C1 + kDiffWaves - 1) / kDiffWaves);
 }
 
 void diffusion_carve(double* base, int nV, int N, DiffWork& w) {
@@ -1177,6 +1350,7 @@ void diffusion_carve(double* base, int nV, int N, DiffWork& w) {
     w.g1 = p; p += N;
     w.g2 = p; p += N;
     w.edge = p; p += edge_doubles<P1>();
+    w.coef = p; p += (size_t)kWgCoef * ((C1 + kDiffWaves - 1) / kDiffWaves);
 }
 
 namespace {
@@ -1216,12 +1390,13 @@ int launch_diffusion_solve(const DiffArgs& a, double* y, const double* h, const 
     return run_solve(R, out, a.N, a.nV, nsteps, q, w, st);
 }
 
-int launch_diffusion_stage(const DiffArgs& a, double* y, const double* h, const int* nsteps,
+int launch_diffusion_stage(const DiffArgs& a, double* y, const DiffSteps& steps, double* h, int* nsteps,
                            int nsolve, DiffWork& w, hipStream_t st) {
     const int C1 = (a.N + P1 - 1) / P1;
     // the fused path addresses point-major arrays with 32-bit buffer offsets
     const bool fits32 = (size_t)a.N * a.nV * sizeof(double) < ((size_t)1 << 31);
     if (C1 > kPcrMax || a.N < 2 * P1 || !fits32) {
+        if (int e = launch_set_steps(steps, a.nV, h, nsteps, st)) return e;
         for (int q = 0; q < nsolve; q++) {
             int e = launch_diffusion_solve(a, y, h, nsteps, q, w, st);
             if (e) return e;
@@ -1231,23 +1406,24 @@ int launch_diffusion_stage(const DiffArgs& a, double* y, const double* h, const 
     DiffRows R{a.B, a.B_row, a.D, a.sc, a.x, y, w.yprev, h, a.N, a.nV, a.alpha,
                a.left, a.right, a.yInf, a.wallConst, a.wallVar, 0};
     const int wpb = kDiffWaves;
-    const dim3 grid((C1 + wpb - 1) / wpb), block(64 * wpb);
-    const size_t pcr_lds = pcr_lds_doubles(C1) * sizeof(double);
+    const int G = (C1 + wpb - 1) / wpb;  // workgroup chunks: rows of the reduced system
+    const dim3 grid(G), block(64 * wpb);
+    const size_t pcr_lds = pcr_lds_doubles(G) * sizeof(double);
     const int nbm = (a.N + 255) / 256;
     hipLaunchKernelGGL((stage_setup_kernel<P1>), dim3(nbm + (3 * (P1 + 1) * 64 + 255) / 256), dim3(256), 0,
-                       st, R, nbm, w.g1, w.g2, w.edge);
+                       st, R, nbm, w.g1, w.g2, w.edge, steps, h, nsteps);
     // the BDF1 instance for launches whose solves are both BDF1 substeps
     auto step = [&](int qs, int qn) {
         if (qs >= 8 || qn >= 8)
             hipLaunchKernelGGL((diff_step_kernel<P1, true>), grid, block, 0, st, R, w.g1, w.g2, w.edge,
-                               nsteps, qs, qn, w.X1, w.own1, w.nxt1);
+                               nsteps, qs, qn, w.X1, w.coef, w.own1, w.nxt1);
         else
             hipLaunchKernelGGL((diff_step_kernel<P1, false>), grid, block, 0, st, R, w.g1, w.g2, w.edge,
-                               nsteps, qs, qn, w.X1, w.own1, w.nxt1);
+                               nsteps, qs, qn, w.X1, w.coef, w.own1, w.nxt1);
     };
     step(-1, 0);
     for (int q = 0; q < nsolve; q++) {
-        hipLaunchKernelGGL(pcr_kernel, dim3(a.nV), dim3(kPcrThreads), pcr_lds, st, w.own1, w.nxt1, C1, a.nV,
+        hipLaunchKernelGGL(pcr_kernel, dim3(a.nV), dim3(kPcrThreads), pcr_lds, st, w.own1, w.nxt1, G, a.nV,
                            nsteps, q, w.X1);
         step(q, q + 1 < nsolve ? q + 1 : -1);
     }

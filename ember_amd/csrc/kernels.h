@@ -96,7 +96,7 @@ struct DiffArgs {
 };
 
 struct DiffWork {
-    double *own1, *nxt1, *X1, *own2, *nxt2, *X2, *cp, *dp, *cmin, *yprev, *g1, *g2, *edge;
+    double *own1, *nxt1, *X1, *own2, *nxt2, *X2, *cp, *dp, *cmin, *yprev, *g1, *g2, *edge, *coef;
 };
 
 struct DiffSteps {
@@ -110,7 +110,7 @@ int launch_diffusion_hmin(const DiffArgs& a, DiffWork& w, double* hmin, hipStrea
 int launch_diffusion_solve(const DiffArgs& a, double* y, const double* h, const int* nsteps, int q,
                            DiffWork& w, hipStream_t st);
 // all nsolve solves of a stage (fused path when the grid allows it)
-int launch_diffusion_stage(const DiffArgs& a, double* y, const double* h, const int* nsteps,
+int launch_diffusion_stage(const DiffArgs& a, double* y, const DiffSteps& steps, double* h, int* nsteps,
                            int nsolve, DiffWork& w, hipStream_t st);
 int launch_explicit_solve(const double* A, const double* B, const double* C, const double* K,
                           int nSys, int N, double* y, const double* h, const int* nsteps, int q,
