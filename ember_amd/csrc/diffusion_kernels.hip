@@ -42,6 +42,7 @@ struct DiffRows {
     double yInf, wallConst;
     int wallVar;          // variable carrying yInf / wallConst (others: 0)
     int q;                // solve index: 0..7 BDF1 substeps, >= 8 BDF2 steps
+    int bdf2;             // fused stage: it has BDF2 solves (solve 0 keeps y_0 for the first)
 
     __device__ __forceinline__ double Bv(int v, int j) const {
         return B[(size_t)(B_row ? B_row[v] : v) * N + j];
@@ -551,29 +552,40 @@ __device__ __forceinline__ void down_pivots(int m, double sh, const double (&A)[
 // workgroup (the chunk summaries meet in LDS), so the reduced system that the separator
 // kernel solves has one row per workgroup (G = ceil(C / 4) rows).  For the back-substitution
 // the workgroup keeps, per interior separator j, the downward-sweep row
-// bt_j u_j + at_j u_(-1) + c_j u_(j+1) = rt_j as (rt_j, at_j, c_j, 1 / bt_j) in coef at
-// ((g * 3 + j) * 4 + i) * 64 + v; u_(-1) and u_(kc-1) are the workgroup separators X2[g-1]
-// (0 for g = 0) and X2[g].
-constexpr int kWgCoef = 3 * 4 * 64;  // coef doubles per workgroup
+// bt_j u_j + at_j u_(-1) + c_j u_(j+1) = rt_j; u_(-1) and u_(kc-1) are the workgroup
+// separators X2[g-1] (0 for g = 0) and X2[g].  Only rt_j depends on the solve's right-hand
+// side: at_j, c_j and 1 / bt_j depend on the matrix and the BDF shift, which a stage holds
+// fixed within each shift class (BDF1 substeps, BDF2 steps), so they are written by the
+// first solve of each class only.  coef layout: rt at (g * 3 + j) * 64 + v; then, per class
+// cl, (at, c, 1 / bt) of row j at 3 * 64 * G + ((cl * G + g) * 3 + j) * 3 * 64 + i * 64 + v.
+constexpr int kWgRows = kDiffWaves - 1;  // interior separators per workgroup
+__host__ __device__ __forceinline__ size_t wg_coef_doubles(int G) { return (size_t)(kWgRows + 2 * kWgRows * 3) * 64 * G; }
+__device__ __forceinline__ double* wg_rhs(double* coef, int g, int v) { return coef + (size_t)g * kWgRows * 64 + v; }
+__device__ __forceinline__ double* wg_mat(double* coef, int G, int cl, int g, int v) {
+    return coef + (size_t)kWgRows * 64 * G + ((size_t)cl * G + g) * kWgRows * 3 * 64 + v;
+}
 struct WgSep {
-    double UL, UR, cf[3][4];
+    double UL, UR, cf[kWgRows][4];
     __device__ __forceinline__ void load(const double* X2, const double* coef, int v, int G, int g,
-                                         int kc) {
+                                         int kc, int cl) {
         const double* Xv = X2 + (size_t)v * G;
         UL = g > 0 ? Xv[g - 1] : 0.0;
         UR = Xv[g];
-        const double* cv = coef + (size_t)g * kWgCoef + v;
+        const double* cr = wg_rhs(const_cast<double*>(coef), g, v);
+        const double* cm = wg_mat(const_cast<double*>(coef), G, cl, g, v);
 #pragma unroll
-        for (int j = 0; j < 3; j++)
+        for (int j = 0; j < kWgRows; j++) {
+            cf[j][0] = j < kc - 1 ? cr[j * 64] : 0.0;
 #pragma unroll
-            for (int i = 0; i < 4; i++) cf[j][i] = j < kc - 1 ? cv[(j * 4 + i) * 64] : 0.0;
+            for (int i = 0; i < 3; i++) cf[j][i + 1] = j < kc - 1 ? cm[(j * 3 + i) * 64] : 0.0;
+        }
     }
     // the separators left and right of chunk 4g + w
     __device__ __forceinline__ void get(int w, int kc, double& xL, double& xR) const {
-        double u[4];
-        u[3] = UR;
+        double u[kDiffWaves];
+        u[kDiffWaves - 1] = UR;
 #pragma unroll
-        for (int j = 2; j >= 0; j--)
+        for (int j = kWgRows - 1; j >= 0; j--)
             u[j] = j < kc - 1 ? (cf[j][0] - cf[j][1] * UL - cf[j][2] * u[j + 1]) * cf[j][3] : UR;
         xR = w == 0 ? u[0] : (w == 1 ? u[1] : (w == 2 ? u[2] : u[3]));
         xL = w == 0 ? UL : (w == 1 ? u[0] : (w == 2 ? u[1] : u[2]));
@@ -608,7 +620,7 @@ __device__ __forceinline__ bool chunk_step(const DiffRows& R, const double* __re
     for (int t = 0; t < P; t++)
         if (FULL || t < m) Y[t] = Yb.ld(v8, s + t);
     WgSep ws;
-    if (q_solve >= 0) ws.load(X2, coef, v, G, chunk / 4, kc);
+    if (q_solve >= 0) ws.load(X2, coef, v, G, chunk / 4, kc, q_solve >= 8);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < P; t++)
@@ -640,7 +652,7 @@ __device__ __forceinline__ bool chunk_step(const DiffRows& R, const double* __re
             }
         }
         // back substitution (yprev takes the old y: the next BDF2 step's y_(n-1))
-        const bool save_prev = q_solve == 0 || q_solve >= 8;
+        const bool save_prev = (q_solve == 0 && R.bdf2) || q_solve >= 8;
         double xn = xR;
 #pragma unroll
         for (int t = P - 1; t >= 0; t--) {
@@ -772,7 +784,7 @@ __device__ __forceinline__ bool chunk_step_interior(const DiffRows& R, const dou
 #pragma unroll
     for (int t = 0; t < P; t++) Y[t] = Yb.ld(v8, s + t);
     WgSep ws;
-    if (q_solve >= 0) ws.load(X2, coef, v, G, chunk / 4, kc);
+    if (q_solve >= 0) ws.load(X2, coef, v, G, chunk / 4, kc, q_solve >= 8);
     if (s2) {
 #pragma unroll
         for (int t = 0; t < P; t++) T2[t] = Pb.ld(v8, s + t);
@@ -829,7 +841,7 @@ __device__ __forceinline__ bool chunk_step_interior(const DiffRows& R, const dou
             else
                 T2[t] = r - A[t] * T1[(t - 1) * 64] * T2[t - 1];
         }
-        const bool save_prev = q_solve == 0 || q_solve >= 8;
+        const bool save_prev = (q_solve == 0 && R.bdf2) || q_solve >= 8;
         double xn = xR;
 #pragma unroll
         for (int t = P - 1; t >= 0; t--) {
@@ -972,8 +984,57 @@ __global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const dou
         }
     double O0 = o[0][0], O1 = o[0][1], O2 = o[0][2];
     double N0 = n[0][0], N1 = n[0][1], N2 = n[0][2];
-    if (kc > 1) {
-        double* cv = coef + (size_t)g * kWgCoef + v;
+    if (kc == kDiffWaves) {
+        // a full workgroup: the same sweeps as below in straight-line code, so that the
+        // downward and upward chains of reciprocals can interleave
+        double* cr = wg_rhs(coef, g, v);
+        const bool first = q_next == 0 || q_next == 8;
+        double* cm = wg_mat(coef, G, q_next >= 8, g, v);
+        double aj[kWgRows], bj[kWgRows], cj[kWgRows], rj[kWgRows];
+#pragma unroll
+        for (int j = 0; j < kWgRows; j++) {
+            aj[j] = o[j][0];
+            bj[j] = o[j][1] + n[j + 1][0];
+            cj[j] = n[j + 1][1];
+            rj[j] = o[j][2] + n[j + 1][2];
+        }
+        double bt = bj[0], rt = rj[0], at = aj[0], ib = rcp_nr(bt);
+        double bh = bj[kWgRows - 1], rh = rj[kWgRows - 1], ch = cj[kWgRows - 1], aN = aj[kWgRows - 1];
+        double rts[kWgRows], ats[kWgRows], ibs[kWgRows];
+        rts[0] = rt; ats[0] = at; ibs[0] = ib;
+#pragma unroll
+        for (int j = 1; j < kWgRows; j++) {
+            const double f = aj[j] * ib;
+            bt = bj[j] - f * cj[j - 1]; rt = rj[j] - f * rt; at = -f * at;
+            ib = rcp_nr(bt);
+            rts[j] = rt; ats[j] = at; ibs[j] = ib;
+            const int u = kWgRows - 1 - j;
+            const double gg = cj[u] * rcp_nr(bh);
+            bh = bj[u] - gg * aN; rh = rj[u] - gg * rh; ch = -gg * ch;
+            aN = aj[u];
+        }
+#pragma unroll
+        for (int j = 0; j < kWgRows; j++) {
+            cr[j * 64] = rts[j];
+            if (first) {
+                cm[(j * 3 + 0) * 64] = ats[j];
+                cm[(j * 3 + 1) * 64] = cj[j];
+                cm[(j * 3 + 2) * 64] = ibs[j];
+            }
+        }
+        const double aE = o[kDiffWaves - 1][0], bE = o[kDiffWaves - 1][1], rE = o[kDiffWaves - 1][2];
+        O0 = -aE * at * ib;
+        O1 = bE - aE * cj[kWgRows - 1] * ib;
+        O2 = rE - aE * rt * ib;
+        const double cpv = n[0][1], ibh = rcp_nr(bh);
+        N0 = n[0][0] - cpv * aN * ibh;
+        N1 = -cpv * ch * ibh;
+        N2 = n[0][2] - cpv * rh * ibh;
+    } else if (kc > 1) {
+        double* cr = wg_rhs(coef, g, v);
+        // the matrix part, once per shift class (its first solve)
+        const bool first = q_next == 0 || q_next == 8;
+        double* cm = wg_mat(coef, G, q_next >= 8, g, v);
         // downward over the interior separators j = 0 .. kc-2 (fill-in column at: u_(-1))
         double bt = 0.0, rt = 0.0, at = 0.0, cL = 0.0, ib = 0.0;
 #pragma unroll
@@ -989,10 +1050,12 @@ __global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const dou
                 }
                 cL = cj;
                 ib = rcp_nr(bt);
-                cv[(j * 4 + 0) * 64] = rt;
-                cv[(j * 4 + 1) * 64] = at;
-                cv[(j * 4 + 2) * 64] = cj;
-                cv[(j * 4 + 3) * 64] = ib;
+                cr[j * 64] = rt;
+                if (first) {
+                    cm[(j * 3 + 0) * 64] = at;
+                    cm[(j * 3 + 1) * 64] = cj;
+                    cm[(j * 3 + 2) * 64] = ib;
+                }
             }
         }
         // the workgroup separator's row (own part of its last chunk) without u_(kc-2)
@@ -1043,7 +1106,7 @@ __global__ void __launch_bounds__(256, 2) diff_step_kernel(DiffRows R, const dou
 // through cross-lane reads and doubles the stride: no barriers), and each lane
 // back-substitutes its run.  The workgroup is the only one of its variable, so its latency
 // is the critical path of the solve.
-constexpr int kPcrThreads = 256;
+constexpr int kPcrThreads = 512;
 constexpr int kRunMax = 16;
 constexpr int kPcrRowsMax = kRunMax * 64;
 static_assert(kPcrMax / kDiffWaves <= kPcrRowsMax, "reduced system larger than one wave's runs");
@@ -1329,7 +1392,7 @@ size_t diffusion_work_doubles(int nV, int N) {
     // own1, nxt1 (3 each), X1, own2, nxt2, X2, cp, dp, hmin chunk mins, yprev; grid metrics;
     // edge table; workgroup back-substitution rows of the fused path
     return (size_t)nV * (6 * C1 + C1 + 6 * C2 + C2 + 2 * (size_t)std::max(C1, C2) + C1 + N) +
-           2 * (size_t)N + edge_doubles<P1>() + (size_t)kWgCoef * ((C1 + kDiffWaves - 1) / kDiffWaves);
+           2 * (size_t)N + edge_doubles<P1>() + wg_coef_doubles((C1 + kDiffWaves - 1) / kDiffWaves);
 }
 
 void diffusion_carve(double* base, int nV, int N, DiffWork& w) {
@@ -1350,7 +1413,7 @@ void diffusion_carve(double* base, int nV, int N, DiffWork& w) {
     w.g1 = p; p += N;
     w.g2 = p; p += N;
     w.edge = p; p += edge_doubles<P1>();
-    w.coef = p; p += (size_t)kWgCoef * ((C1 + kDiffWaves - 1) / kDiffWaves);
+    w.coef = p; p += wg_coef_doubles((C1 + kDiffWaves - 1) / kDiffWaves);
 }
 
 namespace {
@@ -1405,6 +1468,8 @@ int launch_diffusion_stage(const DiffArgs& a, double* y, const DiffSteps& steps,
     }
     DiffRows R{a.B, a.B_row, a.D, a.sc, a.x, y, w.yprev, h, a.N, a.nV, a.alpha,
                a.left, a.right, a.yInf, a.wallConst, a.wallVar, 0};
+    // y_0 is kept in yprev only for the stage's first BDF2 solve (solve 8)
+    R.bdf2 = nsolve > 8;
     const int wpb = kDiffWaves;
     const int G = (C1 + wpb - 1) / wpb;  // workgroup chunks: rows of the reduced system
     const dim3 grid(G), block(64 * wpb);
