@@ -35,9 +35,10 @@ sys.path.insert(0, ROOT)
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled by the
 # counter's reading for a plain copy of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0):
 # profiles/qss_gri30_n4096_stage.md (stage kernel, 4096 GRI points) and
-# profiles/diffusion_k53_n65536.md (diff_step + pcr kernels, one solve of 53 x 65536).
+# profiles/diffusion_k53_n65536.md (one diffusion stage of 53 x 65536: the setup launch,
+# 9 diff_step and 8 pcr launches, summed and divided by the stage's 8 solves).
 STAGE_TRAFFIC = 1555 * 2 * 1024 + 2355 * 1024
-DIFFUSION_TRAFFIC_PER_SOLVE = (48650 + 5220) * 2 * 1024 + (42200 + 1696) * 1024
+DIFFUSION_TRAFFIC_PER_SOLVE = (416743 * 2 * 1024 + 256761 * 1024) // 8
 
 
 METRIC = ("GRI-scale point-integrations/s per split dt @rtol1e-6, batched vs plain GPU; "
