@@ -129,7 +129,10 @@ def main():
         res["bdf"] = run_bdf(sub, dev)
         sub.points = 65536
         res["uneven"] = _subobject(run_uneven(sub, dev))
-        res["diffusion"] = _subobject(run_diffusion(sub, dev))
+        # a diffusion stage takes ~0.3 ms: time 50 of them (after 3 warm-up stages)
+        dsub = argparse.Namespace(**vars(sub))
+        dsub.steps, dsub.warmup = 50, 3
+        res["diffusion"] = _subobject(run_diffusion(dsub, dev))
         res["thermo"] = _subobject(run_thermo(sub, dev))
         res["transport"] = _subobject(run_transport(sub, dev))
         sub.points = 400

@@ -63,6 +63,10 @@ class SplitArrays(C.Structure):
                 ("ddt_cross", _vp), ("delta_diff", _vp), ("delta_prod", _vp), ("work", _vp)]
 
 
+class SplitConvection(C.Structure):
+    _fields_ = [("params", C.c_void_p), ("rho", _vp), ("Wmx", _vp), ("delta_conv", _vp), ("steps", _vp)]
+
+
 # (name, restype, argtypes) for every entry point of include/ember_amd.h
 SIGNATURES = [
     ("emx_default_options", None, [C.POINTER(Options)]),
@@ -103,6 +107,11 @@ SIGNATURES = [
     ("emx_split_step_device", C.c_int,
      [_vp, _sz, _sz, C.POINTER(SplitArrays), _vp, _sz, _vp, _vp, C.POINTER(Grid), _vp, _dp,
       C.c_double, C.c_double, C.c_int, _vp]),
+    ("emx_split_step_conv_device", C.c_int,
+     [_vp, _sz, _sz, C.POINTER(SplitArrays), C.POINTER(SplitConvection), _vp, _sz, _vp, _vp,
+      C.POINTER(Grid), _vp, _dp, C.c_double, C.c_double, C.c_int, _vp]),
+    ("emx_density_derivative_device", C.c_int,
+     [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp]),
     ("emx_convection_integrate_device", C.c_int,
      [_vp, _sz, _sz, C.c_void_p, _vp, _vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp]),
     ("emx_convection_integrate", C.c_int,
@@ -359,6 +368,37 @@ class Context:
             _tptr(D, N * nV), C.byref(g), _tptr(qloss, N), _p(hv), t, dt, 1 if balanced else 0,
             _tptr(status, N, _I32)))
 
+    def split_step_conv_device(self, arrays: "SplitState", conv_params, B, D, x, t, dt, rho=None, Wmx=None,
+                               steps=None, B_row=None, qloss=None, alpha=0, left_bc=0, right_bc=0,
+                               y_inf=0.0, wall_const=0.0, wall_var=1, h_per_var=None, balanced=True,
+                               status=None):
+        """One device-resident split step with its convection stages (SplitSolver::step,
+        D1/4 C1/2 D1/4 P D1/4 C1/2 D1/4) on arrays.state, in place; arrays made with
+        SplitState(..., convection=True); conv_params from convection_params()."""
+        N, nV = arrays.state.shape
+        if arrays.delta_conv is None:
+            raise ValueError("the split state needs convection=True")
+        g = Grid(C.cast(_tptr(x, N), _dp), alpha, left_bc, right_bc, y_inf, wall_const, wall_var)
+        f = arrays.fields()
+        sa = SplitArrays(*(_tptr(t_, (4 * nV + 1) * N if i == len(f) - 1 else N * nV)
+                           for i, t_ in enumerate(f)))
+        sc = SplitConvection(C.cast(C.pointer(conv_params), C.c_void_p), _tptr(rho, N), _tptr(Wmx, N),
+                             _tptr(arrays.delta_conv, N * nV), _tptr(steps, 2 * (nV - 1), _I32))
+        hv = None if h_per_var is None else np.ascontiguousarray(h_per_var, dtype=np.float64)
+        _check(lib().emx_split_step_conv_device(
+            self.h, nV, N, C.byref(sa), C.byref(sc), _tptr(B, B.shape[0] * N), B.shape[0],
+            _tptr(B_row, nV, _I32), _tptr(D, N * nV), C.byref(g), _tptr(qloss, N), _p(hv), t, dt,
+            1 if balanced else 0, _tptr(status, N, _I32)))
+
+    def density_derivative_device(self, state, ddt_conv, ddt_diff, ddt_prod, rho, Wmx, drhodt,
+                                  ddt_cross=None, balanced=True):
+        """FlameSolver::prepareIntegrators' drhodt (nPoints) from device matrices."""
+        N, nV = state.shape
+        _check(lib().emx_density_derivative_device(
+            self.h, nV, N, _tptr(state, N * nV), _tptr(ddt_conv, N * nV), _tptr(ddt_diff, N * nV),
+            _tptr(ddt_prod, N * nV), _tptr(ddt_cross, N * nV), 1 if balanced else 0, _tptr(rho, N),
+            _tptr(Wmx, N), _tptr(drhodt, N)))
+
     def convection_params(self, Yleft=None, **kw):
         """emx_convection_params with Ember's defaults (planar, FixedValue ends, fixedLeft
         continuity, zero strain); Yleft is kept alive on the returned object."""
@@ -459,14 +499,18 @@ class SplitState:
     """Device matrices of SplitSolver's bookkeeping for emx_split_step_device
     (torch CUDA tensors, (nPoints, nVars) = Ember's point-contiguous layout)."""
 
-    def __init__(self, state, ddt_cross=None):
+    def __init__(self, state, ddt_cross=None, convection=False):
         import torch
         self.state = state
         z = lambda: torch.zeros_like(state)
         self.ddt_conv, self.ddt_diff, self.ddt_prod = z(), z(), z()
         self.ddt_cross = ddt_cross
         self.delta_diff, self.delta_prod = z(), z()
-        self.work = torch.zeros((3,) + tuple(state.shape), dtype=state.dtype, device=state.device)
+        # emx_split_step_conv_device: delta_conv, and work for splitConstConv and drhodt
+        self.delta_conv = z() if convection else None
+        N, nV = state.shape
+        self.work = torch.zeros((4 * nV + 1) * N if convection else 3 * N * nV, dtype=state.dtype,
+                                device=state.device)
 
     def fields(self):
         return (self.state, self.ddt_conv, self.ddt_diff, self.ddt_prod, self.ddt_cross,

@@ -107,6 +107,7 @@ struct emx_context {
     // host-array stage entry point)
     // convection stage work arrays
     DevBuf cv_work, cv_rec_t, cv_rec_v, cv_ints, cv_aux, cv_yleft;
+    std::vector<double> cv_yleft_host;  // the inflow composition cv_yleft holds
     int interp_n = kInterpN;
     double interp_tmin = kInterpTmin, interp_tmax = kInterpTmax;
 };
@@ -1044,10 +1045,12 @@ static int diffusion_steps(emx_context* c, const DiffArgs& a, std::vector<double
     return 0;
 }
 
-int emx_split_step_device(emx_context* c, size_t nVars, size_t nPoints, const emx_split_arrays* s,
-                          const double* B, size_t nB, const int32_t* B_row, const double* D,
-                          const emx_grid* g, const double* qloss, const double* h_per_var, double t,
-                          double dt, int balanced, int32_t* status) {
+// SplitSolver::step on the device (emx_split_step_device, emx_split_step_conv_device);
+// conv NULL: empty convection stages.
+static int split_step(emx_context* c, size_t nVars, size_t nPoints, const emx_split_arrays* s,
+                      const emx_split_convection* conv, const double* B, size_t nB, const int32_t* B_row,
+                      const double* D, const emx_grid* g, const double* qloss, const double* h_per_var,
+                      double t, double dt, int balanced, int32_t* status) {
     if (int rc = check_diff(c, nVars, nPoints, g)) return rc;
     if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
     if (!s || !s->state || !s->ddt_conv || !s->ddt_diff || !s->ddt_prod || !s->delta_diff ||
@@ -1055,12 +1058,19 @@ int emx_split_step_device(emx_context* c, size_t nVars, size_t nPoints, const em
         return fail(-1, "split arrays: only ddt_cross may be NULL");
     if (!B_row && nB != nVars) return fail(-1, "B_row is NULL but nB != nVars");
     if (!(dt > 0.0)) return fail(-1, "dt must be positive");
+    if (conv) {
+        if (!conv->params || !conv->delta_conv) return fail(-1, "convection: params and delta_conv are required");
+        if ((conv->rho == nullptr) != (conv->Wmx == nullptr))
+            return fail(-1, "convection: rho and Wmx are given together or not at all");
+    }
     hipStream_t st = c->stream;
     const size_t n = nVars * nPoints;
     double* state = s->state;
     double* start = s->work;
     double* sc_diff = s->work + n;
     double* sc_prod = s->work + 2 * n;
+    double* sc_conv = conv ? s->work + 3 * n : nullptr;
+    double* drhodt = conv && conv->rho ? s->work + 4 * n : nullptr;
     DiffArgs a{B, B_row, D, sc_diff, g->x, (int)nPoints, (int)nVars, g->alpha, g->left_bc,
                g->right_bc, g->y_inf, g->wall_const, g->wall_var};
     std::vector<double> h;
@@ -1075,18 +1085,45 @@ int emx_split_step_device(emx_context* c, size_t nVars, size_t nPoints, const em
     auto launched = [&](int e, const char* what) {
         return e ? fail(-10, std::string(what) + ": " + hipGetErrorString((hipError_t)e)) : 0;
     };
-    // SplitSolver::step: split constants from the previous derivatives
-    if (int rc = launched(launch_split_constants(n, balanced, s->ddt_conv, s->ddt_diff, s->ddt_prod,
-                                                 s->ddt_cross, sc_diff, sc_prod, st), "split constants"))
+    // SplitSolver::step: split constants from the previous derivatives; with convection
+    // stages, FlameSolver::prepareIntegrators' density derivative from the same derivatives
+    if (conv) {
+        if (int rc = launched(launch_split_constants_conv(n, balanced, s->ddt_conv, s->ddt_diff, s->ddt_prod,
+                                                          s->ddt_cross, sc_diff, sc_prod, sc_conv, st),
+                              "split constants"))
+            return rc;
+        if (drhodt)
+            if (int rc = launched(launch_density_derivative(c->dm, (int)nPoints, balanced, state, s->ddt_conv,
+                                                            s->ddt_diff, s->ddt_prod, s->ddt_cross, conv->rho,
+                                                            conv->Wmx, drhodt, st),
+                                  "density derivative"))
+                return rc;
+    } else if (int rc = launched(launch_split_constants(n, balanced, s->ddt_conv, s->ddt_diff, s->ddt_prod,
+                                                        s->ddt_cross, sc_diff, sc_prod, st),
+                                 "split constants")) {
         return rc;
-    // integrateSplitTerms (src/splitSolver.cpp:60-75); the convection stages change nothing
+    }
+    // integrateSplitTerms (src/splitSolver.cpp:60-75)
     if (int rc = launched(launch_fill(n, 0.0, s->delta_diff, s->delta_prod, st), "fill")) return rc;
+    if (conv)
+        if (int rc = launched(launch_fill(n, 0.0, conv->delta_conv, nullptr, st), "fill")) return rc;
     auto diffusion_stage = [&](double t0, double t1) -> int {
         if (int rc = launched(launch_copy(n, state, start, st), "copy")) return rc;
         if (int rc = diffusion_run(c, a, h.data(), t0, t1, state)) return rc;
         return launched(launch_accumulate(n, state, start, s->delta_diff, st), "accumulate");
     };
+    // FlameSolver::integrateConvectionTerms (flameSolver.cpp:736-751); empty without conv
+    auto convection_stage = [&](double t0, double t1, int stage) -> int {
+        if (!conv) return 0;
+        if (int rc = launched(launch_copy(n, state, start, st), "copy")) return rc;
+        int32_t* steps = conv->steps ? conv->steps + (size_t)stage * (c->hm.K + 1) : nullptr;
+        if (int rc = emx_convection_integrate_device(c, nVars, nPoints, conv->params, g->x, start, sc_conv,
+                                                     drhodt, t0, t1, state, steps))
+            return rc;
+        return launched(launch_accumulate(n, state, start, conv->delta_conv, st), "accumulate");
+    };
     if (int rc = diffusion_stage(t, t + 0.25 * dt)) return rc;
+    if (int rc = convection_stage(t, t + 0.5 * dt, 0)) return rc;
     if (int rc = diffusion_stage(t + 0.25 * dt, t + 0.5 * dt)) return rc;
     if (int rc = launched(launch_copy(n, state, start, st), "copy")) return rc;
     // the stage length as SplitSolver forms it: tStageEnd - tStageStart
@@ -1097,10 +1134,47 @@ int emx_split_step_device(emx_context* c, size_t nVars, size_t nPoints, const em
         return rc;
     if (int rc = launched(launch_accumulate(n, state, start, s->delta_prod, st), "accumulate")) return rc;
     if (int rc = diffusion_stage(t + 0.5 * dt, t + 0.75 * dt)) return rc;
+    if (int rc = convection_stage(t + 0.5 * dt, t + dt, 1)) return rc;
     if (int rc = diffusion_stage(t + 0.75 * dt, t + dt)) return rc;
     // calculateTimeDerivatives
+    if (conv)
+        if (int rc = launched(launch_conv_time_derivative(n, dt, conv->delta_conv, sc_conv, s->ddt_conv, st),
+                              "time derivatives"))
+            return rc;
     return launched(launch_time_derivatives(n, dt, s->delta_diff, s->delta_prod, sc_diff, sc_prod,
                                             s->ddt_diff, s->ddt_prod, st), "time derivatives");
+}
+
+int emx_split_step_device(emx_context* c, size_t nVars, size_t nPoints, const emx_split_arrays* s,
+                          const double* B, size_t nB, const int32_t* B_row, const double* D,
+                          const emx_grid* g, const double* qloss, const double* h_per_var, double t,
+                          double dt, int balanced, int32_t* status) {
+    return split_step(c, nVars, nPoints, s, nullptr, B, nB, B_row, D, g, qloss, h_per_var, t, dt, balanced,
+                      status);
+}
+
+int emx_split_step_conv_device(emx_context* c, size_t nVars, size_t nPoints, const emx_split_arrays* s,
+                               const emx_split_convection* conv, const double* B, size_t nB,
+                               const int32_t* B_row, const double* D, const emx_grid* g, const double* qloss,
+                               const double* h_per_var, double t, double dt, int balanced, int32_t* status) {
+    if (!conv) return fail(-1, "convection description is NULL");
+    return split_step(c, nVars, nPoints, s, conv, B, nB, B_row, D, g, qloss, h_per_var, t, dt, balanced,
+                      status);
+}
+
+int emx_density_derivative_device(emx_context* c, size_t nVars, size_t nPoints, const double* state,
+                                  const double* ddt_conv, const double* ddt_diff, const double* ddt_prod,
+                                  const double* ddt_cross, int balanced, const double* rho, const double* Wmx,
+                                  double* drhodt) {
+    if (int rc = check_ctx(c)) return rc;
+    if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
+    if (nPoints > 0x7fffffffULL) return fail(-1, "too many points");
+    if (nPoints == 0) return 0;
+    if (!state || !ddt_conv || !ddt_diff || !ddt_prod || !rho || !Wmx || !drhodt)
+        return fail(-1, "only ddt_cross may be NULL");
+    int e = launch_density_derivative(c->dm, (int)nPoints, balanced, state, ddt_conv, ddt_diff, ddt_prod,
+                                      ddt_cross, rho, Wmx, drhodt, c->stream);
+    return e ? fail(-10, std::string("density derivative launch: ") + hipGetErrorString((hipError_t)e)) : 0;
 }
 
 int emx_correct_mass_fractions_device(emx_context* c, size_t nVars, size_t nPoints, double* state) {
@@ -1249,8 +1323,14 @@ int emx_convection_integrate_device(emx_context* c, size_t nVars, size_t nPoints
         if (!(norm > 0.0) || !std::isfinite(norm)) return fail(-1, "Yleft must hold a positive, finite composition");
         for (int k = 0; k < K; k++) sum += std::max(cp->Yleft[k], 0.0) * (1.0 / norm) * (1.0 / c->hm.W[k]);
         Wleft = 1.0 / sum;
-        HIP_TRY(c->cv_yleft.ensure(K * sizeof(double)));
-        HIP_TRY(hipMemcpyAsync(c->cv_yleft.p, cp->Yleft, K * sizeof(double), hipMemcpyHostToDevice, st));
+        // uploaded when it changes, so that a call captured in a graph after a first call
+        // records no host copy
+        if (c->cv_yleft_host.size() != (size_t)K ||
+            !std::equal(c->cv_yleft_host.begin(), c->cv_yleft_host.end(), cp->Yleft)) {
+            HIP_TRY(c->cv_yleft.ensure(K * sizeof(double)));
+            HIP_TRY(hipMemcpyAsync(c->cv_yleft.p, cp->Yleft, K * sizeof(double), hipMemcpyHostToDevice, st));
+            c->cv_yleft_host.assign(cp->Yleft, cp->Yleft + K);
+        }
     }
     HIP_TRY(c->cv_work.ensure(convection_work_doubles(N, K) * sizeof(double)));
     HIP_TRY(c->cv_rec_t.ensure(kConvMaxRecords * sizeof(double)));

@@ -21,6 +21,15 @@ int launch_rates(const DevMech& m, const double* T, const double* Y, int nonorm,
 int launch_split_constants(size_t n, int balanced, double* ddt_conv, const double* ddt_diff,
                            const double* ddt_prod, const double* ddt_cross, double* sc_diff,
                            double* sc_prod, hipStream_t st);
+int launch_split_constants_conv(size_t n, int balanced, const double* ddt_conv, const double* ddt_diff,
+                                const double* ddt_prod, const double* ddt_cross, double* sc_diff,
+                                double* sc_prod, double* sc_conv, hipStream_t st);
+int launch_density_derivative(const DevMech& m, int N, int balanced, const double* state,
+                              const double* ddt_conv, const double* ddt_diff, const double* ddt_prod,
+                              const double* ddt_cross, const double* rho, const double* Wmx,
+                              double* drhodt, hipStream_t st);
+int launch_conv_time_derivative(size_t n, double dt, const double* delta_conv, const double* sc_conv,
+                                double* ddt_conv, hipStream_t st);
 int launch_fill(size_t n, double v, double* a, double* b, hipStream_t st);
 int launch_copy(size_t n, const double* src, double* dst, hipStream_t st);
 int launch_accumulate(size_t n, const double* state, const double* start, double* delta,

@@ -48,6 +48,62 @@ __global__ void split_constants_kernel(size_t n, int balanced, double* ddt_conv,
     }
 }
 
+// The same constants for a step with its convection stages: splitConstConv is written out
+// (the convection stages read it) and ddtConv stays the previous step's until the step's
+// own is formed from deltaConv (calculateTimeDerivatives); Strang: splitConstConv = 0.
+__global__ void split_constants_conv_kernel(size_t n, int balanced, const double* ddt_conv,
+                                            const double* ddt_diff, const double* ddt_prod,
+                                            const double* ddt_cross, double* sc_diff, double* sc_prod,
+                                            double* sc_conv) {
+    const size_t i = gid();
+    if (i >= n) return;
+    const double cross = ddt_cross ? ddt_cross[i] : 0.0;
+    if (balanced) {
+        const double conv = ddt_conv[i], diff = ddt_diff[i], prod = ddt_prod[i];
+        sc_diff[i] = __dmul_rn(0.25, __dsub_rn(__dadd_rn(__dadd_rn(prod, conv), cross), __dmul_rn(3.0, diff)));
+        sc_conv[i] = __dmul_rn(0.25, __dsub_rn(__dadd_rn(__dadd_rn(prod, diff), cross), __dmul_rn(3.0, conv)));
+        sc_prod[i] = __dmul_rn(0.5, __dsub_rn(__dadd_rn(__dadd_rn(conv, diff), cross), prod));
+    } else {
+        sc_diff[i] = cross;
+        sc_prod[i] = 0.0;
+        sc_conv[i] = 0.0;
+    }
+}
+
+// FlameSolver::prepareIntegrators (src/flameSolver.cpp:203-213): the density derivative of
+// the convection stages from the previous step's split time derivatives,
+// ddt = ddtConv + ddtDiff + ddtProd (+ ddtCross with balanced splitting), and the step's
+// starting state: drhodt = -rho (ddt_T / T + Wmx sum_k ddt_Yk / W_k) (the time derivative of
+// the ideal-gas density rho = P Wmx / (R T) at constant pressure).  One thread per point, the
+// species sum in species order.
+__global__ void density_derivative_kernel(int N, int nv, int balanced, const double* state,
+                                          const double* ddt_conv, const double* ddt_diff,
+                                          const double* ddt_prod, const double* ddt_cross,
+                                          const double* rho, const double* Wmx, const double* rW,
+                                          double* drhodt) {
+    const size_t j = gid();
+    if (j >= (size_t)N) return;
+    const size_t o = j * nv;
+    auto ddt = [&](int v) {
+        double d = __dadd_rn(__dadd_rn(ddt_conv[o + v], ddt_diff[o + v]), ddt_prod[o + v]);
+        if (balanced && ddt_cross) d = __dadd_rn(d, ddt_cross[o + v]);
+        return d;
+    };
+    double tmp = 0.0;
+    for (int k = 0; k < nv - 2; k++) tmp = __dadd_rn(tmp, __dmul_rn(rW[k], ddt(2 + k)));
+    const double T = state[o + 1];
+    drhodt[j] = __dmul_rn(-rho[j], __dadd_rn(__ddiv_rn(ddt(1), T), __dmul_rn(tmp, Wmx[j])));
+}
+
+// SplitSolver::calculateTimeDerivatives (src/splitSolver.cpp:29): ddtConv of a step with
+// its convection stages
+__global__ void conv_time_derivative_kernel(size_t n, double dt, const double* delta_conv,
+                                            const double* sc_conv, double* ddt_conv) {
+    const size_t i = gid();
+    if (i >= n) return;
+    ddt_conv[i] = __dsub_rn(__ddiv_rn(delta_conv[i], dt), sc_conv[i]);
+}
+
 // FlameSolver::prepareIntegrators (src/flameSolver.cpp:160-172): B of the momentum and
 // species systems is rho.inverse(), of the energy system (rho * cp).inverse(); B holds rho
 // in row 0 and cp in row 1 on entry
@@ -130,6 +186,31 @@ int launch_split_constants(size_t n, int balanced, double* ddt_conv, const doubl
                            double* sc_prod, hipStream_t st) {
     hipLaunchKernelGGL(split_constants_kernel, grid_for(n), dim3(kSplitThreads), 0, st, n, balanced,
                        ddt_conv, ddt_diff, ddt_prod, ddt_cross, sc_diff, sc_prod);
+    return (int)hipGetLastError();
+}
+
+int launch_split_constants_conv(size_t n, int balanced, const double* ddt_conv, const double* ddt_diff,
+                                const double* ddt_prod, const double* ddt_cross, double* sc_diff,
+                                double* sc_prod, double* sc_conv, hipStream_t st) {
+    hipLaunchKernelGGL(split_constants_conv_kernel, grid_for(n), dim3(kSplitThreads), 0, st, n, balanced,
+                       ddt_conv, ddt_diff, ddt_prod, ddt_cross, sc_diff, sc_prod, sc_conv);
+    return (int)hipGetLastError();
+}
+
+int launch_density_derivative(const DevMech& m, int N, int balanced, const double* state,
+                              const double* ddt_conv, const double* ddt_diff, const double* ddt_prod,
+                              const double* ddt_cross, const double* rho, const double* Wmx,
+                              double* drhodt, hipStream_t st) {
+    hipLaunchKernelGGL(density_derivative_kernel, grid_for((size_t)N), dim3(kSplitThreads), 0, st, N,
+                       m.nVars, balanced, state, ddt_conv, ddt_diff, ddt_prod, ddt_cross, rho, Wmx, m.rW,
+                       drhodt);
+    return (int)hipGetLastError();
+}
+
+int launch_conv_time_derivative(size_t n, double dt, const double* delta_conv, const double* sc_conv,
+                                double* ddt_conv, hipStream_t st) {
+    hipLaunchKernelGGL(conv_time_derivative_kernel, grid_for(n), dim3(kSplitThreads), 0, st, n, dt,
+                       delta_conv, sc_conv, ddt_conv);
     return (int)hipGetLastError();
 }
 

@@ -272,7 +272,7 @@ int emx_diffusion_integrate_device(emx_context* ctx, size_t nVars, size_t nPoint
  * D1/4 with its delta bookkeeping (:60-121) and calculateTimeDerivatives (:27-32), with
  * FlameSolver's production and diffusion stages (flameSolver.cpp:702-734, 754-854) run
  * on the GPU and the state resident in device memory throughout.  The convection stages
- * are empty (convection is outside this library).  All matrices are nVars x nPoints,
+ * are empty here (emx_split_step_conv_device runs them).  All matrices are nVars x nPoints,
  * point-contiguous, device pointers owned by the caller:
  *   state             in/out
  *   ddt_conv/diff/prod  in: the previous step's time derivatives (zeros initially);
@@ -342,6 +342,47 @@ int emx_convection_integrate(emx_context* ctx, size_t nVars, size_t nPoints,
                              const emx_convection_params* params, const double* x,
                              const double* split_const, const double* drhodt, double t0, double t1,
                              double* state, int32_t* steps);
+
+/* The device-resident split step with its convection stages: emx_split_step_device plus
+ * FlameSolver::integrateConvectionTerms (src/flameSolver.cpp:736-751) in the two stages
+ * C1/2 of integrateSplitTerms (src/splitSolver.cpp:68, :72) through
+ * emx_convection_integrate_device, the balanced splitting's convection constants
+ * splitConstConv (:40; zeros for Strang), deltaConv and ddtConv (:29), and the density
+ * derivative of FlameSolver::prepareIntegrators (src/flameSolver.cpp:203-213) as
+ * emx_density_derivative_device forms it.  arrays as emx_split_step_device, with work of
+ * 4 x nVars x nPoints + nPoints doubles.  conv:
+ *   params      the convection stage's parameters (Yleft is uploaded when it changes, so a
+ *               step captured in a graph after one call records no host copy)
+ *   rho, Wmx    nPoints, device: the density and mean molecular weight at the step's start
+ *               (updateChemicalProperties); both NULL: drhodt = 0
+ *   delta_conv  nVars x nPoints, device, out: the step's convection increment
+ *   steps       device, 2 x (1 + nSpecies), nullable: each convection stage's accepted
+ *               steps as emx_convection_integrate_device reports them
+ * 3 to 4096 grid points (the convection stage's range). */
+typedef struct emx_split_convection {
+    const emx_convection_params* params;
+    const double* rho;
+    const double* Wmx;
+    double* delta_conv;
+    int32_t* steps;
+} emx_split_convection;
+
+int emx_split_step_conv_device(emx_context* ctx, size_t nVars, size_t nPoints,
+                               const emx_split_arrays* arrays, const emx_split_convection* conv,
+                               const double* B, size_t nB, const int32_t* B_row, const double* D,
+                               const emx_grid* grid, const double* qloss, const double* h_per_var,
+                               double t, double dt, int balanced, int32_t* status);
+
+/* FlameSolver::prepareIntegrators' density derivative (src/flameSolver.cpp:203-213) for the
+ * convection stage: with ddt = ddt_conv + ddt_diff + ddt_prod (+ ddt_cross when balanced,
+ * nullable = zeros), drhodt = -rho (ddt_T / T + Wmx sum_k ddt_Yk / W_k) at each point of the
+ * device state matrix.  Device pointers: the ddt matrices nVars x nPoints, rho, Wmx and
+ * drhodt nPoints.  Asynchronous on the context stream. */
+int emx_density_derivative_device(emx_context* ctx, size_t nVars, size_t nPoints,
+                                  const double* state, const double* ddt_conv,
+                                  const double* ddt_diff, const double* ddt_prod,
+                                  const double* ddt_cross, int balanced, const double* rho,
+                                  const double* Wmx, double* drhodt);
 
 /* FlameSolver::correctMassFractions (src/flameSolver.cpp:942-949): per point, negative
  * mass fractions clipped to zero and the rest scaled to sum to one (Cantera

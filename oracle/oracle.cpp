@@ -1409,6 +1409,31 @@ int orc_thermo_properties(void* mech, double pressure, int nPoints, const double
 }
 
 // ---------------------------------------------------------------------------
+// Density derivative of the convection stage, FlameSolver::prepareIntegrators
+// (src/flameSolver.cpp:203-213): ddt = ddtConv + ddtDiff + ddtProd (+ ddtCross with
+// balanced splitting); tmp = W.inverse()^T ddt(species rows), summed here in species order;
+// drhodt = -rho * (ddt(kEnergy) / T + tmp * Wmx).
+int orc_density_derivative(void* mech, int nPoints, const double* state, const double* ddt_conv,
+                           const double* ddt_diff, const double* ddt_prod, const double* ddt_cross,
+                           int balanced, const double* rho, const double* Wmx, double* drhodt) {
+    Mech& m = *static_cast<Mech*>(mech);
+    const int nv = m.K + 2;
+    for (int j = 0; j < nPoints; j++) {
+        const size_t o = (size_t)j * nv;
+        auto ddt = [&](int v) {
+            double d = ddt_conv[o + v] + ddt_diff[o + v];
+            d = d + ddt_prod[o + v];
+            if (balanced && ddt_cross) d = d + ddt_cross[o + v];
+            return d;
+        };
+        double tmp = 0.0;
+        for (int k = 0; k < m.K; k++) tmp = tmp + (1.0 / m.W[k]) * ddt(2 + k);
+        drhodt[j] = -rho[j] * (ddt(1) / state[o + 1] + tmp * Wmx[j]);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Transport properties: the transport block of FlameSolver::updateChemicalProperties
 // (src/flameSolver.cpp:678-692) with Ember's default model, ApproxMixTransport
 // (src/chemistry0d.cpp:12-261) over Cantera's MixTransport: mu (ApproxMixTransport::

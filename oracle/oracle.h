@@ -76,6 +76,14 @@ int orc_thermo_properties(void* mech, double pressure, int nPoints, const double
                           double* rho, double* Wmx, double* cp, double* cpSpec, double* hk,
                           int nthreads);
 
+/* FlameSolver::prepareIntegrators' density derivative (src/flameSolver.cpp:203-213): with
+ * ddt = ddt_conv + ddt_diff + ddt_prod (+ ddt_cross when balanced; nullable = zeros), all
+ * point-major nPoints x (nSpecies + 2), drhodt[j] = -rho[j] (ddt_T / T + Wmx[j] sum_k
+ * ddt_Yk / W_k) at the point's temperature in state. */
+int orc_density_derivative(void* mech, int nPoints, const double* state, const double* ddt_conv,
+                           const double* ddt_diff, const double* ddt_prod, const double* ddt_cross,
+                           int balanced, const double* rho, const double* Wmx, double* drhodt);
+
 /* Transport properties of FlameSolver::updateChemicalProperties with Ember's default
  * transport model ("Approx": ApproxMixTransport with mole-fraction threshold `threshold`)
  * for nPoints points of a point-major state: mu [Pa s], lambda [W/m/K] per point, rhoD

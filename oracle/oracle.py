@@ -81,6 +81,8 @@ def lib():
                                             _dp, _dp, C.c_int]
         L.orc_source_integrate.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, _dp, _dp, _dp,
                                            C.c_double, _dp, _ip, _ip, _ip, _dp, C.c_int]
+        L.orc_density_derivative.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp,
+                                             _dp, _dp]
         L.orc_transport_properties.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                                _dp, _dp, C.c_int]
         L.orc_convection_integrate.argtypes = [C.c_void_p, C.POINTER(ConvParams), C.c_int, _dp, _dp, _dp,
@@ -184,6 +186,22 @@ class Mech:
                                          _p(out["hk"]), int(nthreads))
         if rc:
             raise RuntimeError(f"orc_thermo_properties failed: {rc}")
+        return out
+
+    def density_derivative(self, state, ddt_conv, ddt_diff, ddt_prod, rho, Wmx, ddt_cross=None,
+                           balanced=True):
+        """drhodt (N) of FlameSolver::prepareIntegrators from point-major (N, K + 2) arrays."""
+        a = [np.ascontiguousarray(np.atleast_2d(v), dtype=np.float64)
+             for v in (state, ddt_conv, ddt_diff, ddt_prod)]
+        n = a[0].shape[0]
+        cross = None if ddt_cross is None else np.ascontiguousarray(ddt_cross, dtype=np.float64)
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        Wmx = np.ascontiguousarray(Wmx, dtype=np.float64)
+        out = np.zeros(n)
+        rc = lib().orc_density_derivative(self.h, n, *(_p(v) for v in a), _p(cross), 1 if balanced else 0,
+                                          _p(rho), _p(Wmx), _p(out))
+        if rc:
+            raise RuntimeError(f"orc_density_derivative failed: {rc}")
         return out
 
     def rhs(self, y, opts=None, split_const=None, qloss=0.0):

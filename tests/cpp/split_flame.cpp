@@ -9,8 +9,8 @@
 // (leftBC: Ember's BoundaryCondition value for the left end; with leftBC = 3 (WallFlux)
 // the energy equation carries yInf = Tinf and wallConst = Kwall, flameSolver.cpp:79-82.
 // rVzero, strain, rhou: the convection stages run too, planar, fixedLeft continuity, a
-// constant strain rate, drhodt = 0; the device-resident step, which has no convection
-// stages, is then left out and its trajectory written as zeros)
+// constant strain rate, drhodt = 0; the device-resident step is then
+// emx_split_step_conv_device with the same convection parameters and drhodt = 0)
 // input.bin: int64 nVars, nPoints, nB; doubles x[N], state[N*nV], B[nB*N], D[N*nV];
 //            int32 B_row[nV]
 // output.bin: doubles gpu[nsteps][N*nV], oracle[nsteps][N*nV], device[nsteps][N*nV];
@@ -128,9 +128,9 @@ int main(int argc, char** argv) {
     cpu.B = B;
     cpu.D = D;
     cpu.B_row = B_row;
+    emx_convection_params ep{};
     if (conv) {
         const double rVzero = atof(argv[9]), strain = atof(argv[10]), rhou = atof(argv[11]);
-        emx_convection_params ep{};
         ep.alpha = 0;
         ep.beta = 1;
         ep.left_bc = leftBC;
@@ -185,14 +185,15 @@ int main(int argc, char** argv) {
     // device-resident split step on the same inputs
     std::vector<double> traj_d;
     int dfail = 0;
-    if (conv) {
-        traj_d.assign((size_t)nsteps * N * nV, 0.0);
-    } else {
+    {
         const size_t n = N * nV;
         double *d_state, *d_ddt, *d_delta, *d_work, *d_B, *d_D, *d_x;
         int32_t *d_Brow, *d_status;
+        // with convection: delta_conv after delta_diff and delta_prod, and work for
+        // splitConstConv and drhodt
         bool ok = hipMalloc(&d_state, n * 8) == hipSuccess && hipMalloc(&d_ddt, 3 * n * 8) == hipSuccess &&
-                  hipMalloc(&d_delta, 2 * n * 8) == hipSuccess && hipMalloc(&d_work, 3 * n * 8) == hipSuccess &&
+                  hipMalloc(&d_delta, 3 * n * 8) == hipSuccess &&
+                  hipMalloc(&d_work, (4 * n + N) * 8) == hipSuccess &&
                   hipMalloc(&d_B, nB * N * 8) == hipSuccess && hipMalloc(&d_D, n * 8) == hipSuccess &&
                   hipMalloc(&d_x, N * 8) == hipSuccess && hipMalloc(&d_Brow, nV * 4) == hipSuccess &&
                   hipMalloc(&d_status, N * 4) == hipSuccess;
@@ -210,8 +211,14 @@ int main(int argc, char** argv) {
         std::vector<int32_t> hst(N);
         t = 0.0;
         for (int s = 0; s < nsteps; s++) {
-            ember_amd::check(emx_split_step_device(ctx.get(), nV, N, &sa, d_B, nB, d_Brow, d_D, &g,
-                                                   nullptr, nullptr, t, dt, 1, d_status));
+            if (conv) {
+                emx_split_convection sc{&ep, nullptr, nullptr, d_delta + 2 * n, nullptr};
+                ember_amd::check(emx_split_step_conv_device(ctx.get(), nV, N, &sa, &sc, d_B, nB, d_Brow, d_D,
+                                                            &g, nullptr, nullptr, t, dt, 1, d_status));
+            } else {
+                ember_amd::check(emx_split_step_device(ctx.get(), nV, N, &sa, d_B, nB, d_Brow, d_D, &g,
+                                                       nullptr, nullptr, t, dt, 1, d_status));
+            }
             if (hipMemcpy(host.data(), d_state, n * 8, hipMemcpyDeviceToHost) != hipSuccess ||
                 hipMemcpy(hst.data(), d_status, N * 4, hipMemcpyDeviceToHost) != hipSuccess)
                 return 7;

@@ -4,7 +4,9 @@ mirror over the GPU library against the same driver over the CPU oracle, T and Y
 compared after every step (rtol 1e-6); and the device-resident split step
 (emx_split_step_device) against the host-driven one, bit for bit.  With convection: the
 full Strang sequence D C D P D C D, the convection stages through
-emx_convection_integrate (GPU) and orc_convection_integrate (oracle)."""
+emx_convection_integrate (GPU) and orc_convection_integrate (oracle), and the
+device-resident step with its convection stages (emx_split_step_conv_device) against the
+host-driven one, bit for bit."""
 import os
 import subprocess
 
@@ -51,11 +53,11 @@ def test_split_flame_100_steps(gpu, tmp_path, wall, conv):
     fails = np.fromfile(out, dtype=np.int32, offset=3 * nsteps * N * nV * 8)
     assert fails.tolist() == [0, 0, 0]
     g, c, d = data
-    if not conv:
-        # the device-resident step runs the same kernels with the same arithmetic
-        print(f"[split] device-resident vs host-driven max |diff| {np.abs(d - g).max():.3e}")
-        np.testing.assert_array_equal(d, g)
-    else:
+    # the device-resident step (with convection: emx_split_step_conv_device) runs the same
+    # kernels with the same arithmetic
+    print(f"[split] device-resident vs host-driven max |diff| {np.abs(d - g).max():.3e}")
+    np.testing.assert_array_equal(d, g)
+    if conv:
         # the strain term drove U up from 0 and the inflow advected the profile
         assert g[-1, :, 0].max() > 1.0
         dU = np.abs(g[:, :, 0] - c[:, :, 0]) / (np.abs(c[:, :, 0]) + 1e-3)
