@@ -31,6 +31,19 @@ FLAGS = [
 ]
 
 
+# Per-file machine-scheduler strategy (LLVM AMDGPU's iterative schedulers), chosen by timing
+# the kernels of each file on MI355X with the default and the iterative strategies: the
+# QSS stage, the diffusion chunks and transport run faster with iterative-ilp, the BDF and
+# the adjoint with iterative-maxocc.  Instruction order only: the arithmetic is unchanged.
+FILE_FLAGS = {
+    "qss_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+    "diffusion_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+    "transport_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+    "bdf_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-maxocc"],
+    "adjoint_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-maxocc"],
+}
+
+
 def _sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
 
@@ -58,7 +71,7 @@ def build(force: bool = False, jobs: int = 8) -> str:
             todo.append((src, obj))
     procs = []
     for src, obj in todo:
-        cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj + ".tmp"]
+        cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj + ".tmp"]
         if src.endswith(".cpp"):
             cmd = [HIPCC, *FLAGS, "-x", "hip", "-c", src, "-o", obj + ".tmp"]
         procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True), obj, cmd))
