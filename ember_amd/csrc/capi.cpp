@@ -265,11 +265,19 @@ int upload_mech(emx_context* c) {
     pk.add(L.chunk_sp, &d.chunk_sp);
     pk.add(aux, &d.aux);
     pk.add(auxd, &d.auxd);
-    d.tr_visc = d.tr_cond = d.tr_diff = nullptr;
+    d.tr_visc = d.tr_cond = d.tr_diff = d.tr_wilke = nullptr;
+    std::vector<double> wilke;
     if (m.has_transport()) {
         pk.add(m.visc, &d.tr_visc);
         pk.add(m.cond, &d.tr_cond);
         pk.add(m.diff, &d.tr_diff);
+        // Wilke's weight factor of the pair (k, j), sqrt(W_j) / sqrt(8 (W_j + W_k)), at
+        // [j * K + k] (the transport kernel's Phi_kj = (1 + a_k / a_j)^2 times this)
+        wilke.resize((size_t)m.K * m.K);
+        for (int j = 0; j < m.K; j++)
+            for (int k = 0; k < m.K; k++)
+                wilke[(size_t)j * m.K + k] = std::sqrt(m.W[j]) / std::sqrt(8.0 * (m.W[j] + m.W[k]));
+        pk.add(wilke, &d.tr_wilke);
     }
     pk.add(kf_c, &d.kf_c);
     pk.add(kf_s, &d.kf_s);

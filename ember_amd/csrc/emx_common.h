@@ -127,6 +127,7 @@ struct DevMech {
     const double* tr_visc;
     const double* tr_cond;
     const double* tr_diff;
+    const double* tr_wilke;  // [K * K]: sqrt(W_j) / sqrt(8 (W_j + W_k)) at j * K + k
     // interp tables [interpN][nRounds*64] over [interpTmin, interpTmax]
     int interpN;
     double interpTmin, interpTmax;

@@ -30,22 +30,27 @@ __device__ __forceinline__ double dot5(const double (&p)[5], const double* c) {
     return p[0] * c[0] + p[1] * c[1] + p[2] * c[2] + p[3] * c[3] + p[4] * c[4];
 }
 
-__device__ __forceinline__ int pair_slot(int j, int k, int K) {
-    if (k < j) {
-        const int t = j;
-        j = k;
-        k = t;
-    }
-    return K * j - (j - 1) * j / 2 + k - j;
-}
+// slot of the pair (j, k), j <= k, in the row-by-row upper triangle of the binary fits:
+// K j - (j - 1) j / 2 + k - j.  For a fixed k and i = 0, 1, ... the slot of the pair (i, k)
+// starts at k and grows by K - 1 - i while i < k, by 1 from there on (the diagonal slot
+// included).
 
-// Wilke's Phi_kj for j <= k (ApproxMixTransport::updateViscosity_T's direct form):
-// (1 + (sqrt(mu_k) / sqrt(mu_j)) (W_j / W_k)^(1/4))^2 / (sqrt(8) sqrt(1 + W_k / W_j))
-__device__ __forceinline__ double phi_direct(double sqk, double sqj, double Wk, double Wj) {
-    const double wratjk = sqrt(sqrt(Wj / Wk));
-    const double wratkj1 = sqrt(1.0 + Wk / Wj);
-    const double f = 1.0 + (sqk / sqj) * wratjk;
-    return f * f / (sqrt(8.0) * wratkj1);
+// Wilke's Phi_kj (ApproxMixTransport::updateViscosity_T: the direct form
+// (1 + (sqrt(mu_k) / sqrt(mu_j)) (W_j / W_k)^(1/4))^2 / (sqrt(8) sqrt(1 + W_k / W_j)) for
+// j <= k, Phi_jk / (mu_j / mu_k W_k / W_j) for j > k, the same quantity) is evaluated as
+// (1 + a_k / a_j)^2 g_jk with a_k = sqrt(mu_k) W_k^(-1/4) per species and point and the
+// mechanism constant g_jk = sqrt(W_j) / sqrt(8 (W_j + W_k)) (DevMech::tr_wilke): the same
+// quantity again, with two multiplications per pair instead of square roots and divisions.
+
+// 1 / x by Newton-Raphson from the hardware estimate (v_rcp_f64, relative error below
+// 2^-22): each step r <- r + r (1 - x r) squares the relative error, so two steps reach the
+// rounding level.  For the positive, finite binary diffusion coefficients.
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
 }
 
 // A workgroup of kWavesPerBlock waves takes kWavesPerBlock * kPointsPerWave consecutive
@@ -78,15 +83,17 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) transport_props_kernel(
     __syncthreads();
     const int lane = lane_id();
     const int wave = wave_in_block();
-    double* sX = tW + K + wave * 192;  // per wave: X_k, sqrt(mu_k), mu_k of the current point
-    double* sSq = sX + 64;
-    double* sVisc = sSq + 64;
+    double* sX = tW + K + wave * 192;  // per wave: X_k, 1 / a_k and X_k W_k of the current point
+    double* sIa = sX + 64;
+    double* sXW = sIa + 64;
     const bool sp = lane < K;
     const double Wk = sp ? tW[lane] : 1.0;
+    const double rW4 = sp ? 1.0 / sqrt(sqrt(Wk)) : 0.0;  // W_k^(-1/4)
+    const double* gk = m.tr_wilke + (sp ? lane : 0);   // g_jk at gk[j * K]
     for (int it = 0; it < kPointsPerWave; it++) {
         const int p = (blockIdx.x * kPointsPerWave + it) * kWavesPerBlock + wave;
         if (p >= n) break;  // uniform over the wave
-        wave_lds_sync();  // the previous point's reads of sX, sSq, sVisc are done
+        wave_lds_sync();  // the previous point's reads of sX, sIa, sXW are done
         const double* s = state + (size_t)p * nv;
         const double T = s[1];
         const double y = sp ? s[2 + lane] : 0.0;
@@ -109,29 +116,25 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) transport_props_kernel(
         }
         // MixTransport::thermalConductivity
         const double lam = 0.5 * (wave_sum(sp ? X * cond : 0.0) + 1.0 / wave_sum(sp ? X / cond : 0.0));
+        const double ak = sqv * rW4;
         sX[lane] = X;  // species lanes; 0 elsewhere
-        sSq[lane] = sqv;
-        sVisc[lane] = visc;
+        sIa[lane] = sp ? 1.0 / ak : 0.0;
+        sXW[lane] = X * Wk;
         wave_lds_sync();
-        // ApproxMixTransport::viscosity over the major species
+        // ApproxMixTransport::viscosity over the major species (the loop's skips are uniform:
+        // sX[j] is the same for every lane)
         double contrib = 0.0;
         if (major) {
             double sum = 0.0;
+            double gj = gk[0];
             for (int j = 0; j < K; j++) {
+                const double gn = j + 1 < K ? gk[(j + 1) * K] : 0.0;
                 const double Xj = sX[j];
-                if (!(Xj >= threshold)) continue;
-                const double Wj = tW[j];
-                double phi;
-                if (j <= lane) {
-                    phi = phi_direct(sqv, sSq[j], Wk, Wj);
-                } else {
-                    // the pair's Phi_jk, then Phi_kj = Phi_jk / (vratiokj wratiojk) with
-                    // vratiokj = mu_j / mu_k and wratiojk = W_k / W_j (j > k)
-                    const double vratio = sVisc[j] / visc;
-                    const double wratio = Wk / Wj;
-                    phi = phi_direct(sSq[j], sqv, Wj, Wk) / (vratio * wratio);
+                if (Xj >= threshold) {
+                    const double t = ak * sIa[j] + 1.0;
+                    sum += (Xj * gj) * (t * t);
                 }
-                sum += Xj * phi;
+                gj = gn;
             }
             contrib = X * visc / sum;
         }
@@ -142,13 +145,16 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) transport_props_kernel(
         if (sp) {
             const double tts = T * sqrt_t;
             double s1 = 0.0, s2 = 0.0;
+            int slot = lane;  // the pair (0, lane)
             for (int i = 0; i < K; i++) {
+                const int here = slot;
+                slot += i < lane ? K - 1 - i : 1;
                 if (i == lane) continue;
                 const double Xi = sX[i];
                 if (!major && !(Xi >= threshold)) continue;
-                const double rb = 1.0 / (tts * dot5(pt, tdiff + 5 * pair_slot(lane, i, K)));
+                const double rb = rcp_nr(tts * dot5(pt, tdiff + 5 * here));
                 s1 += Xi * rb;
-                s2 += Xi * tW[i] * rb;
+                s2 += sXW[i] * rb;
             }
             s1 *= m.P;
             s2 *= m.P * X / (mmw - Wk * X);
