@@ -13,7 +13,9 @@
 // method: the last point of every chunk is a separator; a chunk eliminates its
 // interior towards both separators (downward and upward sweeps), the separators
 // form a reduced tridiagonal system, which is solved by the same scheme one level
-// up, and finally every chunk solves its interior against known separators.
+// up, and finally every chunk solves its interior against known separators.  The fused
+// stage path (below) takes the second level inside the chunk kernel's workgroups and
+// solves the remaining system of one row per workgroup with one wave per variable.
 #include <hip/hip_runtime.h>
 
 #include "emx_common.h"
@@ -373,14 +375,16 @@ __global__ void hmin_reduce_kernel(const double* cmin, int C, int nV, double* ou
 }
 
 // ---------------------------------------------------------------------------
-// Fused stage path (C1 = ceil(N / P1) <= kPcrMax): per solve one PCR launch for the
-// separators and one chunk launch that back-substitutes this solve and eliminates the
-// chunk for the next solve with the matrix rows still in registers.
+// Fused stage path (C1 = ceil(N / P1) <= kPcrMax): one setup launch per stage (grid
+// metrics, edge-chunk rows, step sizes), then per solve one separator launch (pcr_kernel)
+// and one chunk launch that back-substitutes this solve and eliminates the chunks for the
+// next solve with the matrix rows still in registers; a workgroup's chunks form one
+// workgroup chunk (two-level partition), so the reduced system has G = ceil(C1 / 4) rows.
 //
-// The reduced system of this path is variable-major: plane k of the chunk summaries
-// (own: alpha, beta, rho; nxt: dbeta, gamma, drho) holds element (v, c) at
-// (k * nV + v) * C + c, and the separators X at v * C + c, so the PCR workgroup of one
-// variable reads and writes contiguous rows.
+// The reduced system of this path is variable-major: plane k of the workgroup summaries
+// (own: alpha, beta, rho; nxt: dbeta, gamma, drho) holds element (v, g) at
+// (k * nV + v) * G + g, and the separators X at v * G + g, so the separator workgroup of
+// one variable reads and writes contiguous rows.
 // ---------------------------------------------------------------------------
 constexpr int kPcrMax = 4096;
 // buffer resource word 3 of gfx9 (raw 32-bit records, no swizzle)
