@@ -1053,6 +1053,20 @@ static int diffusion_steps(emx_context* c, const DiffArgs& a, std::vector<double
     return 0;
 }
 
+// The convection stage's point range, continuity condition, geometry and boundary
+// conditions (emx_convection_integrate_device, and emx_split_step_conv_device before its
+// first stage).
+static int check_conv_params(size_t nPoints, const emx_convection_params* cp) {
+    if (nPoints < 3 || nPoints > (size_t)kConvMaxPoints)
+        return fail(-1, "the convection stage takes 3 to 4096 grid points");
+    if (cp->continuity_bc != 0)
+        return fail(-1, "only the fixedLeft continuity boundary condition is supported");
+    if (cp->alpha < 0 || cp->alpha > 1 || (cp->beta != 1 && cp->beta != 2) || cp->left_bc < 0 ||
+        cp->left_bc > 3 || cp->right_bc < 0 || cp->right_bc > 1)
+        return fail(-1, "invalid grid geometry or boundary condition");
+    return 0;
+}
+
 // SplitSolver::step on the device (emx_split_step_device, emx_split_step_conv_device);
 // conv NULL: empty convection stages.
 static int split_step(emx_context* c, size_t nVars, size_t nPoints, const emx_split_arrays* s,
@@ -1070,6 +1084,8 @@ static int split_step(emx_context* c, size_t nVars, size_t nPoints, const emx_sp
         if (!conv->params || !conv->delta_conv) return fail(-1, "convection: params and delta_conv are required");
         if ((conv->rho == nullptr) != (conv->Wmx == nullptr))
             return fail(-1, "convection: rho and Wmx are given together or not at all");
+        // refused before any stage runs, so that a refusal leaves the state untouched
+        if (int rc = check_conv_params(nPoints, conv->params)) return rc;
     }
     hipStream_t st = c->stream;
     const size_t n = nVars * nPoints;
@@ -1313,13 +1329,7 @@ int emx_convection_integrate_device(emx_context* c, size_t nVars, size_t nPoints
     if (int rc = check_ctx(c)) return rc;
     if ((int)nVars != c->hm.K + 2) return fail(-1, "nVars must equal nSpecies + 2");
     if (!cp || !x || !state_in || !state_out) return fail(-1, "params, x or a state pointer is NULL");
-    if (nPoints < 3 || nPoints > (size_t)kConvMaxPoints)
-        return fail(-1, "the convection stage takes 3 to 4096 grid points");
-    if (cp->continuity_bc != 0)
-        return fail(-1, "only the fixedLeft continuity boundary condition is supported");
-    if (cp->alpha < 0 || cp->alpha > 1 || (cp->beta != 1 && cp->beta != 2) || cp->left_bc < 0 ||
-        cp->left_bc > 3 || cp->right_bc < 0 || cp->right_bc > 1)
-        return fail(-1, "invalid grid geometry or boundary condition");
+    if (int rc = check_conv_params(nPoints, cp)) return rc;
     if (!(t1 > t0)) return fail(-1, "t1 must exceed t0");
     const int N = (int)nPoints, K = c->hm.K;
     hipStream_t st = c->stream;

@@ -102,3 +102,26 @@ def test_split_conv_step_graph_replays(gpu):
             torch.cuda.synchronize()
             assert torch.equal(tmp.state, eager[0])
     assert (eager[-1] != t(case.state)).any()
+
+
+def test_split_conv_step_refusals_leave_the_state(gpu):
+    import torch
+    print("[split-conv] refusals", flush=True)
+    case = W.flame_profile(40, seed=1, name="h2o2")
+    n, nV = case.state.shape
+    ctx = capi.Context("h2o2", global_timestep=DT)
+    t = lambda v: torch.from_numpy(np.ascontiguousarray(v)).to(gpu)
+    B, D, x, Brow = t(case.B), t(case.D), t(case.x), t(case.B_row.astype(np.int32))
+    ss = capi.SplitState(t(case.state), convection=True)
+    h = np.full(nV, DT / 4)
+    bad = ctx.convection_params(continuity_bc=1)  # only fixedLeft is supported
+    with pytest.raises(capi.EmberAmdError, match="continuity"):
+        ctx.split_step_conv_device(ss, bad, B, D, x, 0.0, DT, B_row=Brow, h_per_var=h)
+    rho = torch.ones(n, dtype=torch.float64, device=gpu)
+    with pytest.raises(capi.EmberAmdError, match="rho and Wmx"):
+        ctx.split_step_conv_device(ss, ctx.convection_params(), B, D, x, 0.0, DT, rho=rho, B_row=Brow,
+                                   h_per_var=h)
+    torch.cuda.synchronize()
+    # refused before any stage ran: the state and the bookkeeping are as they were
+    assert torch.equal(ss.state, t(case.state))
+    assert not ss.delta_conv.any() and not ss.ddt_diff.any() and not ss.delta_diff.any()
